@@ -1,0 +1,198 @@
+"""Benchmark of the chi2 + gradient evaluation on one MI355X.
+
+One step = one chi2 + gradient evaluation of the whole workload with every
+input already resident in device memory (patch pixels and metadata, the
+proposal); outputs stay on the device.  Prints one JSON line.
+
+Workloads:
+  multi    (default) one patch of 8 bands x 2 exposures x 128x128 px, 15
+           sources, 9 radii x 4 PSF Gaussians = 1.416e8 pixel-Gaussian pairs
+  single   1 band x 1 exposure x 64x64 px, 1 source, 9 x 3 Gaussians
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload multi]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "pixel·Gaussian evals/s (chi2+grad), batched vs plain, % of FP32 VALU peak"
+FLOP_PER_EVAL = 117          # FP32 flop per pixel-Gaussian pair (FMA = 2)
+PEAK_FP32 = 157.3e12         # MI355X FP32 vector peak, FLOP/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="multi", choices=["multi", "single"])
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--config", type=int, default=None, help="kernel config (default: tuned)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    return ap.parse_args()
+
+
+def build_workload(name, seed):
+    from forcepho_amd import scenes
+    from forcepho_amd.patches import SimplePatch
+    spec = scenes.spec_multi(seed) if name == "multi" else scenes.spec_single(seed)
+    patch, scene, theta = scenes.build_patch(spec, SimplePatch, scenes.render_on_device)
+    desc = ("multi-band plain: 8 bands x 2 exposures x 128x128 px, 15 sources, "
+            "9 radii x 4 PSF Gaussians" if name == "multi" else
+            "single plain: 1 band x 1 exposure x 64x64 px, 1 source, 9 radii x 3 PSF Gaussians")
+    return patch, scene, desc
+
+
+def time_config(prop, patch, proposal_dev, n):
+    import torch
+    s = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(n)]
+    for a, b in ev:
+        a.record(s)
+        prop.launch(proposal_dev, patch)
+        b.record(s)
+    torch.cuda.synchronize()
+    return np.array([a.elapsed_time(b) for a, b in ev]) * 1e-3
+
+
+def valu_probe():
+    """Measured FP32 FMA and v_exp_f32 issue rates of the device (ops/s)."""
+    import torch
+    from forcepho_amd._lib import lib, check
+    import ctypes
+    n_threads = 256 * 256 * 8
+    out = torch.empty(n_threads, dtype=torch.float32, device="cuda")
+    res = {}
+    for kind, name, iters in ((0, "fma_per_s", 4096), (1, "exp_per_s", 1024)):
+        ops = ctypes.c_double()
+        s = torch.cuda.current_stream()
+        check(lib().fpx_probe_valu(kind, iters, out.data_ptr(), n_threads, ctypes.byref(ops),
+                                   s.cuda_stream), "probe")
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        check(lib().fpx_probe_valu(kind, iters, out.data_ptr(), n_threads, ctypes.byref(ops),
+                                   s.cuda_stream), "probe")
+        b.record(s)
+        torch.cuda.synchronize()
+        res[name] = ops.value / (a.elapsed_time(b) * 1e-3)
+    return res
+
+
+def cpu_baseline(patch, proposal, n_evals, budget_s=12.0):
+    """The reference CPU kernel (compiled from its sources into oracle/_ref),
+    or the C restatement if that is absent, single-threaded, on whole
+    evaluations of the same workload (one call per band, as forcepho's
+    CPUProposer does) until about budget_s of CPU time."""
+    import oracle
+    mod = oracle.reference_module()
+    kind = "reference" if mod is not None else "port"
+    fn = ((lambda: oracle.evaluate_reference(patch, proposal, module=mod)) if mod is not None
+          else (lambda: oracle.evaluate_cpu(patch, proposal)))
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or el / reps * (reps + 1) > 2.5 * budget_s:
+            break
+    return {"value": n_evals * reps / el, "unit": "evals/s", "cores": 1, "kind": kind,
+            "sample": f"{reps} full evaluation(s) of the benchmark workload, "
+                      f"{el:.1f} s on one host core"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); "
+              "multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from forcepho_amd._lib import lib
+    from forcepho_amd.proposal import HIPProposer
+
+    torch.cuda.set_device(0)
+    patch, scene, desc = build_workload(args.workload, args.seed)
+    patch.return_residual = False
+    patch.send_to_device()
+    proposal = scene.get_proposal()
+    proposal["q"] *= np.float32(0.97)   # off the truth: gradients are generic
+    n_evals = patch.n_evals()
+
+    # pick the kernel configuration (tile shape) by measurement
+    n_cfg = lib().fpx_n_configs()
+    cands = [args.config] if args.config is not None else list(range(n_cfg))
+    timings = {}
+    for cfg in cands:
+        prop = HIPProposer(patch=patch, config=cfg)
+        raw = np.ascontiguousarray(proposal).view(np.uint8).reshape(-1)
+        pdev = torch.from_numpy(raw.copy()).cuda()
+        time_config(prop, patch, pdev, 3)
+        timings[cfg] = float(np.median(time_config(prop, patch, pdev, 10)))
+    best = min(timings, key=timings.get)
+    prop = HIPProposer(patch=patch, config=best)
+    raw = np.ascontiguousarray(proposal).view(np.uint8).reshape(-1)
+    pdev = torch.from_numpy(raw.copy()).cuda()
+
+    for _ in range(args.warmup):
+        prop.launch(pdev, patch)
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(s)
+        prop.launch(pdev, patch)
+        ends[i].record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per_launch = np.array([a.elapsed_time(b) for a, b in zip(starts, ends)]) * 1e-3
+    region = starts[0].elapsed_time(ends[-1]) * 1e-3
+    kernel_s = float(per_launch.mean())
+
+    # outputs of the timed evaluations: one more evaluation through the full API
+    chi2, derivs = prop.evaluate_proposal(proposal, patch=patch)
+    assert np.all(np.isfinite(chi2)) and np.all(np.isfinite(derivs))
+
+    value = n_evals * args.steps / region
+    achieved = n_evals * FLOP_PER_EVAL / kernel_s
+    probe = valu_probe()
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": region / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded scene, model + Gaussian noise at S/N~50)",
+        "config": {"workload": desc, "evals_per_step": n_evals, "path": "plain",
+                   "kernel_config": best, "config_ms": {str(k): v * 1e3 for k, v in timings.items()},
+                   "wall_s": wall},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                     "kernel_ms": kernel_s * 1e3, "flop_per_eval": FLOP_PER_EVAL},
+        "valu_probe": probe,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(patch, proposal, n_evals)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

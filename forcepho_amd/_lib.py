@@ -1,0 +1,78 @@
+"""ctypes binding of libfpx.so (declarations: include/forcepho_amd.h).
+
+There is no fallback: if the library is missing or fails to load, every
+device entry point raises.  Host-only helpers (struct sizes, tile planning)
+work without a GPU.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+from .build import LIB_PATH
+
+__all__ = ["lib", "check", "FpxError", "TILE_DTYPE", "SLOT_DTYPE",
+           "STRUCT_PATCH", "STRUCT_SOURCE", "STRUCT_RESPONSE", "STRUCT_PSFGAUSS",
+           "STRUCT_TILE", "STRUCT_SLOT", "EXPORTED_SYMBOLS"]
+
+STRUCT_PATCH, STRUCT_SOURCE, STRUCT_RESPONSE, STRUCT_PSFGAUSS, STRUCT_TILE, STRUCT_SLOT = range(6)
+
+# FpxTile / FpxSlot as numpy dtypes (include/forcepho_amd.h)
+TILE_DTYPE = np.dtype([("exposure", np.int32), ("band", np.int32),
+                       ("pix_start", np.int32), ("pix_count", np.int32)])
+SLOT_DTYPE = np.dtype([("patch", np.uintp), ("proposal", np.uintp), ("tiles", np.uintp),
+                       ("pixel_weight", np.uintp), ("pchi2", np.uintp),
+                       ("pdchi2_dp", np.uintp), ("accum", np.uintp), ("ticket", np.uintp),
+                       ("n_bands", np.int32), ("n_tiles", np.int32),
+                       ("flags", np.int32), ("reserved", np.int32)], align=True)
+
+_c_int, _c_size, _p = ctypes.c_int32, ctypes.c_size_t, ctypes.c_void_p
+
+# name -> (restype, argtypes); every symbol include/forcepho_amd.h declares
+_SIGNATURES = {
+    "fpx_abi_version": (_c_int, []),
+    "fpx_last_error": (ctypes.c_char_p, []),
+    "fpx_struct_size": (_c_size, [_c_int]),
+    "fpx_accum_doubles": (_c_size, [_c_int]),
+    "fpx_plan_tiles": (_c_int, [_p, _p, _p, _p, _c_int, _c_int, _p, _c_int]),
+    "fpx_n_configs": (_c_int, []),
+    "fpx_config_info": (_c_int, [_c_int, _p, _p, _p]),
+    "fpx_lds_bytes": (_c_int, [_c_int, _c_int, _p]),
+    "fpx_init": (_c_int, [_c_int]),
+    "fpx_device_info": (_c_int, [_p, _p, _p]),
+    "fpx_evaluate_proposal": (_c_int, [_p, _c_int, _c_int, _c_int, _p]),
+    "fpx_evaluate_batch": (_c_int, [_p, _c_int, _p, _c_int, _c_int, _c_int, _c_int, _p, _p]),
+    "fpx_probe_valu": (_c_int, [_c_int, _c_int, _p, _c_int, _p, _p]),
+}
+EXPORTED_SYMBOLS = tuple(_SIGNATURES)
+
+_LIB = None
+
+
+class FpxError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libfpx.so once and return the ctypes handle."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise FpxError(f"{LIB_PATH} is not built; run forcepho_amd.build.build_library() "
+                           "(or __graft_entry__.build()) first")
+        handle = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_LOCAL)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = handle
+    return _LIB
+
+
+def check(rc, what=""):
+    """Raise FpxError for a negative return code of a C entry point."""
+    if rc < 0:
+        msg = lib().fpx_last_error().decode(errors="replace")
+        raise FpxError(f"{what} failed ({rc}): {msg}")
+    return rc

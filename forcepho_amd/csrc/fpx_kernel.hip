@@ -1,0 +1,739 @@
+// fpx_kernel.hip -- the chi2 + gradient kernel for gfx950 and its C ABI.
+//
+// Work decomposition.  A launch evaluates one or more FpxSlots (a patch plus
+// a proposal).  Each slot's pixels are cut into tiles of 64*PPL pixels of one
+// exposure (fpx_plan_tiles).  One workgroup of W wavefronts owns one tile at a
+// time: every wavefront holds the same 64*PPL pixels (PPL per lane) and the
+// W wavefronts split the exposure's ImageGaussian list between them, so the
+// parallelism is pixels x Gaussian-slices rather than pixels alone (the
+// reference runs one block per band: compute_gaussians_kernel.cu:209,262).
+//
+// Per tile:
+//   prologue  one lane per (source, PSF component) builds the exposure's
+//             ImageGaussian records into LDS (fpx_gauss.h); if they exceed the
+//             LDS budget the sources are processed in chunks and rebuilt.
+//   pass 1    model = sum_g amp_g G_g H_g over the wave's Gaussian slice
+//             (common_kernel.cc:57-83), partial models combined through LDS,
+//             residual r = d - model, chi2 += (r ierr - d ierr)(r ierr + d ierr)
+//             in FP64 (compute_gaussians_kernel.cc:286-293), weight
+//             w2 = -2 r ierr^2.
+//   pass 2    per source, 7 accumulators of w2 * d model / d theta
+//             (common_kernel.cc:95-143 with the per-pixel factor -2 r ierr^2
+//             folded into G and the ra/dec Jacobian CW applied once per source
+//             after the Gaussian loop), wave-reduced into per-wave LDS rows.
+//   combine   per-wave rows summed in fixed order, one FP64 global atomic per
+//             value into the slot's per-band accumulator; the last tile of the
+//             slot (ticket counter) writes pchi2 / Response and re-zeroes the
+//             accumulator.
+// A Gaussian is skipped at a pixel when dx^T F dx > MAX_EXP_ARG
+// (common_kernel.cc:73,108); when that holds for every pixel of a wavefront
+// the whole Gaussian is skipped with a wave-uniform branch.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "forcepho_amd.h"
+#include "fpx_gauss.h"
+
+static_assert(sizeof(FpxPatch) == 152, "Patch layout (patch.cc:131-208)");
+static_assert(offsetof(FpxPatch, n_sources) == 72, "Patch.n_sources offset");
+static_assert(offsetof(FpxPatch, rad2) == 80, "Patch.rad2 offset");
+static_assert(offsetof(FpxPatch, psfgauss_start) == 144, "Patch.psfgauss_start offset");
+static_assert(sizeof(FpxSource) == 264, "Source layout (proposal.cu:13-27)");
+static_assert(offsetof(FpxSource, fluxes) == 24, "Source.fluxes offset");
+static_assert(offsetof(FpxSource, mixture_amplitudes) == 144, "Source.mixture_amplitudes");
+static_assert(offsetof(FpxSource, damplitude_drh) == 184, "Source.damplitude_drh");
+static_assert(offsetof(FpxSource, damplitude_dnsersic) == 224, "Source.damplitude_dnsersic");
+static_assert(sizeof(FpxResponse) == 840, "Response layout (proposal.cu:33-39)");
+static_assert(sizeof(FpxPSFSourceGaussian) == 28, "PSFSourceGaussian layout");
+static_assert(sizeof(FpxTile) == 16, "tile layout");
+static_assert(sizeof(FpxSlot) == 80, "slot layout");
+
+namespace fpx {
+
+constexpr int kAcc = 1 + MAXSOURCES * NPARAMS;        // doubles per band accumulator
+constexpr float kExp2Scale = -0.72134752044448170368f;  // -0.5 * log2(e)
+constexpr float kMaxExpArg = (float)MAX_EXP_ARG;
+
+// ------------------------------------------------------------------------
+// wavefront reductions
+// ------------------------------------------------------------------------
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ bool wave_any(bool p) {
+  return __builtin_amdgcn_ballot_w64(p) != 0;
+}
+
+// LDS size (bytes) of one workgroup; must agree with the carve-up in
+// process_tile.
+__host__ __device__ constexpr size_t lds_bytes_for(int waves, int ppl, int gauss_cap) {
+  return (size_t)gauss_cap * kRec * sizeof(float)      // ImageGaussian records
+         + (size_t)waves * ppl * 64 * sizeof(float)     // partial models
+         + (size_t)MAXSOURCES * 4 * sizeof(float)       // CW per source
+         + (size_t)waves * kAcc * sizeof(double);       // per-wave accumulators
+}
+
+// Build records for sources [s0, s0 + ns) of exposure e into recs.
+template <int W>
+__device__ __forceinline__ void build_chunk(const FpxPatch *__restrict__ P,
+                                            const FpxSource *__restrict__ props,
+                                            int e, int band, int s0, int ns,
+                                            int n_psf, float *__restrict__ recs) {
+  const int n = ns * n_psf;
+  for (int lg = threadIdx.x; lg < n; lg += W * 64) {
+    const int sl = lg / n_psf;
+    const int p = lg - sl * n_psf;
+    build_record(P, props, e, band, s0 + sl, p, recs + lg * kRec);
+  }
+}
+
+// Pass 1: add this wave's slice [gb, ge) of the records to the lane's model.
+template <int PPL>
+__device__ __forceinline__ void model_pass(const float *__restrict__ recs, int gb,
+                                           int ge, const float (&xp)[PPL],
+                                           const float (&yp)[PPL], float (&m)[PPL]) {
+  const float4 *R4 = reinterpret_cast<const float4 *>(recs);
+  for (int g = gb; g < ge; ++g) {
+    const float4 r0 = R4[g * 6 + 0];
+    const float4 r1 = R4[g * 6 + 1];
+    const float xc = r0.x, yc = r0.y, fxx = r0.z, fxy = r0.w;
+    const float fyy = r1.x, a0 = r1.y, a1 = r1.z;
+    float vx[PPL], vy[PPL], ea[PPL];
+    bool on[PPL];
+    bool any = false;
+#pragma unroll
+    for (int i = 0; i < PPL; ++i) {
+      const float dx = xp[i] - xc;
+      const float dy = yp[i] - yc;
+      vx[i] = fxx * dx + fxy * dy;
+      vy[i] = fxy * dx + fyy * dy;
+      ea[i] = dx * vx[i] + dy * vy[i];
+      on[i] = !(ea[i] > kMaxExpArg);
+      any |= on[i];
+    }
+    if (!wave_any(any)) continue;
+#pragma unroll
+    for (int i = 0; i < PPL; ++i) {
+      const float G = on[i] ? __builtin_amdgcn_exp2f(kExp2Scale * ea[i]) : 0.0f;
+      const float s2 = vx[i] * vx[i] + vy[i] * vy[i];
+      m[i] = fmaf(G, fmaf(a1, s2, a0), m[i]);
+    }
+  }
+}
+
+// Pass 2: gradient accumulation over the wave's slice [gb, ge) of a chunk
+// whose first source is s0; flushes one source at a time into wrow.
+template <int PPL>
+__device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int gb,
+                                          int ge, int s0, int n_psf,
+                                          const float (&xp)[PPL], const float (&yp)[PPL],
+                                          const float (&w2)[PPL],
+                                          const float *__restrict__ cwl,
+                                          double *__restrict__ wrow, int lane) {
+  const float4 *R4 = reinterpret_cast<const float4 *>(recs);
+  int g = gb;
+  while (g < ge) {
+    const int sl = g / n_psf;
+    const int gstop = min(ge, (sl + 1) * n_psf);
+    float aF = 0.f, aX = 0.f, aY = 0.f, aQ = 0.f, aP = 0.f, aN = 0.f, aR = 0.f;
+    for (; g < gstop; ++g) {
+      const float4 r0 = R4[g * 6 + 0];
+      const float4 r1 = R4[g * 6 + 1];
+      const float xc = r0.x, yc = r0.y, fxx = r0.z, fxy = r0.w;
+      const float fyy = r1.x, h0 = r1.w;
+      float dx[PPL], dy[PPL], vx[PPL], vy[PPL], px[PPL], py[PPL], ea[PPL];
+      bool on[PPL];
+      bool any = false;
+#pragma unroll
+      for (int i = 0; i < PPL; ++i) {
+        dx[i] = xp[i] - xc;
+        dy[i] = yp[i] - yc;
+        vx[i] = fxx * dx[i] + fxy * dy[i];
+        vy[i] = fxy * dx[i] + fyy * dy[i];
+        px[i] = dx[i] * vx[i];
+        py[i] = dy[i] * vy[i];
+        ea[i] = px[i] + py[i];
+        on[i] = !(ea[i] > kMaxExpArg);
+        any |= on[i];
+      }
+      if (!wave_any(any)) continue;
+      const float4 r2 = R4[g * 6 + 2];
+      const float4 r3 = R4[g * 6 + 3];
+      const float4 r4 = R4[g * 6 + 4];
+      const float4 r5 = R4[g * 6 + 5];
+      const float amp = r2.x, amp12 = r2.y, dAF = r2.z, dAN = r2.w;
+      const float dAR = r3.x, dAQ = r3.y, kq = r3.z, qxx = r3.w;
+      const float qyy = r4.x, qxy = r4.y, dAP = r4.z, kp = r4.w;
+      const float pxx = r5.x, pyy = r5.y, pxy = r5.z;
+#pragma unroll
+      for (int i = 0; i < PPL; ++i) {
+        const float G = on[i] ? __builtin_amdgcn_exp2f(kExp2Scale * ea[i]) : 0.0f;
+        const float gw = G * w2[i];
+        const float s2 = vx[i] * vx[i] + vy[i] * vy[i];
+        const float H = fmaf(s2, 1.0f / 24.0f, h0);
+        const float gH = gw * H;
+        aF = fmaf(dAF, gH, aF);
+        aN = fmaf(dAN, gH, aN);
+        aR = fmaf(dAR, gH, aR);
+        const float C = amp * gH;
+        const float c12 = amp12 * gw;
+        const float ux = fxx * vx[i] + fxy * vy[i];
+        const float uy = fxy * vx[i] + fyy * vy[i];
+        aX = fmaf(C, vx[i], aX);
+        aX = fmaf(-c12, ux, aX);
+        aY = fmaf(C, vy[i], aY);
+        aY = fmaf(-c12, uy, aY);
+        const float dxx = dx[i] * dx[i];
+        const float dyy = dy[i] * dy[i];
+        const float dxy2 = (dx[i] + dx[i]) * dy[i];
+        const float cr = dy[i] * vx[i] + dx[i] * vy[i];
+        const float hC = -0.5f * C;
+        const float t1q = kq + qxx * px[i] + qyy * py[i] + qxy * cr;
+        const float t2q = qxx * dxx + qyy * dyy + qxy * dxy2;
+        aQ = fmaf(dAQ, gH, aQ);
+        aQ = fmaf(c12, t1q, aQ);
+        aQ = fmaf(hC, t2q, aQ);
+        const float t1p = kp + pxx * px[i] + pyy * py[i] + pxy * cr;
+        const float t2p = pxx * dxx + pyy * dyy + pxy * dxy2;
+        aP = fmaf(dAP, gH, aP);
+        aP = fmaf(c12, t1p, aP);
+        aP = fmaf(hC, t2p, aP);
+      }
+    }
+    aF = wave_sum(aF);
+    aX = wave_sum(aX);
+    aY = wave_sum(aY);
+    aQ = wave_sum(aQ);
+    aP = wave_sum(aP);
+    aN = wave_sum(aN);
+    aR = wave_sum(aR);
+    if (lane == 0) {
+      const int s = s0 + sl;
+      const float *cw = cwl + 4 * s;  // [v11 v12 v21 v22] = dpix/dsky
+      double *row = wrow + 1 + NPARAMS * s;
+      row[0] += (double)aF;
+      row[1] += (double)cw[0] * aX + (double)cw[2] * aY;
+      row[2] += (double)cw[1] * aX + (double)cw[3] * aY;
+      row[3] += (double)aQ;
+      row[4] += (double)aP;
+      row[5] += (double)aN;
+      row[6] += (double)aR;
+    }
+  }
+}
+
+template <int W, int PPL>
+__device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPatch *P,
+                          const FpxSource *props, int e, int band, int n_src, int n_psf,
+                          int gauss_cap, unsigned char *smem) {
+  constexpr int kPix = 64 * PPL;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  // the wave index is uniform across the wavefront; saying so puts the
+  // Gaussian loop bounds below in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  float *recs = reinterpret_cast<float *>(smem);
+  float *part = recs + (size_t)gauss_cap * kRec;
+  float *cwl = part + W * kPix;
+  double *wacc = reinterpret_cast<double *>(cwl + MAXSOURCES * 4);
+  const int spc = max(1, gauss_cap / n_psf);  // sources per LDS chunk
+  const int n_chunks = (n_src + spc - 1) / spc;
+  const bool vjp = slot.pixel_weight != nullptr;
+
+  for (int j = tid; j < W * kAcc; j += W * 64) wacc[j] = 0.0;
+  for (int j = tid; j < 4 * n_src; j += W * 64) {
+    const int s = j >> 2;
+    cwl[j] = P->CW[4 * (n_src * e + s) + (j & 3)];
+  }
+
+  float xp[PPL], yp[PPL], w2[PPL];
+  int pix[PPL];
+  bool valid[PPL];
+#pragma unroll
+  for (int i = 0; i < PPL; ++i) {
+    const int off = i * 64 + lane;
+    valid[i] = off < tile.pix_count;
+    pix[i] = tile.pix_start + off;
+    xp[i] = valid[i] ? P->xpix[pix[i]] : 0.0f;
+    yp[i] = valid[i] ? P->ypix[pix[i]] : 0.0f;
+  }
+
+  if (!vjp) {
+    float m[PPL];
+#pragma unroll
+    for (int i = 0; i < PPL; ++i) m[i] = 0.0f;
+    for (int c = 0; c < n_chunks; ++c) {
+      const int s0 = c * spc;
+      const int ns = min(spc, n_src - s0);
+      __syncthreads();
+      build_chunk<W>(P, props, e, band, s0, ns, n_psf, recs);
+      __syncthreads();
+      const int n = ns * n_psf;
+      model_pass<PPL>(recs, (n * wave) / W, (n * (wave + 1)) / W, xp, yp, m);
+    }
+#pragma unroll
+    for (int i = 0; i < PPL; ++i) part[(wave * PPL + i) * 64 + lane] = m[i];
+    __syncthreads();
+    double chi2 = 0.0;
+    float *resid = P->residual;
+#pragma unroll
+    for (int i = 0; i < PPL; ++i) {
+      float model = 0.0f;
+#pragma unroll
+      for (int w = 0; w < W; ++w) model += part[(w * PPL + i) * 64 + lane];
+      const float d = valid[i] ? P->data[pix[i]] : 0.0f;
+      const float ie = valid[i] ? P->ierr[pix[i]] : 0.0f;
+      const float r = d - model;
+      if (wave == 0 && valid[i] && resid != nullptr) resid[pix[i]] = r;
+      const float ri = r * ie;
+      const float di = d * ie;
+      chi2 += ((double)ri - (double)di) * ((double)ri + (double)di);
+      w2[i] = -2.0f * (ri * ie);
+    }
+    if (wave == 0) {
+      chi2 = wave_sum(chi2);
+      if (lane == 0) wacc[0] += chi2;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < PPL; ++i) w2[i] = valid[i] ? slot.pixel_weight[pix[i]] : 0.0f;
+  }
+
+  bool wave_has_weight = false;
+#pragma unroll
+  for (int i = 0; i < PPL; ++i) wave_has_weight |= (w2[i] != 0.0f);
+  wave_has_weight = wave_any(wave_has_weight);
+
+  double *wrow = wacc + wave * kAcc;
+  for (int c = 0; c < n_chunks; ++c) {
+    const int s0 = c * spc;
+    const int ns = min(spc, n_src - s0);
+    if (n_chunks > 1 || vjp) {
+      __syncthreads();
+      build_chunk<W>(P, props, e, band, s0, ns, n_psf, recs);
+      __syncthreads();
+    }
+    const int n = ns * n_psf;
+    if (wave_has_weight)
+      grad_pass<PPL>(recs, (n * wave) / W, (n * (wave + 1)) / W, s0, n_psf, xp, yp, w2,
+                     cwl, wrow, lane);
+  }
+}
+
+// Add the tile's per-wave rows to the slot accumulator (band < 0: nothing
+// to add) and, for the slot's last tile, write the outputs.
+template <int W>
+__device__ void finish_tile(const FpxSlot &slot, int band, int n_src, unsigned char *smem,
+                            int gauss_cap, int ppl, int *s_flag) {
+  const int tid = threadIdx.x;
+  float *cwl = reinterpret_cast<float *>(smem) + (size_t)gauss_cap * kRec + W * 64 * ppl;
+  double *wacc = reinterpret_cast<double *>(cwl + MAXSOURCES * 4);
+  if (band >= 0) {
+    double *acc = slot.accum + (size_t)band * kAcc;
+    const int nv = 1 + NPARAMS * n_src;
+    for (int j = tid; j < nv; j += W * 64) {
+      double v = 0.0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) v += wacc[w * kAcc + j];
+      if (v != 0.0) unsafeAtomicAdd(acc + j, v);
+    }
+  }
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) {
+    const unsigned t = atomicAdd(slot.ticket, 1u);
+    *s_flag = (t == (unsigned)(slot.n_tiles - 1)) ? 1 : 0;
+  }
+  __syncthreads();
+  if (*s_flag) {
+    __threadfence();
+    const int nb = min(slot.n_bands, MAXBANDS);
+    for (int j = tid; j < nb * kAcc; j += W * 64) {
+      const double v = __hip_atomic_load(slot.accum + j, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+      const int b = j / kAcc;
+      const int k = j - b * kAcc;
+      if (k == 0) {
+        slot.pchi2[b] = v;
+      } else {
+        const int q = k - 1;
+        slot.pdchi2_dp[b * (MAXSOURCES * NPARAMS) + q] = (float)v;
+      }
+      slot.accum[j] = 0.0;
+    }
+    if (tid == 0) *slot.ticket = 0u;
+  }
+  __syncthreads();
+}
+
+template <int W, int PPL>
+__device__ void process_tile(const FpxSlot &slot, int local, int gauss_cap,
+                             unsigned char *smem, int *s_flag) {
+  constexpr int kPix = 64 * PPL;
+  // Scalars that every lane holds identically.  The loads below are vector
+  // loads (the kernel writes global memory, so they cannot be proven constant),
+  // hence the explicit broadcasts: they put the loop bounds in SGPRs.
+  const FpxTile tile = slot.tiles[local];
+  const FpxPatch *P = slot.patch;
+  const FpxSource *props = slot.proposal;
+  const int e = __builtin_amdgcn_readfirstlane(tile.exposure);
+  const int band = __builtin_amdgcn_readfirstlane(tile.band);
+  const int n_src = __builtin_amdgcn_readfirstlane(P->n_sources);
+  const int n_psf = __builtin_amdgcn_readfirstlane(P->n_psf_per_source[band]);
+  // A tile whose sizes break the kernel's assumptions contributes nothing
+  // (every lane sees the same values, so the whole workgroup agrees).
+  const bool ok = n_src >= 1 && n_src <= MAXSOURCES && n_psf >= 1 && n_psf <= gauss_cap &&
+                  n_psf <= FPX_MAX_PSF_PER_SOURCE && band >= 0 && band < slot.n_bands &&
+                  band < MAXBANDS && tile.pix_count >= 0 && tile.pix_count <= kPix;
+  if (ok) tile_work<W, PPL>(slot, tile, P, props, e, band, n_src, n_psf, gauss_cap, smem);
+  __syncthreads();
+  finish_tile<W>(slot, ok ? band : -1, ok ? n_src : 0, smem, gauss_cap, PPL, s_flag);
+}
+
+template <int W, int PPL>
+__global__ __launch_bounds__(W * 64) void chi2_grad_kernel(
+    const FpxSlot *__restrict__ slots, int n_slots, const int32_t *__restrict__ item_offsets,
+    int n_items, int gauss_cap, int persistent, uint32_t *__restrict__ queue) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int s_item;
+  __shared__ int s_flag;
+  for (int it = 0;; ++it) {
+    int item;
+    if (persistent) {
+      if (threadIdx.x == 0) s_item = (int)atomicAdd(queue, 1u);
+      __syncthreads();
+      item = __builtin_amdgcn_readfirstlane(s_item);
+      __syncthreads();
+    } else {
+      if (it > 0) break;
+      item = blockIdx.x;
+    }
+    if (item >= n_items) break;
+    int si = 0;
+    int local = item;
+    if (n_slots > 1) {
+      int lo = 0, hi = n_slots - 1;  // largest si with item_offsets[si] <= item
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (item_offsets[mid] <= item) lo = mid; else hi = mid - 1;
+      }
+      si = lo;
+      local = item - item_offsets[si];
+    }
+    const FpxSlot slot = slots[si];
+    process_tile<W, PPL>(slot, local, gauss_cap, smem, &s_flag);
+  }
+  if (persistent) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const unsigned d = atomicAdd(queue + 1, 1u);
+      if (d == gridDim.x - 1) {
+        queue[0] = 0u;
+        queue[1] = 0u;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// VALU throughput probes
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void probe_fma_kernel(float *out, int iters) {
+  float a[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) a[k] = (float)(threadIdx.x + k) * 1e-3f;
+  const float b = 0.999f, c = 1e-4f;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = fmaf(a[k], b, c);
+  }
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s += a[k];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void probe_exp_kernel(float *out, int iters) {
+  float a[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) a[k] = (float)((threadIdx.x + k) & 63) * (1.0f / 64.0f);
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = __builtin_amdgcn_exp2f(-a[k]);
+  }
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s += a[k];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+}  // namespace fpx
+
+// ==========================================================================
+// host side
+// ==========================================================================
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_err(hipError_t e, const char *where) {
+  return set_err(FPX_E_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define FPX_HIP(call)                                  \
+  do {                                                 \
+    hipError_t _e = (call);                            \
+    if (_e != hipSuccess) return hip_err(_e, #call);   \
+  } while (0)
+
+struct KernelCfg {
+  int waves;
+  int ppl;
+  const void *fn;
+};
+
+#define FPX_CFG(W, P) {W, P, reinterpret_cast<const void *>(&fpx::chi2_grad_kernel<W, P>)}
+const KernelCfg kConfigs[] = {
+    FPX_CFG(4, 1), FPX_CFG(4, 2), FPX_CFG(4, 4), FPX_CFG(8, 1),
+    FPX_CFG(8, 2), FPX_CFG(8, 4), FPX_CFG(2, 4), FPX_CFG(16, 1),
+};
+#undef FPX_CFG
+constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
+
+struct DeviceState {
+  bool checked = false;
+  bool ok = false;
+  int n_cu = 0;
+  int lds_max = 0;
+  int clock_khz = 0;
+  size_t lds_set[kNumConfigs] = {};
+};
+
+std::mutex g_mu;
+DeviceState g_dev[64];
+
+int current_device_state(DeviceState **out) {
+  int dev = 0;
+  FPX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return set_err(FPX_E_NODEV, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_mu);
+  DeviceState &st = g_dev[dev];
+  if (!st.checked) {
+    st.checked = true;
+    hipDeviceProp_t prop;
+    FPX_HIP(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+      return set_err(FPX_E_NODEV, std::string("device is ") + prop.gcnArchName +
+                                      ", this library is built for gfx950 only");
+    }
+    st.n_cu = prop.multiProcessorCount;
+    st.lds_max = (int)prop.sharedMemPerBlock;
+    st.clock_khz = prop.clockRate;
+    st.ok = true;
+  }
+  if (!st.ok) return set_err(FPX_E_NODEV, "current device is not a usable gfx950");
+  *out = &st;
+  return FPX_OK;
+}
+
+int check_cfg(int32_t cfg) {
+  if (cfg < 0 || cfg >= kNumConfigs) return set_err(FPX_E_ARG, "invalid kernel configuration");
+  return FPX_OK;
+}
+
+// Largest Gaussian count per LDS chunk: all of them if they fit.
+int pick_gauss_cap(const KernelCfg &kc, int max_gauss, int lds_max) {
+  const size_t fixed = fpx::lds_bytes_for(kc.waves, kc.ppl, 0);
+  const size_t per = fpx::kRec * sizeof(float);
+  const long fit = ((long)lds_max - (long)fixed) / (long)per;
+  return (int)std::min<long>(max_gauss, fit);
+}
+
+int launch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
+           int32_t n_items, int32_t max_gauss, int32_t cfg, int32_t persistent,
+           uint32_t *queue, void *stream) {
+  int rc = check_cfg(cfg);
+  if (rc) return rc;
+  if (!slots || n_slots <= 0 || n_items < 0) return set_err(FPX_E_ARG, "empty or invalid slot list");
+  if (n_slots > 1 && !item_offsets) return set_err(FPX_E_ARG, "item_offsets required for n_slots > 1");
+  if (max_gauss <= 0 || max_gauss > MAXSOURCES * FPX_MAX_PSF_PER_SOURCE)
+    return set_err(FPX_E_ARG, "max_gauss out of range [1, MAXSOURCES*FPX_MAX_PSF_PER_SOURCE]");
+  if (persistent && !queue) return set_err(FPX_E_ARG, "persistent launch needs a queue counter");
+  if (n_items == 0) return FPX_OK;
+  DeviceState *st = nullptr;
+  rc = current_device_state(&st);
+  if (rc) return rc;
+  const KernelCfg &kc = kConfigs[cfg];
+  const int cap = pick_gauss_cap(kc, max_gauss, st->lds_max);
+  if (cap < FPX_MAX_PSF_PER_SOURCE && cap < max_gauss)
+    return set_err(FPX_E_ARG, "LDS too small for one source's Gaussians");
+  const size_t lds = fpx::lds_bytes_for(kc.waves, kc.ppl, cap);
+  {
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (st->lds_set[cfg] < lds) {
+      FPX_HIP(hipFuncSetAttribute(kc.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      st->lds_set[cfg] = lds;
+    }
+  }
+  const int threads = kc.waves * 64;
+  int grid = n_items;
+  if (persistent) {
+    int per_cu = 0;
+    FPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kc.fn, threads, lds));
+    if (per_cu < 1) per_cu = 1;
+    grid = std::min<long>((long)per_cu * st->n_cu, (long)n_items);
+  }
+  void *args[] = {(void *)&slots,   (void *)&n_slots, (void *)&item_offsets, (void *)&n_items,
+                  (void *)&cap,     (void *)&persistent, (void *)&queue};
+  FPX_HIP(hipLaunchKernel(kc.fn, dim3(grid), dim3(threads), args, lds, (hipStream_t)stream));
+  return FPX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fpx_abi_version(void) { return FPX_ABI_VERSION; }
+
+const char *fpx_last_error(void) { return g_err.c_str(); }
+
+size_t fpx_struct_size(int which) {
+  switch (which) {
+    case FPX_STRUCT_PATCH: return sizeof(FpxPatch);
+    case FPX_STRUCT_SOURCE: return sizeof(FpxSource);
+    case FPX_STRUCT_RESPONSE: return sizeof(FpxResponse);
+    case FPX_STRUCT_PSFGAUSS: return sizeof(FpxPSFSourceGaussian);
+    case FPX_STRUCT_TILE: return sizeof(FpxTile);
+    case FPX_STRUCT_SLOT: return sizeof(FpxSlot);
+    default: return 0;
+  }
+}
+
+size_t fpx_accum_doubles(int32_t n_bands) {
+  return n_bands > 0 ? (size_t)n_bands * fpx::kAcc : 0;
+}
+
+int fpx_plan_tiles(const int32_t *exposure_start, const int32_t *exposure_N,
+                   const int16_t *band_start, const int16_t *band_N, int32_t n_bands,
+                   int32_t pix_per_tile, FpxTile *tiles_out, int32_t max_tiles) {
+  if (!exposure_start || !exposure_N || !band_start || !band_N)
+    return set_err(FPX_E_ARG, "null index array");
+  if (n_bands <= 0 || n_bands > MAXBANDS) return set_err(FPX_E_ARG, "n_bands out of range [1, MAXBANDS]");
+  if (pix_per_tile <= 0 || pix_per_tile % FPX_WAVE != 0)
+    return set_err(FPX_E_ARG, "pix_per_tile must be a positive multiple of FPX_WAVE");
+  long count = 0;
+  for (int b = 0; b < n_bands; ++b) {
+    if (band_N[b] < 0 || band_start[b] < 0) return set_err(FPX_E_ARG, "negative band index");
+    for (int k = 0; k < band_N[b]; ++k) {
+      const int e = band_start[b] + k;
+      const int n = exposure_N[e];
+      if (n < 0 || exposure_start[e] < 0) return set_err(FPX_E_ARG, "negative exposure size or start");
+      for (int off = 0; off < n; off += pix_per_tile) {
+        if (tiles_out) {
+          if (count >= max_tiles) return set_err(FPX_E_ARG, "tiles_out too small");
+          FpxTile &t = tiles_out[count];
+          t.exposure = e;
+          t.band = b;
+          t.pix_start = exposure_start[e] + off;
+          t.pix_count = std::min(pix_per_tile, n - off);
+        }
+        ++count;
+        if (count > INT32_MAX) return set_err(FPX_E_ARG, "too many tiles");
+      }
+    }
+  }
+  return (int)count;
+}
+
+int fpx_n_configs(void) { return kNumConfigs; }
+
+int fpx_config_info(int32_t cfg, int32_t *waves_per_group, int32_t *pixels_per_lane,
+                    int32_t *pixels_per_tile) {
+  int rc = check_cfg(cfg);
+  if (rc) return rc;
+  if (waves_per_group) *waves_per_group = kConfigs[cfg].waves;
+  if (pixels_per_lane) *pixels_per_lane = kConfigs[cfg].ppl;
+  if (pixels_per_tile) *pixels_per_tile = 64 * kConfigs[cfg].ppl;
+  return FPX_OK;
+}
+
+int fpx_lds_bytes(int32_t cfg, int32_t max_gauss, size_t *bytes) {
+  int rc = check_cfg(cfg);
+  if (rc) return rc;
+  if (!bytes || max_gauss < 0) return set_err(FPX_E_ARG, "bad argument");
+  *bytes = fpx::lds_bytes_for(kConfigs[cfg].waves, kConfigs[cfg].ppl, max_gauss);
+  return FPX_OK;
+}
+
+int fpx_init(int32_t device) {
+  int n = 0;
+  FPX_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return set_err(FPX_E_NODEV, "no such device");
+  FPX_HIP(hipSetDevice(device));
+  DeviceState *st = nullptr;
+  return current_device_state(&st);
+}
+
+int fpx_device_info(int32_t *n_cu, int32_t *lds_per_group, int32_t *clock_khz) {
+  DeviceState *st = nullptr;
+  int rc = current_device_state(&st);
+  if (rc) return rc;
+  if (n_cu) *n_cu = st->n_cu;
+  if (lds_per_group) *lds_per_group = st->lds_max;
+  if (clock_khz) *clock_khz = st->clock_khz;
+  return FPX_OK;
+}
+
+int fpx_evaluate_proposal(const FpxSlot *slot, int32_t n_tiles, int32_t max_gauss,
+                          int32_t cfg, void *stream) {
+  return launch(slot, 1, nullptr, n_tiles, max_gauss, cfg, 0, nullptr, stream);
+}
+
+int fpx_evaluate_batch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
+                       int32_t n_items, int32_t max_gauss, int32_t cfg, int32_t persistent,
+                       uint32_t *queue, void *stream) {
+  return launch(slots, n_slots, item_offsets, n_items, max_gauss, cfg, persistent ? 1 : 0,
+                queue, stream);
+}
+
+int fpx_probe_valu(int32_t kind, int32_t iters, float *out, int32_t n_threads, double *ops,
+                   void *stream) {
+  if (!out || n_threads <= 0 || n_threads % 256 != 0 || iters <= 0)
+    return set_err(FPX_E_ARG, "probe needs n_threads a positive multiple of 256 and iters > 0");
+  DeviceState *st = nullptr;
+  int rc = current_device_state(&st);
+  if (rc) return rc;
+  const dim3 grid(n_threads / 256), block(256);
+  if (kind == 0) {
+    hipLaunchKernelGGL(fpx::probe_fma_kernel, grid, block, 0, (hipStream_t)stream, out, iters);
+  } else if (kind == 1) {
+    hipLaunchKernelGGL(fpx::probe_exp_kernel, grid, block, 0, (hipStream_t)stream, out, iters);
+  } else {
+    return set_err(FPX_E_ARG, "probe kind must be 0 (fma) or 1 (exp)");
+  }
+  FPX_HIP(hipGetLastError());
+  if (ops) *ops = (double)n_threads * (double)iters * 8.0;
+  return FPX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,217 @@
+"""Device residency of a patch on an MI355X (HIP, device memory via torch).
+
+``HIPPatchMixin`` keeps the method surface of forcepho's GPUPatchMixin
+(forcepho/patches/device_patch.py:257-435): ``send_to_device``,
+``replace_device_meta_ptrs``, ``send_patchstruct_to_device``,
+``swap_on_device``, ``retrieve_array``, ``free`` and ``get_proposer``, plus
+the attributes ``device_ptrs`` (name -> device buffer), ``device_patch`` (the
+FpxPatch struct in device memory) and ``patch_struct`` (its host copy).
+``prepare_model``/``subtract_fixed``/``design_matrix`` follow
+forcepho/patches/device_patch.py:28-238.
+
+torch is used only to own device buffers and for copies; all computation
+happens in libfpx.so.
+"""
+
+import numpy as np
+
+from .._lib import lib, check, TILE_DTYPE
+
+__all__ = ["DevicePatchMixin", "HIPPatchMixin", "device_bytes"]
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("no ROCm device visible: the HIP path needs an MI355X")
+    return torch
+
+
+def device_bytes(arr, device):
+    """Copy a numpy array's bytes to a new uint8 device tensor."""
+    torch = _torch()
+    raw = np.ascontiguousarray(arr).view(np.uint8).reshape(-1)
+    if raw.size == 0:
+        raw = np.zeros(8, dtype=np.uint8)
+    return torch.from_numpy(raw.copy()).to(device)
+
+
+class DevicePatchMixin:
+    """Device-independent logic shared by device mixins."""
+
+    def prepare_model(self, active=None, fixed=None, bounds=None, maxactive=15,
+                      model_kwargs={}):
+        """Subtract fixed sources, load the active scene and wrap the patch in
+        a FastPosterior (forcepho/patches/device_patch.py:28-72).
+
+        Parameters
+        ----------
+        active : Scene
+        fixed : Scene or None
+        bounds : (lower, upper) arrays for a BoundedTransform, or None
+        """
+        from ..model import FastPosterior, Transform
+        proposer, scene = self.subtract_fixed(fixed, active=active)
+        self.return_residual = False
+        self.send_patchstruct_to_device()
+        q = scene.get_all_source_params().copy()
+        if bounds is None:
+            model = FastPosterior(proposer, scene=scene, patch=self,
+                                  transform=Transform(len(q)), **model_kwargs)
+        else:
+            lo, hi = bounds
+            model = FastPosterior(proposer, scene=scene, patch=self, lower=lo, upper=hi,
+                                  **model_kwargs)
+        return model, q
+
+    def subtract_fixed(self, fixed, active=None):
+        """Subtract a fixed scene from the device-side data, then load the
+        active scene's metadata (forcepho/patches/device_patch.py:113-183).
+        After this the device ``data`` holds data - model(fixed)."""
+        scenes = []
+        if fixed is not None:
+            scenes.append(fixed)
+        assert active is not None, "an active scene is required"
+        scenes.append(active)
+        proposer = self.get_proposer()
+        self.return_residual = True
+        self.pack_meta(scenes[0])
+        self.send_to_device()
+        for i, scene in enumerate(scenes[1:]):
+            proposal = scenes[i].get_proposal()
+            proposer.evaluate_proposal(proposal, patch=self)
+            self.pack_meta(scene)
+            self.swap_on_device()
+        return proposer, scenes[-1]
+
+    def design_matrix(self, active, proposer=None):
+        """Per-band design matrices of unit-flux source models
+        (forcepho/patches/device_patch.py:185-238).
+
+        Returns
+        -------
+        Xes : list of (n_source, n_pix_band) arrays, model of each source per
+              unit flux in each band
+        ys : list of (n_pix_band,) arrays, the device-side data split by band
+        """
+        y = self.retrieve_array("data")
+        ys = np.split(y, np.cumsum(self.band_N_pix[:-1]))
+        n_src = len(active.sources)
+        Xes = [np.zeros((n_src, n)) for n in self.band_N_pix]
+        if proposer is None:
+            proposer = self.get_proposer()
+        self.return_residual = True
+        for i, source in enumerate(active.sources):
+            single = active.subscene([i])
+            self.pack_meta(single)
+            self.replace_device_meta_ptrs()
+            self.send_patchstruct_to_device()
+            proposal = single.get_proposal()
+            _, _, residual = proposer.evaluate_proposal(proposal, patch=self, unpack=False)
+            model = y - residual
+            split = np.split(model, np.cumsum(self.band_N_pix[:-1]))
+            for j in range(self.n_bands):
+                Xes[j][i, :] = split[j] / source.flux[j]
+        return Xes, ys
+
+
+class HIPPatchMixin(DevicePatchMixin):
+    """Mix-in that puts a packed patch in MI355X device memory."""
+
+    device = None   # torch device; default cuda:0
+
+    def _dev(self):
+        torch = _torch()
+        if self.device is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        return self.device
+
+    def get_proposer(self):
+        from ..proposal import HIPProposer
+        return HIPProposer(patch=self)
+
+    def send_to_device(self):
+        """Copy every array named in the Patch struct to the device and build
+        the device-side struct; returns ``device_patch``."""
+        dev = self._dev()
+        self.device_ptrs = {}
+        for name in self.patch_struct_dtype.names:
+            if self.patch_struct_dtype[name] != self.ptr_dtype or name == "residual":
+                continue
+            self.device_ptrs[name] = device_bytes(getattr(self, name), dev)
+        if self.return_residual:
+            torch = _torch()
+            self.device_ptrs["residual"] = torch.zeros(self.xpix.nbytes, dtype=torch.uint8,
+                                                       device=dev)
+        self._tiles = {}
+        self.send_patchstruct_to_device()
+        self._dirty_data = False
+        return self.device_patch
+
+    def replace_device_meta_ptrs(self):
+        """Upload the currently packed metadata arrays, replacing (and
+        releasing) the previous device copies."""
+        dev = self._dev()
+        for name in self.meta_names:
+            if self.patch_struct_dtype[name] == self.ptr_dtype:
+                self.device_ptrs[name] = device_bytes(getattr(self, name), dev)
+
+    def send_patchstruct_to_device(self):
+        """Fill the FpxPatch struct with device pointers and scalar values and
+        copy it to the device."""
+        ps = np.zeros(1, dtype=self.patch_struct_dtype)[0]
+        for name in self.patch_struct_dtype.names:
+            if self.patch_struct_dtype[name] == self.ptr_dtype:
+                buf = self.device_ptrs.get(name)
+                if name == "residual" and not self.return_residual:
+                    buf = None
+                ps[name] = 0 if buf is None else buf.data_ptr()
+            else:
+                ps[name] = getattr(self, name)
+        self.patch_struct = ps
+        self.device_patch = device_bytes(np.array([ps]), self._dev())
+        self._struct_generation = getattr(self, "_struct_generation", 0) + 1
+        return self.device_patch
+
+    def swap_on_device(self):
+        """Upload the packed metadata, swap the device data and residual
+        buffers and rebuild the struct: afterwards the device uses the former
+        residual as its data (forcepho/patches/device_patch.py:343-364)."""
+        assert "residual" in self.device_ptrs, "Must instantiate the Patch with `return_residual=True`"
+        self.replace_device_meta_ptrs()
+        d = self.device_ptrs
+        d["data"], d["residual"] = d["residual"], d["data"]
+        self.send_patchstruct_to_device()
+        self._dirty_data = True
+        return self.device_patch
+
+    def retrieve_array(self, arrname="residual"):
+        """Copy a device pixel array back to the host (float32 [npix])."""
+        buf = self.device_ptrs[arrname]
+        return buf.cpu().numpy().view(self.pix_dtype)[:self.npix].copy()
+
+    def device_tiles(self, pix_per_tile):
+        """Device FpxTile table of this patch for a tile size (cached);
+        returns (tensor, n_tiles)."""
+        if pix_per_tile not in self._tiles:
+            L = lib()
+            args = [np.ascontiguousarray(self.exposure_start, dtype=np.int32),
+                    np.ascontiguousarray(self.exposure_N, dtype=np.int32),
+                    np.ascontiguousarray(self.band_start, dtype=np.int16),
+                    np.ascontiguousarray(self.band_N, dtype=np.int16)]
+            ptrs = [a.ctypes.data for a in args]
+            n = check(L.fpx_plan_tiles(*ptrs, int(self.n_bands), int(pix_per_tile), None, 0),
+                      "fpx_plan_tiles")
+            tiles = np.zeros(max(n, 1), dtype=TILE_DTYPE)
+            check(L.fpx_plan_tiles(*ptrs, int(self.n_bands), int(pix_per_tile),
+                                   tiles.ctypes.data, n), "fpx_plan_tiles")
+            self._tiles[pix_per_tile] = (device_bytes(tiles, self._dev()), n)
+        return self._tiles[pix_per_tile]
+
+    def max_gauss(self):
+        return int(self.n_sources) * int(np.max(self.n_psf_per_source))
+
+    def free(self):
+        self.device_ptrs = {}
+        self.device_patch = None
+        self._tiles = {}

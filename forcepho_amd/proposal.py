@@ -1,0 +1,176 @@
+"""Host interface that evaluates a proposal on the MI355X.
+
+``HIPProposer.evaluate_proposal`` keeps the contract of forcepho's
+GPUProposer/CPUProposer.evaluate_proposal (forcepho/proposal.py:96-152,
+180-226): given an array of ``source_struct_dtype`` records it returns
+
+    chi2        float64 [n_bands]
+    chi2_derivs [n_bands, n_sources, NPARAMS]
+    residuals   only when ``patch.return_residual``: list of per-exposure
+                padded arrays (``unpack=True``) or one flat array
+
+The launch itself is one call into libfpx.so (fpx_evaluate_proposal); the
+proposal goes host->device and the results device->host through pinned
+buffers on the current torch stream.
+"""
+
+import sys
+
+import numpy as np
+
+from .kernel_limits import MAXBANDS, MAXRADII, MAXSOURCES, NPARAMS
+from ._lib import lib, check, SLOT_DTYPE
+
+__all__ = ["ProposerBase", "HIPProposer", "GPUProposer", "Proposer",
+           "source_struct_dtype", "response_struct_dtype", "DEFAULT_CONFIG"]
+
+# forcepho/proposal.py:229-244
+source_float_dt = np.float32
+source_struct_dtype = np.dtype([("ra", source_float_dt),
+                                ("dec", source_float_dt),
+                                ("q", source_float_dt),
+                                ("pa", source_float_dt),
+                                ("nsersic", source_float_dt),
+                                ("rh", source_float_dt),
+                                ("fluxes", source_float_dt, MAXBANDS),
+                                ("mixture_amplitudes", source_float_dt, MAXRADII),
+                                ("damplitude_drh", source_float_dt, MAXRADII),
+                                ("damplitude_dnsersic", source_float_dt, MAXRADII)],
+                               align=True)
+response_float_dt = source_float_dt
+response_struct_dtype = np.dtype([("dchi2_dparam", response_float_dt, NPARAMS * MAXSOURCES)],
+                                 align=True)
+
+# kernel configuration used when none is given: 4 wavefronts x 4 pixels/lane
+DEFAULT_CONFIG = 2
+
+
+class ProposerBase:
+
+    def evaluate_proposal(self, proposal, **kwargs):
+        raise NotImplementedError
+
+    def unpack_residuals(self, residuals_flat, patch, reshape=False):
+        """Split a flat padded residual array into per-exposure arrays
+        (forcepho/proposal.py:38-52); ``reshape`` returns images."""
+        residuals = np.split(residuals_flat, np.cumsum(patch.exposure_N)[:-1])
+        if reshape:
+            residuals = [patch.unpack_image(residuals_flat, e) for e in range(len(residuals))]
+        return residuals
+
+
+class HIPProposer(ProposerBase):
+    """Evaluates proposals with the gfx950 kernel.
+
+    Parameters
+    ----------
+    patch : HIPPatchMixin patch, optional default patch
+    config : int, kernel configuration (``fpx_config_info``); default
+        DEFAULT_CONFIG
+    """
+
+    def __init__(self, patch=None, config=None, chi_dtype=np.float64):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("HIPProposer needs a ROCm device")
+        self.patch = patch
+        self.chi_dtype = chi_dtype
+        self.config = DEFAULT_CONFIG if config is None else int(config)
+        L = lib()
+        self.waves, self.ppl, self.pix_per_tile = (np.zeros(1, np.int32) for _ in range(3))
+        check(L.fpx_config_info(self.config, self.waves.ctypes.data, self.ppl.ctypes.data,
+                                self.pix_per_tile.ctypes.data), "fpx_config_info")
+        self.pix_per_tile = int(self.pix_per_tile[0])
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        dev = self.device
+        n_acc = L.fpx_accum_doubles(MAXBANDS)
+        self._accum = torch.zeros(n_acc, dtype=torch.float64, device=dev)
+        self._ticket = torch.zeros(2, dtype=torch.int32, device=dev)
+        self._chi2 = torch.zeros(MAXBANDS, dtype=torch.float64, device=dev)
+        self._resp = torch.zeros(MAXBANDS * MAXSOURCES * NPARAMS, dtype=torch.float32, device=dev)
+        self._prop = torch.zeros(MAXSOURCES * source_struct_dtype.itemsize, dtype=torch.uint8,
+                                 device=dev)
+        self._slot = torch.zeros(SLOT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+        pin = dict(pin_memory=True)
+        self._prop_h = torch.zeros(self._prop.numel(), dtype=torch.uint8, **pin)
+        self._chi2_h = torch.zeros(MAXBANDS, dtype=torch.float64, **pin)
+        self._resp_h = torch.zeros(self._resp.numel(), dtype=torch.float32, **pin)
+        self._slot_key = None
+
+    def _stream(self):
+        import torch
+        return torch.cuda.current_stream(self.device)
+
+    def _slot_for(self, patch, pixel_weight=None):
+        """(Re)build the device FpxSlot when the patch struct or tiles change."""
+        tiles, n_tiles = patch.device_tiles(self.pix_per_tile)
+        pw = 0 if pixel_weight is None else pixel_weight.data_ptr()
+        key = (patch.device_patch.data_ptr(), tiles.data_ptr(), n_tiles, int(patch.n_bands), pw)
+        if key != self._slot_key:
+            slot = np.zeros(1, dtype=SLOT_DTYPE)
+            slot["patch"] = patch.device_patch.data_ptr()
+            slot["proposal"] = self._prop.data_ptr()
+            slot["tiles"] = tiles.data_ptr()
+            slot["pixel_weight"] = pw
+            slot["pchi2"] = self._chi2.data_ptr()
+            slot["pdchi2_dp"] = self._resp.data_ptr()
+            slot["accum"] = self._accum.data_ptr()
+            slot["ticket"] = self._ticket.data_ptr()
+            slot["n_bands"] = int(patch.n_bands)
+            slot["n_tiles"] = n_tiles
+            import torch
+            self._slot.copy_(torch.from_numpy(slot.view(np.uint8).reshape(-1).copy()))
+            self._slot_key = key
+        return n_tiles
+
+    def launch(self, proposal, patch, pixel_weight=None):
+        """Queue one evaluation on the current stream (no synchronisation).
+        ``proposal`` is a numpy source_struct_dtype array or a device uint8
+        tensor already holding the records."""
+        import torch
+        n_src = int(patch.n_sources)
+        if n_src > MAXSOURCES or int(patch.n_bands) > MAXBANDS:
+            raise ValueError("patch exceeds MAXSOURCES/MAXBANDS")
+        if isinstance(proposal, np.ndarray):
+            if len(proposal) != n_src:
+                raise ValueError(f"proposal has {len(proposal)} sources, patch has {n_src}")
+            raw = np.ascontiguousarray(proposal, dtype=source_struct_dtype).view(np.uint8)
+            self._prop_h[:raw.size].copy_(torch.from_numpy(raw.reshape(-1)))
+            self._prop[:raw.size].copy_(self._prop_h[:raw.size], non_blocking=True)
+        else:
+            self._prop[:proposal.numel()].copy_(proposal, non_blocking=True)
+        n_tiles = self._slot_for(patch, pixel_weight)
+        stream = self._stream()
+        check(lib().fpx_evaluate_proposal(self._slot.data_ptr(), n_tiles, patch.max_gauss(),
+                                          self.config, stream.cuda_stream),
+              "fpx_evaluate_proposal")
+        return n_tiles
+
+    def evaluate_proposal(self, proposal, patch=None, verbose=False, unpack=True):
+        """Evaluate chi2 and dchi2/dparam of ``proposal`` on ``patch``."""
+        if patch is None:
+            patch = self.patch
+        n_bands, n_sources = int(patch.n_bands), int(patch.n_sources)
+        if verbose:
+            print(f"fpx launch: config {self.config}, {n_bands} bands, {n_sources} sources",
+                  file=sys.stderr, flush=True)
+        self.launch(proposal, patch)
+        stream = self._stream()
+        self._chi2_h[:n_bands].copy_(self._chi2[:n_bands], non_blocking=True)
+        nr = n_bands * MAXSOURCES * NPARAMS
+        self._resp_h[:nr].copy_(self._resp[:nr], non_blocking=True)
+        stream.synchronize()
+        chi_out = self._chi2_h[:n_bands].numpy().astype(self.chi_dtype, copy=True)
+        derivs = self._resp_h[:nr].numpy().reshape(n_bands, MAXSOURCES, NPARAMS)
+        chi2_derivs = derivs[:, :n_sources].astype(np.float64)
+        if patch.return_residual:
+            residuals = patch.retrieve_array("residual")
+            if unpack:
+                residuals = self.unpack_residuals(residuals, patch)
+            return chi_out, chi2_derivs, residuals
+        return chi_out, chi2_derivs
+
+
+# forcepho's names for the device proposer (forcepho/proposal.py:55,155)
+GPUProposer = HIPProposer
+Proposer = HIPProposer

@@ -1,0 +1,239 @@
+/* forcepho_amd.h -- C ABI of the MI355X chi2 + gradient library (libfpx.so).
+ *
+ * Everything here is plain C: pointers, fixed-width integers and sizes.  The
+ * Python host code binds it with ctypes (forcepho_amd/_lib.py); any other
+ * language can bind it the same way.
+ *
+ * Device pointers are HIP device addresses (hipMalloc / torch tensors on a
+ * ROCm device).  Host pointers are marked "host".  Every entry point returns
+ * 0 on success and a negative FPX_E* code on failure; fpx_last_error() then
+ * describes the failure (thread-local, valid until the next call on that
+ * thread).
+ */
+#ifndef FORCEPHO_AMD_H
+#define FORCEPHO_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "fpx_kernel_limits.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FPX_ABI_VERSION 1
+
+/* ------------------------------------------------------------------------ */
+/* Wire structs.  Byte layouts equal the reference's (checked by           */
+/* static_assert in the library and against numpy dtypes in tests).        */
+/* ------------------------------------------------------------------------ */
+
+/* One PSF Gaussian, already tagged with the Sersic radius bin it convolves.
+ * Replaces PSFSourceGaussian (forcepho/src/patch.cc:211-227); field order is
+ * storage.PSF_COLS + sersic_bin (forcepho/patches/storage.py:27-28). 28 bytes. */
+typedef struct FpxPSFSourceGaussian {
+    float amp;
+    float xcen, ycen;
+    float Cxx, Cyy, Cxy;
+    int32_t sersic_radius_bin;
+} FpxPSFSourceGaussian;
+
+/* Pixel data and per-exposure metadata of one patch, all arrays device
+ * resident.  Replaces Patch (forcepho/src/patch.cc:131-208; numpy dtype
+ * forcepho/patches/patch.py:44-73). 152 bytes.
+ *   data, ierr, xpix, ypix, residual : float[npix], exposures concatenated in
+ *       band order, each exposure padded to a multiple of FPX_WAVE pixels;
+ *       residual may be NULL (then no residual image is written)
+ *   exposure_start, exposure_N : int32[n_exp]
+ *   band_start, band_N          : int16[n_bands]  (indices into exposures)
+ *   rad2                        : float[n_radii]   (arcsec^2)
+ *   D, CW                       : float[n_exp][n_sources][2][2]
+ *   crpix, crval                : float[n_exp][n_sources][2]
+ *   G                           : float[n_exp]
+ *   n_psf_per_source            : int32[n_bands]
+ *   psfgauss                    : FpxPSFSourceGaussian[...]
+ *   psfgauss_start              : int32[n_exp]
+ * As in the reference, every source of exposure e uses the PSF block that
+ * starts at psfgauss[psfgauss_start[e]]. */
+typedef struct FpxPatch {
+    float *data;
+    float *ierr;
+    float *xpix;
+    float *ypix;
+    float *residual;
+    int32_t *exposure_start;
+    int32_t *exposure_N;
+    int16_t *band_start;
+    int16_t *band_N;
+    int32_t n_sources;
+    int32_t n_radii;
+    float *rad2;
+    float *D;
+    float *crpix;
+    float *crval;
+    float *CW;
+    float *G;
+    int32_t *n_psf_per_source;
+    FpxPSFSourceGaussian *psfgauss;
+    int32_t *psfgauss_start;
+} FpxPatch;
+
+/* On-sky parameters of one Sersic source.  Replaces Source
+ * (forcepho/src/proposal.cu:13-27; numpy dtype forcepho/proposal.py:229-240).
+ * 264 bytes.  ra/dec are degrees relative to the patch reference
+ * coordinates, pa radians, rh arcsec. */
+typedef struct FpxSource {
+    float ra, dec;
+    float q, pa;
+    float nsersic, rh;
+    float fluxes[MAXBANDS];
+    float mixture_amplitudes[MAXRADII];
+    float damplitude_drh[MAXRADII];
+    float damplitude_dnsersic[MAXRADII];
+} FpxSource;
+
+/* dchi2/dparam of one band: [MAXSOURCES][NPARAMS] with parameter order
+ * (flux, ra, dec, q, pa, sersic, rh).  Replaces Response
+ * (forcepho/src/proposal.cu:33-39). 840 bytes. */
+typedef struct FpxResponse {
+    float dchi2_dparam[MAXSOURCES * NPARAMS];
+} FpxResponse;
+
+/* One unit of kernel work: a run of at most pixels-per-tile consecutive
+ * pixels of one exposure.  Built on the host by fpx_plan_tiles.  16 bytes. */
+typedef struct FpxTile {
+    int32_t exposure;
+    int32_t band;
+    int32_t pix_start;   /* absolute index into the patch pixel arrays */
+    int32_t pix_count;
+} FpxTile;
+
+/* One evaluation in a launch: a patch, a proposal for it and where the
+ * results go.  An array of these lives in device memory; a launch may hold
+ * many (many patches, many HMC chains on one patch, ...).  80 bytes.
+ *   accum  : double[fpx_accum_doubles(n_bands)], zero before the first launch;
+ *            the kernel leaves it zero again after each launch
+ *   ticket : uint32, zero before the first launch; left zero after each launch
+ *   pixel_weight : NULL for chi2 mode.  Non-NULL selects vector-Jacobian mode:
+ *            no model pass, pchi2 is written as 0 and pdchi2_dp receives
+ *            sum_pix pixel_weight[pix] * d model[pix] / d param.
+ *   flags  : reserved, 0 */
+typedef struct FpxSlot {
+    const FpxPatch *patch;
+    const FpxSource *proposal;
+    const FpxTile *tiles;
+    const float *pixel_weight;
+    double *pchi2;        /* device double[n_bands] */
+    float *pdchi2_dp;     /* device FpxResponse[n_bands] */
+    double *accum;
+    uint32_t *ticket;
+    int32_t n_bands;
+    int32_t n_tiles;
+    int32_t flags;
+    int32_t reserved;
+} FpxSlot;
+
+/* Error codes. */
+#define FPX_OK 0
+#define FPX_E_ARG -1      /* invalid argument or limit exceeded */
+#define FPX_E_HIP -2      /* HIP runtime error */
+#define FPX_E_NODEV -3    /* no usable gfx950 device */
+
+/* Struct ids for fpx_struct_size. */
+#define FPX_STRUCT_PATCH 0
+#define FPX_STRUCT_SOURCE 1
+#define FPX_STRUCT_RESPONSE 2
+#define FPX_STRUCT_PSFGAUSS 3
+#define FPX_STRUCT_TILE 4
+#define FPX_STRUCT_SLOT 5
+
+/* ------------------------------------------------------------------------ */
+/* Host-only helpers (no GPU needed).                                       */
+/* ------------------------------------------------------------------------ */
+
+int fpx_abi_version(void);
+const char *fpx_last_error(void);
+
+/* sizeof() of the wire structs, for layout checks against the host language.
+ * Returns 0 for an unknown id. */
+size_t fpx_struct_size(int which);
+
+/* Doubles of accumulator space one FpxSlot needs for n_bands bands. */
+size_t fpx_accum_doubles(int32_t n_bands);
+
+/* Split every exposure of a packed patch into tiles of at most pix_per_tile
+ * pixels (pix_per_tile a multiple of FPX_WAVE).  Inputs are the host copies of
+ * the patch index arrays.  Writes at most max_tiles tiles to tiles_out (host)
+ * and returns the number of tiles (>= 0), or FPX_E_ARG.  Passing
+ * tiles_out == NULL returns the count only. */
+int fpx_plan_tiles(const int32_t *exposure_start, const int32_t *exposure_N,
+                   const int16_t *band_start, const int16_t *band_N,
+                   int32_t n_bands, int32_t pix_per_tile,
+                   FpxTile *tiles_out, int32_t max_tiles);
+
+/* Kernel configurations (tile shape).  cfg in [0, fpx_n_configs()). */
+int fpx_n_configs(void);
+int fpx_config_info(int32_t cfg, int32_t *waves_per_group,
+                    int32_t *pixels_per_lane, int32_t *pixels_per_tile);
+
+/* Bytes of LDS one workgroup of configuration cfg uses when the largest
+ * exposure of a launch holds max_gauss ImageGaussians (n_sources x
+ * n_psf_per_source).  Returns FPX_E_ARG if cfg is invalid. */
+int fpx_lds_bytes(int32_t cfg, int32_t max_gauss, size_t *bytes);
+
+/* ------------------------------------------------------------------------ */
+/* Device entry points.  stream is a hipStream_t (NULL = default stream).   */
+/* All launches are asynchronous with respect to the host.                  */
+/* ------------------------------------------------------------------------ */
+
+/* Select and check the device (must be gfx950).  Optional: every device
+ * entry point checks the current device on first use. */
+int fpx_init(int32_t device);
+
+/* Compute units, max LDS bytes per workgroup and peak engine clock (kHz) of
+ * the current device. */
+int fpx_device_info(int32_t *n_cu, int32_t *lds_per_group, int32_t *clock_khz);
+
+/* chi2 and dchi2/dparam for one proposal on one patch.
+ * Replaces the PyCUDA launch of EvaluateProposal<<<n_bands, 1024, 48000>>>
+ * (forcepho/proposal.py:133-136, forcepho/src/compute_gaussians_kernel.cu:213-312)
+ * and, per band, the C++ EvaluateProposal(band, patch, proposal)
+ * (forcepho/src/compute_gaussians_kernel.cc:227-315).
+ *   slot      : device pointer to one FpxSlot (its tiles built with the
+ *               pixels_per_tile of cfg)
+ *   n_tiles   : the slot's n_tiles (host copy, sizes the grid)
+ *   max_gauss : n_sources x max over bands of n_psf_per_source (host copy)
+ * Results: pchi2[b] (double, sum over the band's pixels of
+ * (r*ierr - d*ierr)*(r*ierr + d*ierr)), pdchi2_dp[b] (float Response) and,
+ * when patch->residual != NULL, residual[pix] = data - model. */
+int fpx_evaluate_proposal(const FpxSlot *slot, int32_t n_tiles,
+                          int32_t max_gauss, int32_t cfg, void *stream);
+
+/* Many evaluations in one launch (uneven patches, many chains).
+ *   slots        : device FpxSlot[n_slots]
+ *   item_offsets : device int32[n_slots + 1], prefix sums of the slots'
+ *                  n_tiles (item_offsets[0] = 0); may be NULL if n_slots == 1
+ *   n_items      : item_offsets[n_slots] (host copy)
+ *   max_gauss    : max over slots of n_sources x max n_psf_per_source
+ *   persistent   : 0 = one workgroup per tile; 1 = a resident grid that
+ *                  pulls tiles from the queue counter
+ *   queue        : device uint32[2], zero before the first persistent launch
+ *                  and left zero after each (ignored when persistent == 0) */
+int fpx_evaluate_batch(const FpxSlot *slots, int32_t n_slots,
+                       const int32_t *item_offsets, int32_t n_items,
+                       int32_t max_gauss, int32_t cfg, int32_t persistent,
+                       uint32_t *queue, void *stream);
+
+/* Throughput probe of the VALU pipes used by the hot loop: kind 0 = FP32
+ * FMA, kind 1 = v_exp_f32.  Launches a fixed-size kernel writing one float
+ * per thread to out (device, n_threads floats); the caller times it.
+ * Returns the operations the launch performs in *ops. */
+int fpx_probe_valu(int32_t kind, int32_t iters, float *out, int32_t n_threads,
+                   double *ops, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FORCEPHO_AMD_H */

@@ -1,0 +1,54 @@
+"""Shared helpers of the test suite: golden fixtures and tolerances."""
+
+import glob
+import os
+
+import numpy as np
+
+from forcepho_amd.patches import PatchBase, HIPPatchMixin, PSF_DTYPE
+from forcepho_amd.proposal import source_struct_dtype
+
+GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# tolerance of the north-star parity bar
+CHI2_RTOL = 1e-6     # chi2 relative to the reference CPU kernel
+GRAD_RTOL = 1e-4     # gradient relative error ...
+GRAD_FLOOR = 1e-6    # ... above a floor of this fraction of the summed magnitude
+
+
+class GoldenPatch(PatchBase, HIPPatchMixin):
+    """A packed patch loaded from a fixture."""
+
+    def __init__(self, npz, return_residual=False):
+        super().__init__(return_residual=return_residual)
+        for name in ["xpix", "ypix", "data", "ierr", "exposure_start", "exposure_N",
+                     "band_start", "band_N", "rad2", "D", "crpix", "crval", "CW", "G",
+                     "n_psf_per_source", "psfgauss_start", "original_sizes"]:
+            setattr(self, name, np.array(npz[name]))
+        self.psfgauss = np.array(npz["psfgauss"]).view(PSF_DTYPE).copy()
+        self.n_sources = int(npz["n_sources"])
+        self.n_radii = int(npz["n_radii"])
+        self.n_bands = int(npz["n_bands"])
+        self.n_exp = len(self.exposure_N)
+
+
+def golden_names():
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN_DIR, "*.npz")))
+
+
+def load_golden(name, return_residual=False):
+    """(patch, proposal, expected dict) of fixture ``name``."""
+    npz = np.load(os.path.join(GOLDEN_DIR, name + ".npz"))
+    patch = GoldenPatch(npz, return_residual=return_residual)
+    proposal = np.array(npz["proposal"]).view(source_struct_dtype).copy()
+    expected = {k: np.array(npz[k]) for k in ("chi2", "dchi2", "residual")}
+    return patch, proposal, expected
+
+
+def grad_errors(g, g64, mass):
+    """Per-entry error of a gradient against the float64 value, in units of
+    its tolerance GRAD_RTOL |g64| + GRAD_FLOOR mass (<= 1 passes)."""
+    g = np.asarray(g, dtype=np.float64)
+    tol = GRAD_RTOL * np.abs(g64) + GRAD_FLOOR * mass
+    tol = np.where(tol > 0, tol, 1e-300)
+    return np.abs(g - g64) / tol

@@ -1,0 +1,94 @@
+"""Parity of the HIP kernel with the reference CPU kernel (golden fixtures),
+the C restatement and the float64 oracle.  Calls go through the C ABI."""
+
+import numpy as np
+import pytest
+
+import fpx_testutil as U
+import oracle
+import oracle.fp64 as fp64
+
+pytestmark = pytest.mark.gpu
+
+NAMES = U.golden_names()
+
+
+@pytest.fixture(scope="module")
+def fp64_cache():
+    return {}
+
+
+def _fp64(name, patch, proposal, cache):
+    if name not in cache:
+        cache[name] = fp64.evaluate(patch, proposal, mass=True)
+    return cache[name]
+
+
+def _evaluate(name, config=None, return_residual=True):
+    from forcepho_amd.proposal import HIPProposer
+    patch, proposal, expected = U.load_golden(name, return_residual=return_residual)
+    patch.send_to_device()
+    prop = HIPProposer(patch=patch, config=config)
+    out = prop.evaluate_proposal(proposal, patch=patch, unpack=False)
+    return patch, proposal, expected, out
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_fixture_parity(name, fp64_cache):
+    print(f"[gpu] fixture {name}", flush=True)
+    patch, proposal, expected, (chi2, dchi2, resid) = _evaluate(name)
+    # chi2 against the reference CPU kernel
+    rel = np.abs(chi2 - expected["chi2"]) / np.maximum(np.abs(expected["chi2"]), 1e-300)
+    assert np.all(rel <= U.CHI2_RTOL), (name, rel)
+    # gradients against the float64 oracle
+    c64, g64, r64, mass = _fp64(name, patch, proposal, fp64_cache)
+    err = U.grad_errors(dchi2, g64, mass)
+    assert err.max() <= 1.0, (name, err.max(), np.unravel_index(err.argmax(), err.shape))
+    # and against the reference CPU kernel, whose own float32 error is included
+    err_ref = U.grad_errors(expected["dchi2"], g64, mass)
+    scale = U.GRAD_RTOL * np.abs(g64) + U.GRAD_FLOOR * mass
+    both = np.abs(dchi2 - expected["dchi2"]) / np.where(scale > 0, scale, 1e-300)
+    assert np.all(both <= err + err_ref + 1e-9), name
+    # residual image against the reference, relative to the data scale
+    dscale = np.max(np.abs(patch.data)) + 1e-30
+    assert np.max(np.abs(resid - expected["residual"])) <= 2e-6 * dscale, name
+
+
+@pytest.mark.parametrize("config", range(8))
+def test_all_configs_agree(config):
+    name = "multi_small_s2"
+    print(f"[gpu] config {config} on {name}", flush=True)
+    patch, proposal, expected, (chi2, dchi2, resid) = _evaluate(name, config=config)
+    c64, g64, r64, mass = fp64.evaluate(patch, proposal, mass=True)
+    rel = np.abs(chi2 - expected["chi2"]) / np.abs(expected["chi2"])
+    assert rel.max() <= U.CHI2_RTOL
+    assert U.grad_errors(dchi2, g64, mass).max() <= 1.0
+
+
+def test_repeatable_and_no_residual():
+    """Two evaluations agree; without a residual buffer the same chi2 and
+    gradients come back (verify_reference.py:133-134 repeatability)."""
+    name = "uneven_s6"
+    print(f"[gpu] repeat {name}", flush=True)
+    from forcepho_amd.proposal import HIPProposer
+    patch, proposal, expected = U.load_golden(name, return_residual=False)
+    patch.send_to_device()
+    prop = HIPProposer(patch=patch)
+    a = prop.evaluate_proposal(proposal, patch=patch)
+    b = prop.evaluate_proposal(proposal, patch=patch)
+    assert len(a) == 2
+    np.testing.assert_allclose(a[0], b[0], rtol=1e-14)
+    np.testing.assert_allclose(a[1], b[1], rtol=1e-6, atol=1e-6 * np.abs(a[1]).max())
+    rel = np.abs(a[0] - expected["chi2"]) / np.abs(expected["chi2"])
+    assert rel.max() <= U.CHI2_RTOL
+
+
+def test_chi2_convention():
+    """sum (r^2 - d^2) ierr^2 == chi2 (verify_reference.py:141-143) using the
+    kernel's own residual image."""
+    patch, proposal, expected, (chi2, dchi2, resid) = _evaluate("padding_s5")
+    r = resid.astype(np.float64) * patch.ierr
+    d = patch.data.astype(np.float64) * patch.ierr
+    per_pix = (r - d) * (r + d)
+    bands = np.split(per_pix, np.cumsum(patch.band_N_pix[:-1]))
+    np.testing.assert_allclose(chi2, [x.sum() for x in bands], rtol=1e-6)
