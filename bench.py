@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--config", type=int, default=None, help="kernel config (default: tuned)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="also run one device copy of this many MiB (counter calibration)")
     return ap.parse_args()
 
 
@@ -65,14 +67,16 @@ def time_config(prop, patch, proposal_dev, n):
 
 
 def valu_probe():
-    """Measured FP32 FMA and v_exp_f32 issue rates of the device (ops/s)."""
+    """Measured FP32 FMA and v_exp_f32 rates of the device: ops/s over the
+    launch, cycles per wavefront instruction and the shader clock the
+    wavefronts saw (s_memtime against the 100 MHz s_memrealtime)."""
     import torch
     from forcepho_amd._lib import lib, check
     import ctypes
     n_threads = 256 * 256 * 8
     out = torch.empty(n_threads, dtype=torch.float32, device="cuda")
     res = {}
-    for kind, name, iters in ((0, "fma_per_s", 4096), (1, "exp_per_s", 1024)):
+    for kind, name, iters in ((0, "fma", 8192), (1, "exp", 2048)):
         ops = ctypes.c_double()
         s = torch.cuda.current_stream()
         check(lib().fpx_probe_valu(kind, iters, out.data_ptr(), n_threads, ctypes.byref(ops),
@@ -84,7 +88,11 @@ def valu_probe():
                                    s.cuda_stream), "probe")
         b.record(s)
         torch.cuda.synchronize()
-        res[name] = ops.value / (a.elapsed_time(b) * 1e-3)
+        v = out.view(-1, 64).cpu().numpy()
+        cyc, ref = np.median(v[:, 0]), np.median(v[:, 1])
+        res[name + "_per_s"] = ops.value / (a.elapsed_time(b) * 1e-3)
+        res[name + "_cycles_per_wave_inst"] = float(cyc / (iters * 8))
+        res[name + "_clock_ghz"] = float(cyc / ref * 0.1)
     return res
 
 
@@ -143,6 +151,14 @@ def main():
     raw = np.ascontiguousarray(proposal).view(np.uint8).reshape(-1)
     pdev = torch.from_numpy(raw.copy()).cuda()
 
+    if args.copy_mb:
+        # a plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE
+        # counters calibrate the chi2 kernel's
+        a = torch.ones(args.copy_mb * 2 ** 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     for _ in range(args.warmup):
         prop.launch(pdev, patch)
     torch.cuda.synchronize()

@@ -1,0 +1,119 @@
+"""torch.autograd access to the patch model.
+
+``PatchResidual`` maps the scene parameter vector theta
+[flux_1..flux_B, ra, dec, q, pa, sersic, rhalf] x n_sources (the order of
+forcepho's Scene, forcepho/sources.py:46-97) to the residual image
+data - model(theta) over the packed pixels.  Its backward pass is the
+kernel's gradient pass alone (FpxSlot.pixel_weight mode): given dL/dresidual
+it recomputes every ImageGaussian and its moments from the proposal and
+returns dL/dtheta, never storing per-pixel intermediates.  The theano Ops
+of forcepho/model.py:515-551 played the same role for pymc3.
+
+``PatchModel.chi2(theta)`` builds the reference's chi2 convention
+sum (r ierr)^2 - (d ierr)^2 (compute_gaussians_kernel.cc:286-293) on top,
+so ``-0.5 * chi2`` is forcepho's log-likelihood and its autograd gradient
+matches FastPosterior.lnprob_grad in the untransformed space.
+"""
+
+import numpy as np
+
+from .model import stack_grad
+
+__all__ = ["PatchModel", "PatchResidual"]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class PatchModel:
+    """A device-resident patch, its scene and a proposer, for autograd use.
+
+    Parameters
+    ----------
+    patch : HIPPatchMixin patch (packed; sent to the device here with a
+            residual buffer)
+    scene : Scene whose active sources map onto the patch's sources
+    proposer : HIPProposer (created if None)
+    """
+
+    def __init__(self, patch, scene, proposer=None):
+        torch = _torch()
+        from .proposal import HIPProposer
+        self.patch, self.scene = patch, scene
+        patch.return_residual = True
+        if getattr(patch, "device_patch", None) is None or "residual" not in getattr(
+                patch, "device_ptrs", {}):
+            patch.send_to_device()
+        self.proposer = proposer or HIPProposer(patch=patch)
+        dev = self.proposer.device
+        self.data = torch.from_numpy(np.asarray(patch.data, np.float64)).to(dev)
+        self.ierr = torch.from_numpy(np.asarray(patch.ierr, np.float64)).to(dev)
+        self.n_sources, self.n_bands = int(patch.n_sources), int(patch.n_bands)
+        self.ndim = self.n_sources * (self.n_bands + 6)
+
+    def proposal(self, theta):
+        self.scene.set_all_source_params(np.asarray(theta, dtype=np.float64))
+        return self.scene.get_proposal()
+
+    def forward_residual(self, proposal):
+        """Launch the chi2 kernel for ``proposal``; returns (residual float32
+        device tensor [npix] (a copy), chi2 per band, dchi2 [n_bands, n_src, 7])."""
+        torch = _torch()
+        p = self.proposer
+        p.launch(proposal, self.patch)
+        resid = self.patch.device_ptrs["residual"].view(torch.float32)[:self.patch.npix].clone()
+        nb = self.n_bands
+        chi2 = p._chi2[:nb].cpu().numpy()
+        resp = p._resp[:nb * 210].view(nb, 30, 7)[:, :self.n_sources].cpu().numpy()
+        return resid, chi2, resp.astype(np.float64)
+
+    def residual(self, theta):
+        """Differentiable residual image; theta: float64 tensor [ndim]."""
+        return PatchResidual.apply(theta, self)
+
+    def chi2(self, theta):
+        """Differentiable sum over pixels of (r ierr)^2 - (d ierr)^2 (float64)."""
+        r = self.residual(theta).double()
+        return ((r * self.ierr) ** 2 - (self.data * self.ierr) ** 2).sum()
+
+
+class PatchResidual:
+    """Namespace holding the autograd.Function (created lazily so that this
+    module imports without torch)."""
+
+    _fn = None
+
+    @classmethod
+    def apply(cls, theta, model):
+        if cls._fn is None:
+            cls._fn = _make_function()
+        return cls._fn.apply(theta, model)
+
+
+def _make_function():
+    torch = _torch()
+
+    class _Residual(torch.autograd.Function):
+
+        @staticmethod
+        def forward(ctx, theta, model):
+            proposal = model.proposal(theta.detach().cpu().numpy())
+            resid, _, _ = model.forward_residual(proposal)
+            ctx.model = model
+            ctx.proposal = proposal
+            ctx.theta_meta = (theta.dtype, theta.device)
+            return resid
+
+        @staticmethod
+        def backward(ctx, grad_resid):
+            model = ctx.model
+            # residual = data - model, so dL/dtheta = -sum_pix g * dmodel/dtheta
+            pw = (-grad_resid).to(torch.float32).contiguous()
+            d = model.proposer.vjp(ctx.proposal, pw, model.patch)
+            g = stack_grad(d, model.n_sources, model.n_bands)
+            dtype, device = ctx.theta_meta
+            return torch.from_numpy(g).to(dtype=dtype, device=device), None
+
+    return _Residual

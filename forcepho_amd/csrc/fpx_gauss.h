@@ -11,15 +11,22 @@
 //   amp = G a_psf sqrt(det F) / (2 pi) * flux_band * a_mix[bin]
 //   dF/dq = -F dSigma/dq F with dSigma/dq = rad2 (T dT^T + dT T^T)
 //   dA/dq = amp/2 tr(Sigma dF/dq)   (same for pa)
-// The record is laid out for the two pixel passes (fpx_kernel.hip):
-//   [0..7]   xc yc fxx fxy fyy a0 a1 h0        model pass (and geometry)
-//   [8..23]  amp amp/12 dA/dflux dA/dsersic dA/drh
+// The record is laid out for the two pixel passes (fpx_kernel.hip), 24
+// floats (6 float4):
+//   [0..3]   xc yc hx hy      centre and the half-widths of the box that holds
+//                             the whole dx^T F dx <= MAX_EXP_ARG ellipse
+//   [4..8]   fxx fxy fyy h0 amp                       model pass
+//   [9..22]  amp/12 dA/dflux dA/dsersic dA/drh
 //            dA/dq kq dFxx/dq dFyy/dq dFxy/dq
-//            dA/dpa kp dFxx/dpa dFyy/dpa dFxy/dpa  pad
-// with h0 = 1 - (fxx + fyy)/24, a0 = amp h0, a1 = amp/24 and
-// kq = -(dFxx/dq + dFyy/dq)/2 (kp likewise), so that the pixel integral
-// factor H = 1 + (vx^2 + vy^2 - fxx - fyy)/24 of common_kernel.cc:77 is
-// H = h0 + (vx^2 + vy^2)/24.
+//            dA/dpa kp dFxx/dpa dFyy/dpa dFxy/dpa    gradient pass
+//   [23]     pad
+// with h0 = 1 - (fxx + fyy)/24 and kq = -(dFxx/dq + dFyy/dq)/2 (kp
+// likewise), so that the pixel integral factor
+// H = 1 + (vx^2 + vy^2 - fxx - fyy)/24 of common_kernel.cc:77 is
+// H = h0 + (vx^2 + vy^2)/24.  The ellipse dx^T Sigma^-1 dx <= 36 spans
+// +-6 sqrt(Sigma_xx) in x (sqrt(Sigma_yy) in y); hx, hy add a relative
+// 1e-3 and an absolute 0.01 px margin so that no pixel the float32 test
+// would keep lies outside the box.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,10 +37,12 @@ namespace fpx {
 constexpr int kRec = 24;  // floats per ImageGaussian record
 
 enum RecField {
-  kXc = 0, kYc, kFxx, kFxy, kFyy, kA0, kA1, kH0,
-  kAmp, kAmp12, kDAF, kDAN, kDAR,
+  kXc = 0, kYc, kHx, kHy,
+  kFxx, kFxy, kFyy, kH0, kAmp,
+  kAmp12, kDAF, kDAN, kDAR,
   kDAQ, kKQ, kQxx, kQyy, kQxy,
-  kDAP, kKP, kPxx, kPyy, kPxy, kPad
+  kDAP, kKP, kPxx, kPyy, kPxy,
+  kPad
 };
 
 struct M22 {
@@ -129,11 +138,12 @@ __device__ __forceinline__ void build_record(const FpxPatch *__restrict__ P,
   const float h0 = 1.0f - (fxx + fyy) * (1.0f / 24.0f);
   out[kXc] = xs + psf.xcen;
   out[kYc] = ys + psf.ycen;
+  const float r_max = sqrtf((float)MAX_EXP_ARG);
+  out[kHx] = r_max * sqrtf(fmaxf(Sxx, 0.0f)) * 1.001f + 0.01f;
+  out[kHy] = r_max * sqrtf(fmaxf(Syy, 0.0f)) * 1.001f + 0.01f;
   out[kFxx] = fxx;
   out[kFxy] = fxy;
   out[kFyy] = fyy;
-  out[kA0] = amp * h0;
-  out[kA1] = amp * (1.0f / 24.0f);
   out[kH0] = h0;
   out[kAmp] = amp;
   out[kAmp12] = amp * (1.0f / 12.0f);

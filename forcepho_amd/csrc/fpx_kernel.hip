@@ -26,8 +26,11 @@
 //             slot (ticket counter) writes pchi2 / Response and re-zeroes the
 //             accumulator.
 // A Gaussian is skipped at a pixel when dx^T F dx > MAX_EXP_ARG
-// (common_kernel.cc:73,108); when that holds for every pixel of a wavefront
-// the whole Gaussian is skipped with a wave-uniform branch.
+// (common_kernel.cc:73,108).  Two wave-uniform shortcuts skip work that this
+// test would discard anyway: a Gaussian whose MAX_EXP_ARG box misses the
+// wavefront's pixel bounding box is not evaluated at all, and one whose
+// exponent exceeds the limit at every pixel of the wavefront stops after the
+// exponent.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -79,13 +82,55 @@ __device__ __forceinline__ bool wave_any(bool p) {
   return __builtin_amdgcn_ballot_w64(p) != 0;
 }
 
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ __forceinline__ float uniform_f(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
+// Pixel bounding box of a wavefront: [x0, x1] x [y0, y1] (empty when the
+// wavefront holds no valid pixel: x0 > x1).
+struct Box {
+  float x0, x1, y0, y1;
+};
+
+// True when the Gaussian's MAX_EXP_ARG box (centre r0.xy, half-widths r0.zw)
+// can reach some pixel of the box.  Wave-uniform.
+__device__ __forceinline__ bool box_hits(const float4 &r0, const Box &b) {
+  const float gx = fmaxf(fmaxf(b.x0 - r0.x, r0.x - b.x1), 0.0f);
+  const float gy = fmaxf(fmaxf(b.y0 - r0.y, r0.y - b.y1), 0.0f);
+  return wave_any(gx <= r0.z && gy <= r0.w);
+}
+
 // LDS size (bytes) of one workgroup; must agree with the carve-up in
 // process_tile.
+// Per-wave accumulator row: chi2 (double) then MAXSOURCES*NPARAMS gradient
+// sums (float; each covers one tile's pixels).
+constexpr int kRowBytes = 8 + MAXSOURCES * NPARAMS * 4;
+
 __host__ __device__ constexpr size_t lds_bytes_for(int waves, int ppl, int gauss_cap) {
   return (size_t)gauss_cap * kRec * sizeof(float)      // ImageGaussian records
          + (size_t)waves * ppl * 64 * sizeof(float)     // partial models
          + (size_t)MAXSOURCES * 4 * sizeof(float)       // CW per source
-         + (size_t)waves * kAcc * sizeof(double);       // per-wave accumulators
+         + (size_t)waves * kRowBytes;                   // per-wave accumulators
+}
+
+// Accessors of a per-wave accumulator row.
+__device__ __forceinline__ double &row_chi2(unsigned char *row) {
+  return *reinterpret_cast<double *>(row);
+}
+__device__ __forceinline__ float *row_grad(unsigned char *row) {
+  return reinterpret_cast<float *>(row + 8);
 }
 
 // Build records for sources [s0, s0 + ns) of exposure e into recs.
@@ -106,13 +151,16 @@ __device__ __forceinline__ void build_chunk(const FpxPatch *__restrict__ P,
 template <int PPL>
 __device__ __forceinline__ void model_pass(const float *__restrict__ recs, int gb,
                                            int ge, const float (&xp)[PPL],
-                                           const float (&yp)[PPL], float (&m)[PPL]) {
+                                           const float (&yp)[PPL], const Box &box,
+                                           float (&m)[PPL]) {
   const float4 *R4 = reinterpret_cast<const float4 *>(recs);
   for (int g = gb; g < ge; ++g) {
     const float4 r0 = R4[g * 6 + 0];
     const float4 r1 = R4[g * 6 + 1];
-    const float xc = r0.x, yc = r0.y, fxx = r0.z, fxy = r0.w;
-    const float fyy = r1.x, a0 = r1.y, a1 = r1.z;
+    const float amp = recs[g * kRec + kAmp];
+    if (!box_hits(r0, box)) continue;
+    const float xc = r0.x, yc = r0.y, fxx = r1.x, fxy = r1.y;
+    const float fyy = r1.z, h0 = r1.w;
     float vx[PPL], vy[PPL], ea[PPL];
     bool on[PPL];
     bool any = false;
@@ -131,7 +179,7 @@ __device__ __forceinline__ void model_pass(const float *__restrict__ recs, int g
     for (int i = 0; i < PPL; ++i) {
       const float G = on[i] ? __builtin_amdgcn_exp2f(kExp2Scale * ea[i]) : 0.0f;
       const float s2 = vx[i] * vx[i] + vy[i] * vy[i];
-      m[i] = fmaf(G, fmaf(a1, s2, a0), m[i]);
+      m[i] = fmaf(G * amp, fmaf(s2, 1.0f / 24.0f, h0), m[i]);
     }
   }
 }
@@ -142,20 +190,22 @@ template <int PPL>
 __device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int gb,
                                           int ge, int s0, int n_psf,
                                           const float (&xp)[PPL], const float (&yp)[PPL],
-                                          const float (&w2)[PPL],
+                                          const float (&w2)[PPL], const Box &box,
                                           const float *__restrict__ cwl,
-                                          double *__restrict__ wrow, int lane) {
+                                          unsigned char *__restrict__ wrow, int lane) {
   const float4 *R4 = reinterpret_cast<const float4 *>(recs);
   int g = gb;
   while (g < ge) {
     const int sl = g / n_psf;
     const int gstop = min(ge, (sl + 1) * n_psf);
     float aF = 0.f, aX = 0.f, aY = 0.f, aQ = 0.f, aP = 0.f, aN = 0.f, aR = 0.f;
+    bool touched = false;
     for (; g < gstop; ++g) {
       const float4 r0 = R4[g * 6 + 0];
       const float4 r1 = R4[g * 6 + 1];
-      const float xc = r0.x, yc = r0.y, fxx = r0.z, fxy = r0.w;
-      const float fyy = r1.x, h0 = r1.w;
+      if (!box_hits(r0, box)) continue;
+      const float xc = r0.x, yc = r0.y, fxx = r1.x, fxy = r1.y;
+      const float fyy = r1.z, h0 = r1.w;
       float dx[PPL], dy[PPL], vx[PPL], vy[PPL], px[PPL], py[PPL], ea[PPL];
       bool on[PPL];
       bool any = false;
@@ -172,6 +222,7 @@ __device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int gb
         any |= on[i];
       }
       if (!wave_any(any)) continue;
+      touched = true;
       const float4 r2 = R4[g * 6 + 2];
       const float4 r3 = R4[g * 6 + 3];
       const float4 r4 = R4[g * 6 + 4];
@@ -215,6 +266,7 @@ __device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int gb
         aP = fmaf(hC, t2p, aP);
       }
     }
+    if (!touched) continue;  // no Gaussian of this source reached the wave's pixels
     aF = wave_sum(aF);
     aX = wave_sum(aX);
     aY = wave_sum(aY);
@@ -225,14 +277,14 @@ __device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int gb
     if (lane == 0) {
       const int s = s0 + sl;
       const float *cw = cwl + 4 * s;  // [v11 v12 v21 v22] = dpix/dsky
-      double *row = wrow + 1 + NPARAMS * s;
-      row[0] += (double)aF;
-      row[1] += (double)cw[0] * aX + (double)cw[2] * aY;
-      row[2] += (double)cw[1] * aX + (double)cw[3] * aY;
-      row[3] += (double)aQ;
-      row[4] += (double)aP;
-      row[5] += (double)aN;
-      row[6] += (double)aR;
+      float *row = row_grad(wrow) + NPARAMS * s;
+      row[0] += aF;
+      row[1] += cw[0] * aX + cw[2] * aY;
+      row[2] += cw[1] * aX + cw[3] * aY;
+      row[3] += aQ;
+      row[4] += aP;
+      row[5] += aN;
+      row[6] += aR;
     }
   }
 }
@@ -250,12 +302,13 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
   float *recs = reinterpret_cast<float *>(smem);
   float *part = recs + (size_t)gauss_cap * kRec;
   float *cwl = part + W * kPix;
-  double *wacc = reinterpret_cast<double *>(cwl + MAXSOURCES * 4);
+  unsigned char *wacc = reinterpret_cast<unsigned char *>(cwl + MAXSOURCES * 4);
   const int spc = max(1, gauss_cap / n_psf);  // sources per LDS chunk
   const int n_chunks = (n_src + spc - 1) / spc;
   const bool vjp = slot.pixel_weight != nullptr;
 
-  for (int j = tid; j < W * kAcc; j += W * 64) wacc[j] = 0.0;
+  for (int j = tid; j < W * kRowBytes / 4; j += W * 64)
+    reinterpret_cast<float *>(wacc)[j] = 0.0f;
   for (int j = tid; j < 4 * n_src; j += W * 64) {
     const int s = j >> 2;
     cwl[j] = P->CW[4 * (n_src * e + s) + (j & 3)];
@@ -272,6 +325,24 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
     xp[i] = valid[i] ? P->xpix[pix[i]] : 0.0f;
     yp[i] = valid[i] ? P->ypix[pix[i]] : 0.0f;
   }
+  Box box;
+  {
+    constexpr float kInf = __builtin_huge_valf();
+    float x0 = kInf, x1 = -kInf, y0 = kInf, y1 = -kInf;
+#pragma unroll
+    for (int i = 0; i < PPL; ++i) {
+      if (valid[i]) {
+        x0 = fminf(x0, xp[i]);
+        x1 = fmaxf(x1, xp[i]);
+        y0 = fminf(y0, yp[i]);
+        y1 = fmaxf(y1, yp[i]);
+      }
+    }
+    box.x0 = uniform_f(wave_min(x0));
+    box.x1 = uniform_f(wave_max(x1));
+    box.y0 = uniform_f(wave_min(y0));
+    box.y1 = uniform_f(wave_max(y1));
+  }
 
   if (!vjp) {
     float m[PPL];
@@ -284,7 +355,7 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
       build_chunk<W>(P, props, e, band, s0, ns, n_psf, recs);
       __syncthreads();
       const int n = ns * n_psf;
-      model_pass<PPL>(recs, (n * wave) / W, (n * (wave + 1)) / W, xp, yp, m);
+      model_pass<PPL>(recs, (n * wave) / W, (n * (wave + 1)) / W, xp, yp, box, m);
     }
 #pragma unroll
     for (int i = 0; i < PPL; ++i) part[(wave * PPL + i) * 64 + lane] = m[i];
@@ -307,7 +378,7 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
     }
     if (wave == 0) {
       chi2 = wave_sum(chi2);
-      if (lane == 0) wacc[0] += chi2;
+      if (lane == 0) row_chi2(wacc) += chi2;
     }
   } else {
 #pragma unroll
@@ -319,7 +390,7 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
   for (int i = 0; i < PPL; ++i) wave_has_weight |= (w2[i] != 0.0f);
   wave_has_weight = wave_any(wave_has_weight);
 
-  double *wrow = wacc + wave * kAcc;
+  unsigned char *wrow = wacc + wave * kRowBytes;
   for (int c = 0; c < n_chunks; ++c) {
     const int s0 = c * spc;
     const int ns = min(spc, n_src - s0);
@@ -330,7 +401,7 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
     }
     const int n = ns * n_psf;
     if (wave_has_weight)
-      grad_pass<PPL>(recs, (n * wave) / W, (n * (wave + 1)) / W, s0, n_psf, xp, yp, w2,
+      grad_pass<PPL>(recs, (n * wave) / W, (n * (wave + 1)) / W, s0, n_psf, xp, yp, w2, box,
                      cwl, wrow, lane);
   }
 }
@@ -342,14 +413,17 @@ __device__ void finish_tile(const FpxSlot &slot, int band, int n_src, unsigned c
                             int gauss_cap, int ppl, int *s_flag) {
   const int tid = threadIdx.x;
   float *cwl = reinterpret_cast<float *>(smem) + (size_t)gauss_cap * kRec + W * 64 * ppl;
-  double *wacc = reinterpret_cast<double *>(cwl + MAXSOURCES * 4);
+  unsigned char *wacc = reinterpret_cast<unsigned char *>(cwl + MAXSOURCES * 4);
   if (band >= 0) {
     double *acc = slot.accum + (size_t)band * kAcc;
     const int nv = 1 + NPARAMS * n_src;
     for (int j = tid; j < nv; j += W * 64) {
       double v = 0.0;
 #pragma unroll
-      for (int w = 0; w < W; ++w) v += wacc[w * kAcc + j];
+      for (int w = 0; w < W; ++w) {
+        unsigned char *row = wacc + w * kRowBytes;
+        v += (j == 0) ? row_chi2(row) : (double)row_grad(row)[j - 1];
+      }
       if (v != 0.0) unsafeAtomicAdd(acc + j, v);
     }
   }
@@ -451,35 +525,34 @@ __global__ __launch_bounds__(W * 64) void chi2_grad_kernel(
 }
 
 // ------------------------------------------------------------------------
-// VALU throughput probes
+// VALU throughput probes: KIND 0 runs 8 independent FP32 FMA chains, KIND 1
+// 8 independent v_exp_f32 chains.  Lane 0 of every wavefront stores the
+// elapsed shader-clock cycles (s_memtime) and lane 1 the elapsed 100 MHz
+// reference ticks (s_memrealtime) of its loop; other lanes store the chain
+// sum so the loop cannot be removed.
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void probe_fma_kernel(float *out, int iters) {
-  float a[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) a[k] = (float)(threadIdx.x + k) * 1e-3f;
-  const float b = 0.999f, c = 1e-4f;
-  for (int i = 0; i < iters; ++i) {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) a[k] = fmaf(a[k], b, c);
-  }
-  float s = 0.f;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) s += a[k];
-  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
-}
-
-__global__ __launch_bounds__(256) void probe_exp_kernel(float *out, int iters) {
+template <int KIND>
+__global__ __launch_bounds__(256) void probe_kernel(float *out, int iters) {
   float a[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) a[k] = (float)((threadIdx.x + k) & 63) * (1.0f / 64.0f);
+  const float b = 0.999f, c = 1e-4f;
+  const unsigned long long t0 = __builtin_amdgcn_s_memtime();
+  const unsigned long long r0 = __builtin_amdgcn_s_memrealtime();
   for (int i = 0; i < iters; ++i) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) a[k] = __builtin_amdgcn_exp2f(-a[k]);
+    for (int k = 0; k < 8; ++k) a[k] = KIND == 0 ? fmaf(a[k], b, c) : __builtin_amdgcn_exp2f(-a[k]);
   }
   float s = 0.f;
 #pragma unroll
   for (int k = 0; k < 8; ++k) s += a[k];
-  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+  const unsigned long long t1 = __builtin_amdgcn_s_memtime();
+  const unsigned long long r1 = __builtin_amdgcn_s_memrealtime();
+  const int lane = threadIdx.x & 63;
+  float v = s;
+  if (lane == 0) v = (float)(t1 - t0);
+  if (lane == 1) v = (float)(r1 - r0);
+  out[blockIdx.x * blockDim.x + threadIdx.x] = v;
 }
 
 }  // namespace fpx
@@ -527,6 +600,10 @@ struct DeviceState {
   int lds_max = 0;
   int clock_khz = 0;
   size_t lds_set[kNumConfigs] = {};
+  // resident workgroups per CU, cached per configuration and LDS size, so that
+  // a launch recorded into a graph issues no occupancy query
+  size_t occ_lds[kNumConfigs] = {};
+  int occ_per_cu[kNumConfigs] = {};
 };
 
 std::mutex g_mu;
@@ -599,8 +676,17 @@ int launch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
   int grid = n_items;
   if (persistent) {
     int per_cu = 0;
-    FPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kc.fn, threads, lds));
-    if (per_cu < 1) per_cu = 1;
+    {
+      std::lock_guard<std::mutex> lock(g_mu);
+      if (st->occ_lds[cfg] == lds) per_cu = st->occ_per_cu[cfg];
+    }
+    if (per_cu == 0) {
+      FPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kc.fn, threads, lds));
+      if (per_cu < 1) per_cu = 1;
+      std::lock_guard<std::mutex> lock(g_mu);
+      st->occ_lds[cfg] = lds;
+      st->occ_per_cu[cfg] = per_cu;
+    }
     grid = std::min<long>((long)per_cu * st->n_cu, (long)n_items);
   }
   void *args[] = {(void *)&slots,   (void *)&n_slots, (void *)&item_offsets, (void *)&n_items,
@@ -725,9 +811,9 @@ int fpx_probe_valu(int32_t kind, int32_t iters, float *out, int32_t n_threads, d
   if (rc) return rc;
   const dim3 grid(n_threads / 256), block(256);
   if (kind == 0) {
-    hipLaunchKernelGGL(fpx::probe_fma_kernel, grid, block, 0, (hipStream_t)stream, out, iters);
+    hipLaunchKernelGGL(fpx::probe_kernel<0>, grid, block, 0, (hipStream_t)stream, out, iters);
   } else if (kind == 1) {
-    hipLaunchKernelGGL(fpx::probe_exp_kernel, grid, block, 0, (hipStream_t)stream, out, iters);
+    hipLaunchKernelGGL(fpx::probe_kernel<1>, grid, block, 0, (hipStream_t)stream, out, iters);
   } else {
     return set_err(FPX_E_ARG, "probe kind must be 0 (fma) or 1 (exp)");
   }

@@ -1,0 +1,234 @@
+"""Device-resident HMC over many chains and many patches.
+
+Every chain's position lives on the device; one leapfrog step is
+    z -> theta = lower + sigmoid(z) (upper - lower)        (BoundedTransform,
+                                                            forcepho/model.py:596-637)
+      -> Source records (fluxes, shapes, mixture amplitudes and their
+         derivatives)                                      (Galaxy.proposal,
+                                                            forcepho/sources.py:607-621)
+      -> one batched chi2 + gradient launch for all chains  (fpx_evaluate_batch)
+      -> dU/dz = 1/2 stack_grad(dchi2) dq/dz - dlndetjac/dz (FastPosterior.evaluate,
+                                                            forcepho/model.py:302-387)
+      -> momentum / position updates,
+all as device operations with no host round trip, so a whole trajectory can
+be recorded once into a graph and replayed.  The mixture amplitudes use the
+analytic LogNormalMixtureAmplitudes model (sources.py), evaluated here on the
+device for every (chain, source).  Momentum draws and the Metropolis test run
+outside the graph.
+"""
+
+import numpy as np
+
+from .kernel_limits import MAXBANDS, MAXRADII, MAXSOURCES, NPARAMS
+from .batch import BatchEvaluator
+from .sources import LogNormalMixtureAmplitudes
+
+__all__ = ["HMCEnsemble"]
+
+# float offsets of the Source record fields (include/forcepho_amd.h FpxSource)
+_F_SHAPE, _F_FLUX = 0, 6
+_F_AMP = 6 + MAXBANDS
+_F_DRH = _F_AMP + MAXRADII
+_F_DN = _F_DRH + MAXRADII
+_REC = _F_DN + MAXRADII   # 66 floats
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class HMCEnsemble:
+    """HMC chains on device-resident patches.
+
+    Parameters
+    ----------
+    patches : list of HIPPatchMixin patches (sent to the device)
+    scenes : list of Scenes (their sources' amplitude models give the radii)
+    chains : int or list, chains per patch
+    bounds : list of (lower, upper) flat parameter bounds per patch
+    """
+
+    def __init__(self, patches, scenes, chains, bounds, config=None, persistent=True):
+        torch = _torch()
+        self.batch = BatchEvaluator(patches, chains=chains, config=config,
+                                    persistent=persistent)
+        b = self.batch
+        dev = b.device
+        self.device = dev
+        self.n_slots = b.n_slots
+        self.S = max(int(p.n_sources) for p in patches)
+        self.NB = max(int(p.n_bands) for p in patches)
+        self.K = self.NB + 6
+        n = self.n_slots
+        lo = np.zeros((n, self.S, self.K))
+        hi = np.ones((n, self.S, self.K))
+        mask = np.zeros((n, self.S, self.K), dtype=bool)
+        lnr = np.full((n, MAXRADII), np.log(1e-4))
+        rmask = np.zeros((n, MAXRADII), dtype=bool)
+        for s, (pi, _) in enumerate(b.slot_map):
+            patch = patches[pi]
+            ns, nb = int(patch.n_sources), int(patch.n_bands)
+            l, h = (np.asarray(x, dtype=np.float64).reshape(ns, nb + 6) for x in bounds[pi])
+            lo[s, :ns, :nb], hi[s, :ns, :nb] = l[:, :nb], h[:, :nb]
+            lo[s, :ns, self.NB:], hi[s, :ns, self.NB:] = l[:, nb:], h[:, nb:]
+            mask[s, :ns, :nb] = True
+            mask[s, :ns, self.NB:] = True
+            model = scenes[pi].sources[0].amplitude_model
+            if not isinstance(model, LogNormalMixtureAmplitudes):
+                raise TypeError("device amplitudes implement LogNormalMixtureAmplitudes only")
+            r = len(model.radii)
+            lnr[s, :r] = model._lnr
+            rmask[s, :r] = True
+        f64 = dict(dtype=torch.float64, device=dev)
+        self.lo = torch.tensor(lo, **f64)
+        self.hi = torch.tensor(hi, **f64)
+        self.mask = torch.tensor(mask, device=dev)
+        self.fmask = self.mask.to(torch.float64)
+        self.lnr = torch.tensor(lnr, **f64)
+        self.rmask = torch.tensor(rmask, device=dev)
+        self.z = torch.zeros((n, self.S, self.K), **f64)
+        self.p = torch.zeros_like(self.z)
+        self.g = torch.zeros_like(self.z)
+        self.U = torch.zeros(n, **f64)
+        self.eps = torch.full((n, 1, 1), 0.01, **f64)
+        self.props = b.proposals.view(torch.float32).view(n, MAXSOURCES, _REC)
+        self.props.zero_()
+        self.resp = b.resp.view(n, MAXBANDS, MAXSOURCES, NPARAMS)
+        self.chi2 = b.chi2.view(n, MAXBANDS)
+        self.graph = None
+        self.graph_steps = None
+
+    # ------------------------------------------------------------------
+    def set_theta(self, thetas):
+        """Initial positions from flat parameter vectors (one per slot, in
+        slot order), mapped through the bounded transform."""
+        torch = _torch()
+        z = np.zeros((self.n_slots, self.S, self.K))
+        lo, hi = self.lo.cpu().numpy(), self.hi.cpu().numpy()
+        for s, (pi, _) in enumerate(self.batch.slot_map):
+            patch = self.batch.patches[pi]
+            ns, nb = int(patch.n_sources), int(patch.n_bands)
+            t = np.asarray(thetas[s], dtype=np.float64).reshape(ns, nb + 6)
+            full = np.zeros((self.S, self.K))
+            full[:ns, :nb], full[:ns, self.NB:] = t[:, :nb], t[:, nb:]
+            y = (full - lo[s]) / (hi[s] - lo[s])
+            y = np.clip(y, 1e-12, 1 - 1e-12)
+            z[s] = np.where(self.mask[s].cpu().numpy(), -np.log(1.0 / y - 1.0), 0.0)
+        self.z.copy_(torch.from_numpy(z))
+
+    def theta(self):
+        """Current positions in the constrained space, flat per slot."""
+        th = (self.lo + torch_sigmoid(self.z) * (self.hi - self.lo)).cpu().numpy()
+        out = []
+        for s, (pi, _) in enumerate(self.batch.slot_map):
+            patch = self.batch.patches[pi]
+            ns, nb = int(patch.n_sources), int(patch.n_bands)
+            out.append(np.concatenate([th[s, :ns, :nb], th[s, :ns, self.NB:]], axis=1).reshape(-1))
+        return out
+
+    # ------------------------------------------------------------------
+    def _amplitudes(self, n, rh):
+        """A, dA/dn, dA/drh [slot, source, MAXRADII] of the log-normal model."""
+        torch = _torch()
+        sig = 0.6 + 0.25 * n
+        mu = torch.log(rh) - 0.15 * (n - 1.0)
+        z = (self.lnr[:, None, :] - mu[..., None]) / sig[..., None]
+        z2 = torch.where(self.rmask[:, None, :], z * z, torch.full_like(z, float("inf")))
+        w = torch.exp(-0.5 * (z2 - z2.min(dim=-1, keepdim=True).values))
+        A = w / w.sum(dim=-1, keepdim=True)
+        zz = torch.where(self.rmask[:, None, :], z, torch.zeros_like(z))
+        dl_drh = zz / (sig * rh)[..., None]
+        dl_dn = zz / sig[..., None] * (-0.15) + zz * zz * 0.25 / sig[..., None]
+        dA_dn = A * (dl_dn - (A * dl_dn).sum(-1, keepdim=True))
+        dA_drh = A * (dl_drh - (A * dl_drh).sum(-1, keepdim=True))
+        return A, dA_dn, dA_drh
+
+    def _potential_and_grad(self):
+        """U(z) and dU/dz for the current z into self.U / self.g (device)."""
+        torch = _torch()
+        S, NB = self.S, self.NB
+        sz = torch_sigmoid(self.z)
+        rng = self.hi - self.lo
+        theta = self.lo + sz * rng
+        jac = rng * sz * (1.0 - sz)
+        # Source records
+        props = self.props
+        props[:, :S, _F_FLUX:_F_FLUX + NB] = (theta[:, :, :NB] * self.fmask[:, :, :NB]).float()
+        props[:, :S, _F_SHAPE:_F_SHAPE + 6] = theta[:, :, NB:].float()
+        A, dA_dn, dA_drh = self._amplitudes(theta[:, :, NB + 4], theta[:, :, NB + 5])
+        props[:, :S, _F_AMP:_F_AMP + MAXRADII] = A.float()
+        props[:, :S, _F_DRH:_F_DRH + MAXRADII] = dA_drh.float()
+        props[:, :S, _F_DN:_F_DN + MAXRADII] = dA_dn.float()
+        self.batch.launch()
+        # stack_grad on the device
+        d = self.resp[:, :, :S, :].double()
+        g_theta = torch.cat([d[:, :NB, :, 0].permute(0, 2, 1), d[:, :, :, 1:].sum(dim=1)], dim=2)
+        chi2 = self.chi2.sum(dim=1)
+        lndet = (torch.log(jac) * self.fmask).sum(dim=(1, 2))
+        self.U.copy_(0.5 * chi2 - lndet)
+        self.g.copy_((0.5 * g_theta * jac - (1.0 - 2.0 * sz)) * self.fmask)
+
+    def _trajectory(self, n_steps):
+        self._potential_and_grad()
+        self.p.sub_(0.5 * self.eps * self.g)
+        for i in range(n_steps):
+            self.z.add_(self.eps * self.p * self.fmask)
+            self._potential_and_grad()
+            if i < n_steps - 1:
+                self.p.sub_(self.eps * self.g)
+        self.p.sub_(0.5 * self.eps * self.g)
+
+    def capture(self, n_steps):
+        """Record one n_steps leapfrog trajectory of every chain in a graph."""
+        torch = _torch()
+        s = torch.cuda.Stream(device=self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        z0, p0 = self.z.clone(), self.p.clone()
+        with torch.cuda.stream(s):
+            self._trajectory(n_steps)          # warm-up outside the graph
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        self.z.copy_(z0)
+        self.p.copy_(p0)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._trajectory(n_steps)
+        self.graph_steps = n_steps
+        self.z.copy_(z0)
+        self.p.copy_(p0)
+
+    def trajectory(self, n_steps, use_graph=True):
+        """Advance every chain by one leapfrog trajectory (no Metropolis)."""
+        if use_graph:
+            if self.graph is None or self.graph_steps != n_steps:
+                self.capture(n_steps)
+            self.graph.replay()
+        else:
+            self._trajectory(n_steps)
+
+    def potential(self):
+        """U(z) of every chain at the current position (device launch)."""
+        self._potential_and_grad()
+        return self.U.clone(), self.g.clone()
+
+    def hmc_step(self, n_steps, generator=None, use_graph=True):
+        """One HMC transition of every chain: fresh momenta, a leapfrog
+        trajectory, Metropolis accept/reject.  Returns acceptance flags."""
+        torch = _torch()
+        p0 = torch.randn(self.p.shape, generator=generator, dtype=torch.float64,
+                         device=self.device) * self.fmask
+        z0 = self.z.clone()
+        U0, _ = self.potential()
+        self.p.copy_(p0)
+        self.trajectory(n_steps, use_graph=use_graph)
+        U1, _ = self.potential()
+        H0 = U0 + 0.5 * (p0 * p0).sum(dim=(1, 2))
+        H1 = U1 + 0.5 * (self.p * self.p).sum(dim=(1, 2))
+        u = torch.rand(self.n_slots, generator=generator, dtype=torch.float64, device=self.device)
+        accept = torch.log(u) < (H0 - H1)
+        self.z.copy_(torch.where(accept[:, None, None], self.z, z0))
+        return accept
+
+
+def torch_sigmoid(x):
+    return _torch().sigmoid(x)

@@ -6,9 +6,10 @@ padding rule of forcepho/patches/pixel_patch.py:473-484:
   * each exposure is padded to a multiple of ``warp_size`` (64, one CDNA
     wavefront) with flux = 0, ierr = 0 and x = y = -1;
   * band_start = cumsum(band_N) shifted, exposure_start = running offset.
-Within an exposure pixels are emitted in 8x8 superpixel order (row-major
-blocks, row-major inside a block) so that one wavefront of 64 lanes covers
-a compact 8x8 footprint; ``pixel_order="flat"`` keeps plain row-major order.
+Within an exposure pixels are emitted in superpixel order: 16x16 squares
+made of four 8x8 squares, so that one wavefront of 64 lanes covers an 8x8
+footprint and a 256-pixel tile a 16x16 one; ``pixel_order="flat"`` keeps
+plain row-major order.
 """
 
 import numpy as np
@@ -49,13 +50,19 @@ class Stamp:
         return self.pixel_values.size
 
 
-def superpixel_order(nx, ny, block=8):
-    """Flat (row-major) indices of an (nx, ny) image visited block by block."""
+def superpixel_order(nx, ny, block=16, sub=8):
+    """Flat (row-major) indices of an (nx, ny) image visited in block x block
+    squares (row-major), each visited as sub x sub squares (row-major), each
+    of those row-major: a run of 64 packed pixels is one 8x8 square and a run
+    of 256 one 16x16 square."""
     idx = np.arange(nx * ny).reshape(nx, ny)
     out = []
     for bx in range(0, nx, block):
         for by in range(0, ny, block):
-            out.append(idx[bx:bx + block, by:by + block].reshape(-1))
+            blk = idx[bx:bx + block, by:by + block]
+            for sx in range(0, blk.shape[0], sub):
+                for sy in range(0, blk.shape[1], sub):
+                    out.append(blk[sx:sx + sub, sy:sy + sub].reshape(-1))
     return np.concatenate(out)
 
 

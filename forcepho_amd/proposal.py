@@ -96,6 +96,7 @@ class HIPProposer(ProposerBase):
         self._chi2_h = torch.zeros(MAXBANDS, dtype=torch.float64, **pin)
         self._resp_h = torch.zeros(self._resp.numel(), dtype=torch.float32, **pin)
         self._slot_key = None
+        self._h2d_done = None   # event of the last copy out of the pinned proposal buffer
 
     def _stream(self):
         import torch
@@ -135,8 +136,12 @@ class HIPProposer(ProposerBase):
             if len(proposal) != n_src:
                 raise ValueError(f"proposal has {len(proposal)} sources, patch has {n_src}")
             raw = np.ascontiguousarray(proposal, dtype=source_struct_dtype).view(np.uint8)
+            if self._h2d_done is not None:
+                self._h2d_done.synchronize()   # the pinned buffer may still be in flight
             self._prop_h[:raw.size].copy_(torch.from_numpy(raw.reshape(-1)))
             self._prop[:raw.size].copy_(self._prop_h[:raw.size], non_blocking=True)
+            self._h2d_done = torch.cuda.Event()
+            self._h2d_done.record(self._stream())
         else:
             self._prop[:proposal.numel()].copy_(proposal, non_blocking=True)
         n_tiles = self._slot_for(patch, pixel_weight)
@@ -169,6 +174,25 @@ class HIPProposer(ProposerBase):
                 residuals = self.unpack_residuals(residuals, patch)
             return chi_out, chi2_derivs, residuals
         return chi_out, chi2_derivs
+
+
+    def vjp(self, proposal, pixel_weight, patch=None):
+        """sum_pix pixel_weight[pix] * d model[pix] / d param for every band
+        and source: float64 [n_bands, n_sources, NPARAMS].  ``pixel_weight``
+        is a float32 device tensor over the packed pixels.  The kernel runs
+        only its gradient pass, recomputing the Gaussians from the proposal.
+        """
+        if patch is None:
+            patch = self.patch
+        if pixel_weight.dtype.itemsize != 4 or pixel_weight.numel() < patch.npix:
+            raise ValueError("pixel_weight must be float32 over all packed pixels")
+        n_bands, n_sources = int(patch.n_bands), int(patch.n_sources)
+        self.launch(proposal, patch, pixel_weight=pixel_weight)
+        nr = n_bands * MAXSOURCES * NPARAMS
+        self._resp_h[:nr].copy_(self._resp[:nr], non_blocking=True)
+        self._stream().synchronize()
+        d = self._resp_h[:nr].numpy().reshape(n_bands, MAXSOURCES, NPARAMS)
+        return d[:, :n_sources].astype(np.float64)
 
 
 # forcepho's names for the device proposer (forcepho/proposal.py:55,155)

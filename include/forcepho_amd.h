@@ -226,9 +226,11 @@ int fpx_evaluate_batch(const FpxSlot *slots, int32_t n_slots,
                        uint32_t *queue, void *stream);
 
 /* Throughput probe of the VALU pipes used by the hot loop: kind 0 = FP32
- * FMA, kind 1 = v_exp_f32.  Launches a fixed-size kernel writing one float
- * per thread to out (device, n_threads floats); the caller times it.
- * Returns the operations the launch performs in *ops. */
+ * FMA, kind 1 = v_exp_f32, 8 independent chains of `iters` steps per lane.
+ * Writes one float per thread to out (device, n_threads floats): lane 0 of
+ * each wavefront the shader-clock cycles of its loop, lane 1 the 100 MHz
+ * reference ticks of the same interval.  Returns the operations the launch
+ * performs in *ops; the caller may also time the launch. */
 int fpx_probe_valu(int32_t kind, int32_t iters, float *out, int32_t n_threads,
                    double *ops, void *stream);
 

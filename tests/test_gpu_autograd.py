@@ -1,0 +1,121 @@
+"""torch.autograd over the patch model: backward (the kernel's gradient pass
+with dL/dresidual as pixel weights) against the chi2-mode gradient, the
+float64 oracle and central differences; a 32-step leapfrog driven by it."""
+
+import numpy as np
+import pytest
+
+import fpx_testutil as U
+import oracle
+import oracle.fp64 as fp64
+from forcepho_amd import scenes
+from forcepho_amd.model import stack_grad
+
+pytestmark = pytest.mark.gpu
+
+
+def _model(side, seed=0, n_bands=8, n_sources=15):
+    from forcepho_amd.patches import SimplePatch
+    from forcepho_amd.autograd import PatchModel
+    spec = scenes.spec_multi(seed, side=side, n_bands=n_bands, n_sources=n_sources)
+    render = oracle.render_model if side <= 48 else scenes.render_on_device
+    patch, scene, theta = scenes.build_patch(spec, SimplePatch, render, return_residual=True)
+    patch.send_to_device()
+    return PatchModel(patch, scene), theta
+
+
+def _offset(theta, n_bands):
+    t = theta.copy().reshape(-1, n_bands + 6)
+    t[:, n_bands + 2] *= 0.95      # q
+    t[:, n_bands + 3] += 0.05      # pa
+    t[:, :n_bands] *= 1.03         # fluxes
+    return t.reshape(-1)
+
+
+def test_autograd_matches_kernel_gradient_and_fp64():
+    import torch
+    model, theta = _model(side=32)
+    th = _offset(theta, 8)
+    t = torch.tensor(th, dtype=torch.float64, requires_grad=True)
+    chi2 = model.chi2(t)
+    chi2.backward()
+    g_auto = t.grad.numpy()
+    # the chi2-mode kernel gradient of the same proposal
+    _, c_band, d = model.forward_residual(model.proposal(th))
+    g_kernel = stack_grad(d, model.n_sources, model.n_bands)
+    np.testing.assert_allclose(chi2.item(), c_band.sum(), rtol=1e-6)
+    # float64 reference of both
+    c64, g64, _, mass = fp64.evaluate(model.patch, model.proposal(th), mass=True)
+    g64s = stack_grad(g64, model.n_sources, model.n_bands)
+    ms = stack_grad(mass, model.n_sources, model.n_bands)
+    tol = U.GRAD_RTOL * np.abs(g64s) + U.GRAD_FLOOR * ms
+    assert np.all(np.abs(g_auto - g64s) <= tol)
+    assert np.all(np.abs(g_kernel - g64s) <= tol)
+
+
+def test_autograd_central_differences_full_size():
+    """The 8-band x 2-exposure x 128x128, 15-source patch: autograd gradient
+    against central differences of the kernel's own chi2 (step 1e-4 in
+    unit-scaled parameters).  The differences carry the float32 rounding of
+    the residual image: each pixel's chi2 term (r^2 - d^2) ierr^2 moves by
+    about 2 |r ierr| |m ierr| 2^-24, so the chi2 difference has a rounding
+    floor of a few times sqrt(sum of squares) of that, which enters the
+    tolerance next to a 2% relative term."""
+    import torch
+    model, theta = _model(side=128)
+    nb = 8
+    th = _offset(theta, nb)
+    t = torch.tensor(th, dtype=torch.float64, requires_grad=True)
+    r = model.residual(t)
+    chi2 = ((r.double() * model.ierr) ** 2 - (model.data * model.ierr) ** 2).sum()
+    chi2.backward()
+    g = t.grad.numpy()
+    rr = r.detach().double() * model.ierr
+    mm = (model.data - r.detach().double()) * model.ierr
+    noise = 4.0 * 2.0 ** -24 * float(torch.sqrt(((2 * rr.abs() * mm.abs()) ** 2).sum()))
+    pix_deg = 0.03 / 3600.0
+    units = np.tile(np.concatenate([np.abs(th[:nb]).clip(1e-3), [pix_deg, pix_deg, 1, 1, 1, 0.1]]),
+                    model.n_sources)
+    h = 1e-4
+    rng = np.random.default_rng(0)
+    idx = rng.choice(len(th), size=24, replace=False)
+    for i in idx:
+        tp, tm = th.copy(), th.copy()
+        tp[i] += h * units[i]
+        tm[i] -= h * units[i]
+        with torch.no_grad():
+            fd = (model.chi2(torch.tensor(tp)).item() - model.chi2(torch.tensor(tm)).item()) / (2 * h * units[i])
+        tol = 2e-2 * abs(g[i]) + 2 * noise / (2 * h * units[i])
+        assert abs(fd - g[i]) <= tol, (i, fd, g[i], tol)
+
+
+def test_leapfrog_trajectory_conserves_energy():
+    """32 leapfrog steps of HMC on -lnL = chi2/2 in unit-scaled parameters,
+    gradients from autograd: the Hamiltonian stays put and the trajectory
+    reverses."""
+    import torch
+    from forcepho_amd.hmc import leapfrog, hamiltonian
+    model, theta = _model(side=48, n_bands=4, n_sources=6)
+    nb = 4
+    pix_deg = 0.03 / 3600.0
+    units = np.tile(np.concatenate([np.abs(theta[:nb]).clip(1e-3) * 0.01,
+                                    [0.1 * pix_deg, 0.1 * pix_deg, 0.01, 0.01, 0.01, 0.001]]),
+                    model.n_sources)
+    th0 = _offset(theta, nb)
+
+    def U_and_grad(q):
+        t = torch.tensor(q * units, dtype=torch.float64, requires_grad=True)
+        u = 0.5 * model.chi2(t)
+        u.backward()
+        return u.item(), t.grad.numpy() * units
+
+    rng = np.random.default_rng(2)
+    q0 = th0 / units
+    p0 = rng.normal(size=len(q0))
+    U0, _ = U_and_grad(q0)
+    q1, p1, U1, calls = leapfrog(q0, p0, U_and_grad, eps=0.02, n_steps=32)
+    assert calls == 33
+    H0, H1 = hamiltonian(U0, p0), hamiltonian(U1, p1)
+    assert abs(H1 - H0) < 0.05 * max(1.0, abs(np.sum(p0 ** 2)))
+    q2, p2, U2, _ = leapfrog(q1, -p1, U_and_grad, eps=0.02, n_steps=32)
+    np.testing.assert_allclose(q2, q0, atol=1e-3)

@@ -1,0 +1,106 @@
+"""Batched evaluation of uneven patches and several chains in one launch,
+against per-patch checkers (C restatement for chi2, float64 oracle for the
+gradients)."""
+
+import numpy as np
+import pytest
+
+import fpx_testutil as U
+import oracle
+import oracle.fp64 as fp64
+from forcepho_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _patches(n=7, seed=40):
+    from forcepho_amd.patches import SimplePatch
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        nb = int(rng.integers(1, 4))
+        spec = scenes.draw_patch_spec(rng, nb, rng.integers(1, 3, size=nb),
+                                      [int(rng.integers(20, 60))] * nb,
+                                      int(rng.integers(1, 6)), rng.integers(2, 5, size=nb))
+        patch, scene, theta = scenes.build_patch(spec, SimplePatch, oracle.render_model)
+        patch.send_to_device()
+        out.append((patch, scene))
+    return out
+
+
+@pytest.fixture(scope="module")
+def batch_setup():
+    pats = _patches()
+    chains = [2, 1, 1, 3, 1, 1, 2]
+    rng = np.random.default_rng(1)
+    proposals = []
+    for (patch, scene), nc in zip(pats, chains):
+        for c in range(nc):
+            prop = scene.get_proposal().copy()
+            prop["q"] *= np.float32(rng.uniform(0.85, 1.0))
+            prop["pa"] += np.float32(rng.uniform(-0.2, 0.2))
+            prop["fluxes"] *= np.float32(rng.uniform(0.9, 1.1))
+            proposals.append(prop)
+    refs = []
+    k = 0
+    for (patch, scene), nc in zip(pats, chains):
+        for c in range(nc):
+            c32, _, _ = oracle.evaluate_cpu(patch, proposals[k])
+            c64, g64, _, mass = fp64.evaluate(patch, proposals[k], mass=True)
+            refs.append((c32, g64, mass))
+            k += 1
+    return pats, chains, proposals, refs
+
+
+def _check(results, refs):
+    for (chi2, d), (c32, g64, mass) in zip(results, refs):
+        rel = np.abs(chi2 - c32) / np.abs(c32)
+        assert rel.max() <= U.CHI2_RTOL, rel
+        assert U.grad_errors(d, g64, mass).max() <= 1.0
+
+
+@pytest.mark.parametrize("persistent", [True, False])
+@pytest.mark.parametrize("config", [0, 2, 5])
+def test_batch_matches_checkers(batch_setup, config, persistent):
+    from forcepho_amd.batch import BatchEvaluator
+    pats, chains, proposals, refs = batch_setup
+    print(f"[gpu] batch config {config} persistent {persistent}", flush=True)
+    ev = BatchEvaluator([p for p, _ in pats], chains=chains, config=config,
+                        persistent=persistent)
+    ev.set_proposals(proposals)
+    ev.launch()
+    _check(ev.results(), refs)
+    # a second launch reuses the self-cleaning accumulators and queue
+    ev.launch()
+    ev.launch()
+    _check(ev.results(), refs)
+
+
+def test_batch_autotune_keeps_results(batch_setup):
+    from forcepho_amd.batch import BatchEvaluator
+    pats, chains, proposals, refs = batch_setup
+    ev = BatchEvaluator([p for p, _ in pats], chains=chains)
+    ev.set_proposals(proposals)
+    timings = ev.autotune(repeats=2, configs=[1, 2, 5])
+    assert len(timings) == 6 and all(t > 0 for t in timings.values())
+    ev.launch()
+    _check(ev.results(), refs)
+
+
+def test_vjp_mode_matches_fp64():
+    """pixel_weight mode: sum_pix w * dmodel/dtheta against the float64
+    oracle with the same weights."""
+    from forcepho_amd.proposal import HIPProposer
+    import torch
+    patch, proposal, _ = U.load_golden("uneven_s6")
+    patch.send_to_device()
+    rng = np.random.default_rng(3)
+    w = rng.normal(size=patch.npix).astype(np.float32) * patch.ierr
+    prop = HIPProposer(patch=patch)
+    d = prop.vjp(proposal, torch.from_numpy(w).cuda(), patch)
+    _, g64, _, mass = fp64.evaluate(patch, proposal, pixel_weight=w, mass=True)
+    assert U.grad_errors(d, g64, mass).max() <= 1.0
+    # the chi2-mode evaluation afterwards is unaffected
+    chi2, dchi2 = prop.evaluate_proposal(proposal, patch=patch)
+    c32, _, _ = oracle.evaluate_cpu(patch, proposal)
+    assert np.max(np.abs(chi2 - c32) / np.abs(c32)) <= U.CHI2_RTOL

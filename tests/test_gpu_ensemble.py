@@ -1,0 +1,121 @@
+"""Device-resident HMC ensemble: potential and gradient against the host
+Posterior path, graph-captured trajectories against eager ones, and the
+gradient error distribution against the float64 oracle."""
+
+import numpy as np
+import pytest
+
+import fpx_testutil as U
+import oracle
+import oracle.fp64 as fp64
+from forcepho_amd import scenes
+from forcepho_amd.model import stack_grad
+
+pytestmark = pytest.mark.gpu
+
+
+def _bounds(theta, nb):
+    t = theta.reshape(-1, nb + 6)
+    lo, hi = t.copy(), t.copy()
+    lo[:, :nb], hi[:, :nb] = t[:, :nb] * 0.5, t[:, :nb] * 1.5 + 1e-3
+    pix = 0.03 / 3600.0
+    lo[:, nb:nb + 2] -= 3 * pix
+    hi[:, nb:nb + 2] += 3 * pix
+    lo[:, nb + 2], hi[:, nb + 2] = 0.2, 1.0
+    lo[:, nb + 3], hi[:, nb + 3] = -2.0, 2.0
+    lo[:, nb + 4], hi[:, nb + 4] = 0.8, 5.5
+    lo[:, nb + 5], hi[:, nb + 5] = 0.03, 0.4
+    return lo.reshape(-1), hi.reshape(-1)
+
+
+@pytest.fixture(scope="module")
+def ensemble():
+    from forcepho_amd.patches import SimplePatch
+    from forcepho_amd.ensemble import HMCEnsemble
+    rng = np.random.default_rng(77)
+    pats, scs, bnds, thetas = [], [], [], []
+    for k in range(3):
+        nb = [1, 3, 2][k]
+        spec = scenes.draw_patch_spec(rng, nb, [1] * nb, [40 + 8 * k] * nb, [1, 4, 2][k],
+                                      [3] * nb)
+        patch, scene, theta = scenes.build_patch(spec, SimplePatch, oracle.render_model)
+        patch.send_to_device()
+        pats.append(patch)
+        scs.append(scene)
+        bnds.append(_bounds(theta, nb))
+        thetas.append(theta)
+    chains = [2, 1, 2]
+    ens = HMCEnsemble(pats, scs, chains, bnds)
+    start = []
+    for pi, c in ens.batch.slot_map:
+        t = thetas[pi].copy()
+        nb = int(pats[pi].n_bands)
+        t.reshape(-1, nb + 6)[:, nb + 2] *= 0.97 - 0.02 * c
+        start.append(t)
+    ens.set_theta(start)
+    return ens, pats, scs, bnds, start
+
+
+def test_potential_matches_host_posterior(ensemble):
+    from forcepho_amd.model import FastPosterior
+    from forcepho_amd.proposal import HIPProposer
+    ens, pats, scs, bnds, start = ensemble
+    Ud, gd = ens.potential()
+    Ud, gd = Ud.cpu().numpy(), gd.cpu().numpy()
+    for s, (pi, _) in enumerate(ens.batch.slot_map):
+        patch = pats[pi]
+        nb = int(patch.n_bands)
+        post = FastPosterior(HIPProposer(patch=patch), patch, scene=scs[pi],
+                             lower=bnds[pi][0], upper=bnds[pi][1])
+        z = post.transform.inverse_transform(start[s])
+        lnp, grad = post.lnprob_and_grad(z)
+        assert Ud[s] == pytest.approx(-lnp, rel=1e-6)
+        g = np.concatenate([gd[s, :, :nb], gd[s, :, ens.NB:]], axis=1)[:patch.n_sources].reshape(-1)
+        np.testing.assert_allclose(g, -grad, rtol=1e-4, atol=1e-5 * np.abs(grad).max())
+
+
+def test_graph_replay_matches_eager(ensemble):
+    import torch
+    ens = ensemble[0]
+    z0 = ens.z.clone()
+    gen = torch.Generator(device=ens.device)
+    gen.manual_seed(5)
+    p0 = torch.randn(ens.p.shape, generator=gen, dtype=torch.float64, device=ens.device) * ens.fmask
+    ens.eps.fill_(0.01)
+    ens.p.copy_(p0)
+    ens.trajectory(8, use_graph=False)
+    z_eager, p_eager = ens.z.clone(), ens.p.clone()
+    for rep in range(3):
+        ens.z.copy_(z0)
+        ens.p.copy_(p0)
+        ens.trajectory(8, use_graph=True)
+        torch.cuda.synchronize()
+        np.testing.assert_allclose(ens.z.cpu().numpy(), z_eager.cpu().numpy(), rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(ens.p.cpu().numpy(), p_eager.cpu().numpy(), rtol=1e-9, atol=1e-12)
+        ens.potential()           # eager work between replays
+    ens.z.copy_(z0)
+
+
+def test_trajectory_energy_and_gradient_errors(ensemble):
+    import torch
+    ens, pats, scs, bnds, start = ensemble
+    gen = torch.Generator(device=ens.device)
+    gen.manual_seed(9)
+    ens.eps.fill_(0.005)
+    acc = [ens.hmc_step(32, generator=gen) for _ in range(3)]
+    assert sum(int(a.sum()) for a in acc) >= 1
+    # gradient errors of the chi2 kernel at the chains' final positions
+    theta = ens.theta()
+    errs = []
+    for s, (pi, _) in enumerate(ens.batch.slot_map):
+        scs[pi].set_all_source_params(theta[s])
+        prop = scs[pi].get_proposal()
+        from forcepho_amd.proposal import HIPProposer
+        _, d = HIPProposer(patch=pats[pi]).evaluate_proposal(prop, patch=pats[pi])
+        _, g64, _, mass = fp64.evaluate(pats[pi], prop, mass=True)
+        errs.append(U.grad_errors(d, g64, mass).reshape(-1))
+    errs = np.concatenate(errs)
+    print("gradient error / tolerance quantiles (50, 90, 99, 100%):",
+          np.percentile(errs, [50, 90, 99, 100]))
+    assert errs.max() <= 1.0
+    assert np.median(errs) <= 0.1
