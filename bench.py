@@ -136,20 +136,16 @@ def main():
     proposal["q"] *= np.float32(0.97)   # off the truth: gradients are generic
     n_evals = patch.n_evals()
 
-    # pick the kernel configuration (tile shape) by measurement
-    n_cfg = lib().fpx_n_configs()
-    cands = [args.config] if args.config is not None else list(range(n_cfg))
-    timings = {}
-    for cfg in cands:
-        prop = HIPProposer(patch=patch, config=cfg)
-        raw = np.ascontiguousarray(proposal).view(np.uint8).reshape(-1)
-        pdev = torch.from_numpy(raw.copy()).cuda()
-        time_config(prop, patch, pdev, 3)
-        timings[cfg] = float(np.median(time_config(prop, patch, pdev, 10)))
-    best = min(timings, key=timings.get)
-    prop = HIPProposer(patch=patch, config=best)
+    # pick the kernel configuration (tile shape, persistent or not) by measurement
     raw = np.ascontiguousarray(proposal).view(np.uint8).reshape(-1)
     pdev = torch.from_numpy(raw.copy()).cuda()
+    prop = HIPProposer(patch=patch)
+    if args.config is not None:
+        tuned = prop.autotune(pdev, patch, repeats=10, configs=[args.config])
+    else:
+        tuned = prop.autotune(pdev, patch, repeats=10)
+    timings = {f"{c}{'p' if p else ''}": t for (c, p), t in tuned.items()}
+    best = f"{prop.config}{'p' if prop.persistent else ''}"
 
     if args.copy_mb:
         # a plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE

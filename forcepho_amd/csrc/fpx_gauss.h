@@ -19,7 +19,7 @@
 //   [9..22]  amp/12 dA/dflux dA/dsersic dA/drh
 //            dA/dq kq dFxx/dq dFyy/dq dFxy/dq
 //            dA/dpa kp dFxx/dpa dFyy/dpa dFxy/dpa    gradient pass
-//   [23]     pad
+//   [23]     source index (int bits)
 // with h0 = 1 - (fxx + fyy)/24 and kq = -(dFxx/dq + dFyy/dq)/2 (kp
 // likewise), so that the pixel integral factor
 // H = 1 + (vx^2 + vy^2 - fxx - fyy)/24 of common_kernel.cc:77 is
@@ -42,7 +42,7 @@ enum RecField {
   kAmp12, kDAF, kDAN, kDAR,
   kDAQ, kKQ, kQxx, kQyy, kQxy,
   kDAP, kKP, kPxx, kPyy, kPxy,
-  kPad
+  kSrc
 };
 
 struct M22 {
@@ -68,19 +68,88 @@ __device__ __forceinline__ void neg_sandwich(float fxx, float fxy, float fyy,
   oyy = -(m21 * fxy + m22 * fyy);
 }
 
-// Build the record of source s, PSF component p (0 <= p < n_psf_per_source[band])
-// for exposure e of band `band`.  PSF lookup ignores the source index, as
-// the reference does (compute_gaussians_kernel.cc:100, .cu:77).
-__device__ __forceinline__ void build_record(const FpxPatch *__restrict__ P,
-                                             const FpxSource *__restrict__ props,
-                                             int e, int band, int s, int p,
-                                             float *__restrict__ out) {
-  const FpxSource *src = props + s;
+// Per (exposure, source) geometry: T = D R(pa) S(q) and its q / pa
+// derivatives, the source centre in pixels CW (sky - crval) + crpix.
+struct SourceGeometry {
+  M22 T, dTq, dTp;
+  float xs, ys;
+};
+
+template <bool kDerivs>
+__device__ __forceinline__ void source_geometry(const FpxPatch *__restrict__ P,
+                                                const FpxSource *__restrict__ src, int e,
+                                                int s, SourceGeometry &g) {
   const int base = P->n_sources * e + s;
   const float *Dp = P->D + 4 * base;
   const float *CWp = P->CW + 4 * base;
   const float *crpix = P->crpix + 2 * base;
   const float *crval = P->crval + 2 * base;
+  const float q = src->q;
+  float sn, cs;
+  sincosf(src->pa, &sn, &cs);
+  const M22 D = {Dp[0], Dp[1], Dp[2], Dp[3]};
+  const M22 R = {cs, -sn, sn, cs};
+  const float iq = 1.0f / q;
+  const M22 S = {iq, 0.f, 0.f, q};
+  const M22 DR = mat_mul(D, R);
+  g.T = mat_mul(DR, S);
+  if (kDerivs) {
+    const M22 dR = {-sn, -cs, cs, -sn};
+    const M22 dS = {-1.0f / (q * q), 0.f, 0.f, 1.0f};
+    g.dTq = mat_mul(DR, dS);
+    g.dTp = mat_mul(mat_mul(D, dR), S);
+  }
+  const float sx = src->ra - crval[0];
+  const float sy = src->dec - crval[1];
+  g.xs = CWp[0] * sx + CWp[1] * sy + crpix[0];
+  g.ys = CWp[2] * sx + CWp[3] * sy + crpix[1];
+}
+
+// Sigma = covar T T^T + C_psf
+__device__ __forceinline__ void convolved_sigma(const M22 &T, float covar,
+                                                const FpxPSFSourceGaussian &psf, float &Sxx,
+                                                float &Sxy, float &Syy) {
+  Sxx = covar * (T.a * T.a + T.b * T.b) + psf.Cxx;
+  Sxy = covar * (T.a * T.c + T.b * T.d) + psf.Cxy;
+  Syy = covar * (T.c * T.c + T.d * T.d) + psf.Cyy;
+}
+
+// Half-width of the box holding dx^T Sigma^-1 dx <= MAX_EXP_ARG along an
+// axis of variance S, with a relative 1e-3 and an absolute 0.01 px margin.
+__device__ __forceinline__ float cutoff_halfwidth(float S) {
+  return sqrtf((float)MAX_EXP_ARG) * sqrtf(fmaxf(S, 0.0f)) * 1.001f + 0.01f;
+}
+
+// True when the MAX_EXP_ARG box of Gaussian (s, p) of exposure e can reach
+// a pixel of [x0, x1] x [y0, y1].
+__device__ __forceinline__ bool gauss_reaches(const FpxPatch *__restrict__ P,
+                                              const FpxSource *__restrict__ props, int e,
+                                              int s, int p, float x0, float x1, float y0,
+                                              float y1) {
+  SourceGeometry g;
+  source_geometry<false>(P, props + s, e, s, g);
+  const FpxPSFSourceGaussian psf = P->psfgauss[P->psfgauss_start[e] + p];
+  float Sxx, Sxy, Syy;
+  convolved_sigma(g.T, P->rad2[psf.sersic_radius_bin], psf, Sxx, Sxy, Syy);
+  const float xc = g.xs + psf.xcen;
+  const float yc = g.ys + psf.ycen;
+  const float gx = fmaxf(fmaxf(x0 - xc, xc - x1), 0.0f);
+  const float gy = fmaxf(fmaxf(y0 - yc, yc - y1), 0.0f);
+  return gx <= cutoff_halfwidth(Sxx) && gy <= cutoff_halfwidth(Syy);
+}
+
+// Build the record of source s, PSF component p (0 <= p < n_psf_per_source[band])
+// for exposure e of band `band`.  PSF lookup ignores the source index, as
+// the reference does (compute_gaussians_kernel.cc:100, .cu:77).  The pad
+// slot holds the source index (as int bits).
+__device__ __forceinline__ void build_record(const FpxPatch *__restrict__ P,
+                                             const FpxSource *__restrict__ props,
+                                             int e, int band, int s, int p,
+                                             float *__restrict__ out) {
+  const FpxSource *src = props + s;
+  SourceGeometry g;
+  source_geometry<true>(P, src, e, s, g);
+  const M22 &T = g.T, &dTq = g.dTq, &dTp = g.dTp;
   const float gcal = P->G[e];
   const FpxPSFSourceGaussian psf = P->psfgauss[P->psfgauss_start[e] + p];
   const int rb = psf.sersic_radius_bin;
@@ -89,31 +158,9 @@ __device__ __forceinline__ void build_record(const FpxPatch *__restrict__ P,
   const float da_dn = src->damplitude_dnsersic[rb];
   const float da_dr = src->damplitude_drh[rb];
   const float flux = src->fluxes[band];
-  const float q = src->q;
 
-  float sn, cs;
-  sincosf(src->pa, &sn, &cs);
-  const M22 D = {Dp[0], Dp[1], Dp[2], Dp[3]};
-  const M22 R = {cs, -sn, sn, cs};
-  const M22 dR = {-sn, -cs, cs, -sn};
-  const float iq = 1.0f / q;
-  const M22 S = {iq, 0.f, 0.f, q};
-  const M22 dS = {-1.0f / (q * q), 0.f, 0.f, 1.0f};
-  const M22 DR = mat_mul(D, R);
-  const M22 T = mat_mul(DR, S);
-  const M22 dTq = mat_mul(DR, dS);
-  const M22 dTp = mat_mul(mat_mul(D, dR), S);
-
-  // source centre in pixels: CW (sky - crval) + crpix
-  const float sx = src->ra - crval[0];
-  const float sy = src->dec - crval[1];
-  const float xs = CWp[0] * sx + CWp[1] * sy + crpix[0];
-  const float ys = CWp[2] * sx + CWp[3] * sy + crpix[1];
-
-  // Sigma = covar T T^T + C_psf
-  const float Sxx = covar * (T.a * T.a + T.b * T.b) + psf.Cxx;
-  const float Sxy = covar * (T.a * T.c + T.b * T.d) + psf.Cxy;
-  const float Syy = covar * (T.c * T.c + T.d * T.d) + psf.Cyy;
+  float Sxx, Sxy, Syy;
+  convolved_sigma(T, covar, psf, Sxx, Sxy, Syy);
   const float idet = 1.0f / (Sxx * Syy - Sxy * Sxy);
   const float fxx = Syy * idet;
   const float fxy = -Sxy * idet;
@@ -136,11 +183,10 @@ __device__ __forceinline__ void build_record(const FpxPatch *__restrict__ P,
   const float dAP = 0.5f * amp * (Sxx * pxx + 2.0f * Sxy * pxy + Syy * pyy);
 
   const float h0 = 1.0f - (fxx + fyy) * (1.0f / 24.0f);
-  out[kXc] = xs + psf.xcen;
-  out[kYc] = ys + psf.ycen;
-  const float r_max = sqrtf((float)MAX_EXP_ARG);
-  out[kHx] = r_max * sqrtf(fmaxf(Sxx, 0.0f)) * 1.001f + 0.01f;
-  out[kHy] = r_max * sqrtf(fmaxf(Syy, 0.0f)) * 1.001f + 0.01f;
+  out[kXc] = g.xs + psf.xcen;
+  out[kYc] = g.ys + psf.ycen;
+  out[kHx] = cutoff_halfwidth(Sxx);
+  out[kHy] = cutoff_halfwidth(Syy);
   out[kFxx] = fxx;
   out[kFxy] = fxy;
   out[kFyy] = fyy;
@@ -160,7 +206,7 @@ __device__ __forceinline__ void build_record(const FpxPatch *__restrict__ P,
   out[kPxx] = pxx;
   out[kPyy] = pyy;
   out[kPxy] = pxy;
-  out[kPad] = 0.0f;
+  out[kSrc] = __builtin_bit_cast(float, s);
 }
 
 }  // namespace fpx

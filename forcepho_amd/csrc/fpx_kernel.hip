@@ -9,9 +9,11 @@
 // reference runs one block per band: compute_gaussians_kernel.cu:209,262).
 //
 // Per tile:
-//   prologue  one lane per (source, PSF component) builds the exposure's
-//             ImageGaussian records into LDS (fpx_gauss.h); if they exceed the
-//             LDS budget the sources are processed in chunks and rebuilt.
+//   prologue  one lane per (source, PSF component) tests whether the
+//             Gaussian can reach the tile, the survivors are compacted in
+//             order and their ImageGaussian records built into LDS
+//             (fpx_gauss.h); if they exceed the LDS budget the sources are
+//             processed in chunks and rebuilt.
 //   pass 1    model = sum_g amp_g G_g H_g over the wave's Gaussian slice
 //             (common_kernel.cc:57-83), partial models combined through LDS,
 //             residual r = d - model, chi2 += (r ierr - d ierr)(r ierr + d ierr)
@@ -26,11 +28,11 @@
 //             slot (ticket counter) writes pchi2 / Response and re-zeroes the
 //             accumulator.
 // A Gaussian is skipped at a pixel when dx^T F dx > MAX_EXP_ARG
-// (common_kernel.cc:73,108).  Two wave-uniform shortcuts skip work that this
-// test would discard anyway: a Gaussian whose MAX_EXP_ARG box misses the
-// wavefront's pixel bounding box is not evaluated at all, and one whose
-// exponent exceeds the limit at every pixel of the wavefront stops after the
-// exponent.
+// (common_kernel.cc:73,108).  Two shortcuts skip work that this test would
+// discard anyway: the prologue builds records only for Gaussians whose
+// MAX_EXP_ARG box reaches the tile's pixel bounding box, and a Gaussian whose
+// exponent exceeds the limit at every pixel of a wavefront stops after the
+// exponent (wave-uniform branch).
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -64,7 +66,7 @@ constexpr float kExp2Scale = -0.72134752044448170368f;  // -0.5 * log2(e)
 constexpr float kMaxExpArg = (float)MAX_EXP_ARG;
 
 // ------------------------------------------------------------------------
-// wavefront reductions
+// wavefront helpers
 // ------------------------------------------------------------------------
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
@@ -98,34 +100,49 @@ __device__ __forceinline__ float uniform_f(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
 }
 
-// Pixel bounding box of a wavefront: [x0, x1] x [y0, y1] (empty when the
-// wavefront holds no valid pixel: x0 > x1).
-struct Box {
-  float x0, x1, y0, y1;
-};
-
-// True when the Gaussian's MAX_EXP_ARG box (centre r0.xy, half-widths r0.zw)
-// can reach some pixel of the box.  Wave-uniform.
-__device__ __forceinline__ bool box_hits(const float4 &r0, const Box &b) {
-  const float gx = fmaxf(fmaxf(b.x0 - r0.x, r0.x - b.x1), 0.0f);
-  const float gy = fmaxf(fmaxf(b.y0 - r0.y, r0.y - b.y1), 0.0f);
-  return wave_any(gx <= r0.z && gy <= r0.w);
+// Lanes below this one whose bit is set in mask.
+__device__ __forceinline__ int lanes_below(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
 }
 
-// LDS size (bytes) of one workgroup; must agree with the carve-up in
-// process_tile.
-// Per-wave accumulator row: chi2 (double) then MAXSOURCES*NPARAMS gradient
-// sums (float; each covers one tile's pixels).
+// ------------------------------------------------------------------------
+// LDS carve-up of one workgroup
+//   records  gauss_cap x kRec floats   ImageGaussians of the current chunk,
+//                                       only those that reach the tile, in
+//                                       (source, component) order
+//   part     W x 64*PPL floats         per-wave partial models
+//   cwl      MAXSOURCES x 4 floats     CW of every source for the exposure
+//   rows     W x kRowBytes             per-wave accumulators: chi2 (double)
+//                                       then MAXSOURCES*NPARAMS gradient sums
+//   index    gauss_cap + 1 ints        compaction scratch, then the count
+// ------------------------------------------------------------------------
 constexpr int kRowBytes = 8 + MAXSOURCES * NPARAMS * 4;
 
 __host__ __device__ constexpr size_t lds_bytes_for(int waves, int ppl, int gauss_cap) {
-  return (size_t)gauss_cap * kRec * sizeof(float)      // ImageGaussian records
-         + (size_t)waves * ppl * 64 * sizeof(float)     // partial models
-         + (size_t)MAXSOURCES * 4 * sizeof(float)       // CW per source
-         + (size_t)waves * kRowBytes;                   // per-wave accumulators
+  return (size_t)gauss_cap * kRec * sizeof(float) + (size_t)waves * ppl * 64 * sizeof(float) +
+         (size_t)MAXSOURCES * 4 * sizeof(float) + (size_t)waves * kRowBytes +
+         (size_t)(gauss_cap + 1) * sizeof(int);
 }
 
-// Accessors of a per-wave accumulator row.
+struct Lds {
+  float *recs;
+  float *part;
+  float *cwl;
+  unsigned char *rows;
+  int *index;
+};
+
+__device__ __forceinline__ Lds carve(unsigned char *smem, int waves, int ppl, int gauss_cap) {
+  Lds l;
+  l.recs = reinterpret_cast<float *>(smem);
+  l.part = l.recs + (size_t)gauss_cap * kRec;
+  l.cwl = l.part + waves * 64 * ppl;
+  l.rows = reinterpret_cast<unsigned char *>(l.cwl + MAXSOURCES * 4);
+  l.index = reinterpret_cast<int *>(l.rows + waves * kRowBytes);
+  return l;
+}
+
 __device__ __forceinline__ double &row_chi2(unsigned char *row) {
   return *reinterpret_cast<double *>(row);
 }
@@ -133,32 +150,63 @@ __device__ __forceinline__ float *row_grad(unsigned char *row) {
   return reinterpret_cast<float *>(row + 8);
 }
 
-// Build records for sources [s0, s0 + ns) of exposure e into recs.
+// Pixel bounding box of the tile (all wavefronts hold the same pixels);
+// empty (x0 > x1) when no pixel is valid.
+struct Box {
+  float x0, x1, y0, y1;
+};
+
+// Build the records of the Gaussians of sources [s0, s0 + ns) that can reach
+// the box, compacted in (source, component) order; returns their count.
+// Phase 1 flags every Gaussian (one lane each), phase 2 compacts the flags
+// into an index list with one wavefront (ballot + prefix count), phase 3
+// builds the surviving records.
 template <int W>
-__device__ __forceinline__ void build_chunk(const FpxPatch *__restrict__ P,
-                                            const FpxSource *__restrict__ props,
-                                            int e, int band, int s0, int ns,
-                                            int n_psf, float *__restrict__ recs) {
+__device__ int build_chunk(const FpxPatch *__restrict__ P, const FpxSource *__restrict__ props,
+                           int e, int band, int s0, int ns, int n_psf, const Box &box,
+                           const Lds &l, int wave, int lane) {
   const int n = ns * n_psf;
+  int *index = l.index;
+  __syncthreads();  // previous users of the records and index are done
   for (int lg = threadIdx.x; lg < n; lg += W * 64) {
     const int sl = lg / n_psf;
-    const int p = lg - sl * n_psf;
-    build_record(P, props, e, band, s0 + sl, p, recs + lg * kRec);
+    index[lg] = gauss_reaches(P, props, e, s0 + sl, lg - sl * n_psf, box.x0, box.x1,
+                              box.y0, box.y1) ? 1 : 0;
   }
+  __syncthreads();
+  if (wave == 0) {
+    int count = 0;
+    for (int b0 = 0; b0 < n; b0 += 64) {
+      const int lg = b0 + lane;
+      const bool keep = lg < n && index[lg] != 0;
+      const unsigned long long m = __builtin_amdgcn_ballot_w64(keep);
+      // every lane has read its flag above; writes land at or below b0 + lane
+      if (keep) index[count + lanes_below(m)] = lg;
+      count += __builtin_popcountll(m);
+    }
+    if (lane == 0) index[n > 0 ? n : 0] = count;
+  }
+  __syncthreads();
+  const int n_act = __builtin_amdgcn_readfirstlane(index[n]);
+  for (int i = threadIdx.x; i < n_act; i += W * 64) {
+    const int lg = index[i];
+    const int sl = lg / n_psf;
+    build_record(P, props, e, band, s0 + sl, lg - sl * n_psf, l.recs + i * kRec);
+  }
+  __syncthreads();
+  return n_act;
 }
 
 // Pass 1: add this wave's slice [gb, ge) of the records to the lane's model.
 template <int PPL>
-__device__ __forceinline__ void model_pass(const float *__restrict__ recs, int gb,
-                                           int ge, const float (&xp)[PPL],
-                                           const float (&yp)[PPL], const Box &box,
+__device__ __forceinline__ void model_pass(const float *__restrict__ recs, int gb, int ge,
+                                           const float (&xp)[PPL], const float (&yp)[PPL],
                                            float (&m)[PPL]) {
   const float4 *R4 = reinterpret_cast<const float4 *>(recs);
   for (int g = gb; g < ge; ++g) {
     const float4 r0 = R4[g * 6 + 0];
     const float4 r1 = R4[g * 6 + 1];
     const float amp = recs[g * kRec + kAmp];
-    if (!box_hits(r0, box)) continue;
     const float xc = r0.x, yc = r0.y, fxx = r1.x, fxy = r1.y;
     const float fyy = r1.z, h0 = r1.w;
     float vx[PPL], vy[PPL], ea[PPL];
@@ -184,109 +232,117 @@ __device__ __forceinline__ void model_pass(const float *__restrict__ recs, int g
   }
 }
 
-// Pass 2: gradient accumulation over the wave's slice [gb, ge) of a chunk
-// whose first source is s0; flushes one source at a time into wrow.
+// Reduce one source's 7 accumulators over the wavefront and add them (with
+// the ra/dec Jacobian CW applied) to the wave's row.
+__device__ __forceinline__ void flush_source(int s, float aF, float aX, float aY, float aQ,
+                                             float aP, float aN, float aR,
+                                             const float *__restrict__ cwl,
+                                             unsigned char *__restrict__ wrow, int lane) {
+  aF = wave_sum(aF);
+  aX = wave_sum(aX);
+  aY = wave_sum(aY);
+  aQ = wave_sum(aQ);
+  aP = wave_sum(aP);
+  aN = wave_sum(aN);
+  aR = wave_sum(aR);
+  if (lane == 0) {
+    const float *cw = cwl + 4 * s;  // [v11 v12 v21 v22] = dpix/dsky
+    float *row = row_grad(wrow) + NPARAMS * s;
+    row[0] += aF;
+    row[1] += cw[0] * aX + cw[2] * aY;
+    row[2] += cw[1] * aX + cw[3] * aY;
+    row[3] += aQ;
+    row[4] += aP;
+    row[5] += aN;
+    row[6] += aR;
+  }
+}
+
+// Pass 2: gradient accumulation over the wave's slice [gb, ge) of the
+// records; a source's sums are flushed when the next record belongs to
+// another source.
 template <int PPL>
-__device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int gb,
-                                          int ge, int s0, int n_psf,
+__device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int gb, int ge,
                                           const float (&xp)[PPL], const float (&yp)[PPL],
-                                          const float (&w2)[PPL], const Box &box,
-                                          const float *__restrict__ cwl,
+                                          const float (&w2)[PPL], const float *__restrict__ cwl,
                                           unsigned char *__restrict__ wrow, int lane) {
   const float4 *R4 = reinterpret_cast<const float4 *>(recs);
-  int g = gb;
-  while (g < ge) {
-    const int sl = g / n_psf;
-    const int gstop = min(ge, (sl + 1) * n_psf);
-    float aF = 0.f, aX = 0.f, aY = 0.f, aQ = 0.f, aP = 0.f, aN = 0.f, aR = 0.f;
-    bool touched = false;
-    for (; g < gstop; ++g) {
-      const float4 r0 = R4[g * 6 + 0];
-      const float4 r1 = R4[g * 6 + 1];
-      if (!box_hits(r0, box)) continue;
-      const float xc = r0.x, yc = r0.y, fxx = r1.x, fxy = r1.y;
-      const float fyy = r1.z, h0 = r1.w;
-      float dx[PPL], dy[PPL], vx[PPL], vy[PPL], px[PPL], py[PPL], ea[PPL];
-      bool on[PPL];
-      bool any = false;
-#pragma unroll
-      for (int i = 0; i < PPL; ++i) {
-        dx[i] = xp[i] - xc;
-        dy[i] = yp[i] - yc;
-        vx[i] = fxx * dx[i] + fxy * dy[i];
-        vy[i] = fxy * dx[i] + fyy * dy[i];
-        px[i] = dx[i] * vx[i];
-        py[i] = dy[i] * vy[i];
-        ea[i] = px[i] + py[i];
-        on[i] = !(ea[i] > kMaxExpArg);
-        any |= on[i];
-      }
-      if (!wave_any(any)) continue;
-      touched = true;
-      const float4 r2 = R4[g * 6 + 2];
-      const float4 r3 = R4[g * 6 + 3];
-      const float4 r4 = R4[g * 6 + 4];
-      const float4 r5 = R4[g * 6 + 5];
-      const float amp = r2.x, amp12 = r2.y, dAF = r2.z, dAN = r2.w;
-      const float dAR = r3.x, dAQ = r3.y, kq = r3.z, qxx = r3.w;
-      const float qyy = r4.x, qxy = r4.y, dAP = r4.z, kp = r4.w;
-      const float pxx = r5.x, pyy = r5.y, pxy = r5.z;
-#pragma unroll
-      for (int i = 0; i < PPL; ++i) {
-        const float G = on[i] ? __builtin_amdgcn_exp2f(kExp2Scale * ea[i]) : 0.0f;
-        const float gw = G * w2[i];
-        const float s2 = vx[i] * vx[i] + vy[i] * vy[i];
-        const float H = fmaf(s2, 1.0f / 24.0f, h0);
-        const float gH = gw * H;
-        aF = fmaf(dAF, gH, aF);
-        aN = fmaf(dAN, gH, aN);
-        aR = fmaf(dAR, gH, aR);
-        const float C = amp * gH;
-        const float c12 = amp12 * gw;
-        const float ux = fxx * vx[i] + fxy * vy[i];
-        const float uy = fxy * vx[i] + fyy * vy[i];
-        aX = fmaf(C, vx[i], aX);
-        aX = fmaf(-c12, ux, aX);
-        aY = fmaf(C, vy[i], aY);
-        aY = fmaf(-c12, uy, aY);
-        const float dxx = dx[i] * dx[i];
-        const float dyy = dy[i] * dy[i];
-        const float dxy2 = (dx[i] + dx[i]) * dy[i];
-        const float cr = dy[i] * vx[i] + dx[i] * vy[i];
-        const float hC = -0.5f * C;
-        const float t1q = kq + qxx * px[i] + qyy * py[i] + qxy * cr;
-        const float t2q = qxx * dxx + qyy * dyy + qxy * dxy2;
-        aQ = fmaf(dAQ, gH, aQ);
-        aQ = fmaf(c12, t1q, aQ);
-        aQ = fmaf(hC, t2q, aQ);
-        const float t1p = kp + pxx * px[i] + pyy * py[i] + pxy * cr;
-        const float t2p = pxx * dxx + pyy * dyy + pxy * dxy2;
-        aP = fmaf(dAP, gH, aP);
-        aP = fmaf(c12, t1p, aP);
-        aP = fmaf(hC, t2p, aP);
-      }
+  float aF = 0.f, aX = 0.f, aY = 0.f, aQ = 0.f, aP = 0.f, aN = 0.f, aR = 0.f;
+  bool touched = false;
+  int cur = -1;
+  for (int g = gb; g < ge; ++g) {
+    const float4 r0 = R4[g * 6 + 0];
+    const float4 r1 = R4[g * 6 + 1];
+    const float4 r2 = R4[g * 6 + 2];
+    const float4 r3 = R4[g * 6 + 3];
+    const float4 r4 = R4[g * 6 + 4];
+    const float4 r5 = R4[g * 6 + 5];
+    const int src = __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, r5.w));
+    if (src != cur) {
+      if (touched) flush_source(cur, aF, aX, aY, aQ, aP, aN, aR, cwl, wrow, lane);
+      aF = aX = aY = aQ = aP = aN = aR = 0.f;
+      touched = false;
+      cur = src;
     }
-    if (!touched) continue;  // no Gaussian of this source reached the wave's pixels
-    aF = wave_sum(aF);
-    aX = wave_sum(aX);
-    aY = wave_sum(aY);
-    aQ = wave_sum(aQ);
-    aP = wave_sum(aP);
-    aN = wave_sum(aN);
-    aR = wave_sum(aR);
-    if (lane == 0) {
-      const int s = s0 + sl;
-      const float *cw = cwl + 4 * s;  // [v11 v12 v21 v22] = dpix/dsky
-      float *row = row_grad(wrow) + NPARAMS * s;
-      row[0] += aF;
-      row[1] += cw[0] * aX + cw[2] * aY;
-      row[2] += cw[1] * aX + cw[3] * aY;
-      row[3] += aQ;
-      row[4] += aP;
-      row[5] += aN;
-      row[6] += aR;
+    const float xc = r0.x, yc = r0.y, fxx = r1.x, fxy = r1.y;
+    const float fyy = r1.z, h0 = r1.w;
+    float dx[PPL], dy[PPL], vx[PPL], vy[PPL], px[PPL], py[PPL], ea[PPL];
+    bool on[PPL];
+    bool any = false;
+#pragma unroll
+    for (int i = 0; i < PPL; ++i) {
+      dx[i] = xp[i] - xc;
+      dy[i] = yp[i] - yc;
+      vx[i] = fxx * dx[i] + fxy * dy[i];
+      vy[i] = fxy * dx[i] + fyy * dy[i];
+      px[i] = dx[i] * vx[i];
+      py[i] = dy[i] * vy[i];
+      ea[i] = px[i] + py[i];
+      on[i] = !(ea[i] > kMaxExpArg);
+      any |= on[i];
+    }
+    if (!wave_any(any)) continue;
+    touched = true;
+    const float amp = r2.x, amp12 = r2.y, dAF = r2.z, dAN = r2.w;
+    const float dAR = r3.x, dAQ = r3.y, kq = r3.z, qxx = r3.w;
+    const float qyy = r4.x, qxy = r4.y, dAP = r4.z, kp = r4.w;
+    const float pxx = r5.x, pyy = r5.y, pxy = r5.z;
+#pragma unroll
+    for (int i = 0; i < PPL; ++i) {
+      const float G = on[i] ? __builtin_amdgcn_exp2f(kExp2Scale * ea[i]) : 0.0f;
+      const float gw = G * w2[i];
+      const float s2 = vx[i] * vx[i] + vy[i] * vy[i];
+      const float H = fmaf(s2, 1.0f / 24.0f, h0);
+      const float gH = gw * H;
+      aF = fmaf(dAF, gH, aF);
+      aN = fmaf(dAN, gH, aN);
+      aR = fmaf(dAR, gH, aR);
+      const float C = amp * gH;
+      const float c12 = amp12 * gw;
+      const float ux = fxx * vx[i] + fxy * vy[i];
+      const float uy = fxy * vx[i] + fyy * vy[i];
+      aX = fmaf(C, vx[i], aX);
+      aX = fmaf(-c12, ux, aX);
+      aY = fmaf(C, vy[i], aY);
+      aY = fmaf(-c12, uy, aY);
+      const float dxx = dx[i] * dx[i];
+      const float dyy = dy[i] * dy[i];
+      const float dxy2 = (dx[i] + dx[i]) * dy[i];
+      const float cr = dy[i] * vx[i] + dx[i] * vy[i];
+      const float hC = -0.5f * C;
+      const float t1q = kq + qxx * px[i] + qyy * py[i] + qxy * cr;
+      const float t2q = qxx * dxx + qyy * dyy + qxy * dxy2;
+      aQ = fmaf(dAQ, gH, aQ);
+      aQ = fmaf(c12, t1q, aQ);
+      aQ = fmaf(hC, t2q, aQ);
+      const float t1p = kp + pxx * px[i] + pyy * py[i] + pxy * cr;
+      const float t2p = pxx * dxx + pyy * dyy + pxy * dxy2;
+      aP = fmaf(dAP, gH, aP);
+      aP = fmaf(c12, t1p, aP);
+      aP = fmaf(hC, t2p, aP);
     }
   }
+  if (touched) flush_source(cur, aF, aX, aY, aQ, aP, aN, aR, cwl, wrow, lane);
 }
 
 template <int W, int PPL>
@@ -299,19 +355,15 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
   // the wave index is uniform across the wavefront; saying so puts the
   // Gaussian loop bounds below in scalar registers
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  float *recs = reinterpret_cast<float *>(smem);
-  float *part = recs + (size_t)gauss_cap * kRec;
-  float *cwl = part + W * kPix;
-  unsigned char *wacc = reinterpret_cast<unsigned char *>(cwl + MAXSOURCES * 4);
+  const Lds l = carve(smem, W, PPL, gauss_cap);
   const int spc = max(1, gauss_cap / n_psf);  // sources per LDS chunk
   const int n_chunks = (n_src + spc - 1) / spc;
   const bool vjp = slot.pixel_weight != nullptr;
 
-  for (int j = tid; j < W * kRowBytes / 4; j += W * 64)
-    reinterpret_cast<float *>(wacc)[j] = 0.0f;
+  for (int j = tid; j < W * kRowBytes / 4; j += W * 64) reinterpret_cast<float *>(l.rows)[j] = 0.0f;
   for (int j = tid; j < 4 * n_src; j += W * 64) {
     const int s = j >> 2;
-    cwl[j] = P->CW[4 * (n_src * e + s) + (j & 3)];
+    l.cwl[j] = P->CW[4 * (n_src * e + s) + (j & 3)];
   }
 
   float xp[PPL], yp[PPL], w2[PPL];
@@ -344,21 +396,19 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
     box.y1 = uniform_f(wave_max(y1));
   }
 
+  int n_act = 0;
   if (!vjp) {
     float m[PPL];
 #pragma unroll
     for (int i = 0; i < PPL; ++i) m[i] = 0.0f;
     for (int c = 0; c < n_chunks; ++c) {
       const int s0 = c * spc;
-      const int ns = min(spc, n_src - s0);
-      __syncthreads();
-      build_chunk<W>(P, props, e, band, s0, ns, n_psf, recs);
-      __syncthreads();
-      const int n = ns * n_psf;
-      model_pass<PPL>(recs, (n * wave) / W, (n * (wave + 1)) / W, xp, yp, box, m);
+      n_act = build_chunk<W>(P, props, e, band, s0, min(spc, n_src - s0), n_psf, box, l, wave,
+                             lane);
+      model_pass<PPL>(l.recs, (n_act * wave) / W, (n_act * (wave + 1)) / W, xp, yp, m);
     }
 #pragma unroll
-    for (int i = 0; i < PPL; ++i) part[(wave * PPL + i) * 64 + lane] = m[i];
+    for (int i = 0; i < PPL; ++i) l.part[(wave * PPL + i) * 64 + lane] = m[i];
     __syncthreads();
     double chi2 = 0.0;
     float *resid = P->residual;
@@ -366,7 +416,7 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
     for (int i = 0; i < PPL; ++i) {
       float model = 0.0f;
 #pragma unroll
-      for (int w = 0; w < W; ++w) model += part[(w * PPL + i) * 64 + lane];
+      for (int w = 0; w < W; ++w) model += l.part[(w * PPL + i) * 64 + lane];
       const float d = valid[i] ? P->data[pix[i]] : 0.0f;
       const float ie = valid[i] ? P->ierr[pix[i]] : 0.0f;
       const float r = d - model;
@@ -378,7 +428,7 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
     }
     if (wave == 0) {
       chi2 = wave_sum(chi2);
-      if (lane == 0) row_chi2(wacc) += chi2;
+      if (lane == 0) row_chi2(l.rows) += chi2;
     }
   } else {
 #pragma unroll
@@ -390,19 +440,15 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
   for (int i = 0; i < PPL; ++i) wave_has_weight |= (w2[i] != 0.0f);
   wave_has_weight = wave_any(wave_has_weight);
 
-  unsigned char *wrow = wacc + wave * kRowBytes;
+  unsigned char *wrow = l.rows + wave * kRowBytes;
   for (int c = 0; c < n_chunks; ++c) {
     const int s0 = c * spc;
-    const int ns = min(spc, n_src - s0);
-    if (n_chunks > 1 || vjp) {
-      __syncthreads();
-      build_chunk<W>(P, props, e, band, s0, ns, n_psf, recs);
-      __syncthreads();
-    }
-    const int n = ns * n_psf;
+    if (n_chunks > 1 || vjp)
+      n_act = build_chunk<W>(P, props, e, band, s0, min(spc, n_src - s0), n_psf, box, l, wave,
+                             lane);
     if (wave_has_weight)
-      grad_pass<PPL>(recs, (n * wave) / W, (n * (wave + 1)) / W, s0, n_psf, xp, yp, w2, box,
-                     cwl, wrow, lane);
+      grad_pass<PPL>(l.recs, (n_act * wave) / W, (n_act * (wave + 1)) / W, xp, yp, w2, l.cwl,
+                     wrow, lane);
   }
 }
 
@@ -412,8 +458,7 @@ template <int W>
 __device__ void finish_tile(const FpxSlot &slot, int band, int n_src, unsigned char *smem,
                             int gauss_cap, int ppl, int *s_flag) {
   const int tid = threadIdx.x;
-  float *cwl = reinterpret_cast<float *>(smem) + (size_t)gauss_cap * kRec + W * 64 * ppl;
-  unsigned char *wacc = reinterpret_cast<unsigned char *>(cwl + MAXSOURCES * 4);
+  const Lds l = carve(smem, W, ppl, gauss_cap);
   if (band >= 0) {
     double *acc = slot.accum + (size_t)band * kAcc;
     const int nv = 1 + NPARAMS * n_src;
@@ -421,7 +466,7 @@ __device__ void finish_tile(const FpxSlot &slot, int band, int n_src, unsigned c
       double v = 0.0;
 #pragma unroll
       for (int w = 0; w < W; ++w) {
-        unsigned char *row = wacc + w * kRowBytes;
+        unsigned char *row = l.rows + w * kRowBytes;
         v += (j == 0) ? row_chi2(row) : (double)row_grad(row)[j - 1];
       }
       if (v != 0.0) unsafeAtomicAdd(acc + j, v);
@@ -445,8 +490,7 @@ __device__ void finish_tile(const FpxSlot &slot, int band, int n_src, unsigned c
       if (k == 0) {
         slot.pchi2[b] = v;
       } else {
-        const int q = k - 1;
-        slot.pdchi2_dp[b * (MAXSOURCES * NPARAMS) + q] = (float)v;
+        slot.pdchi2_dp[b * (MAXSOURCES * NPARAMS) + (k - 1)] = (float)v;
       }
       slot.accum[j] = 0.0;
     }
@@ -641,7 +685,7 @@ int check_cfg(int32_t cfg) {
 // Largest Gaussian count per LDS chunk: all of them if they fit.
 int pick_gauss_cap(const KernelCfg &kc, int max_gauss, int lds_max) {
   const size_t fixed = fpx::lds_bytes_for(kc.waves, kc.ppl, 0);
-  const size_t per = fpx::kRec * sizeof(float);
+  const size_t per = fpx::lds_bytes_for(kc.waves, kc.ppl, 1) - fixed;  // record + index slot
   const long fit = ((long)lds_max - (long)fixed) / (long)per;
   return (int)std::min<long>(max_gauss, fit);
 }

@@ -190,9 +190,28 @@ class HIPPatchMixin(DevicePatchMixin):
         buf = self.device_ptrs[arrname]
         return buf.cpu().numpy().view(self.pix_dtype)[:self.npix].copy()
 
+    def tile_order(self, tiles):
+        """Permutation of a tile table putting tiles near a source first: the
+        work queue then hands out the expensive tiles early.  Distance from
+        the tile's pixel centroid to the nearest source centre of the packed
+        scene (crpix), in pixels."""
+        if len(tiles) == 0 or getattr(self, "crpix", None) is None:
+            return np.arange(len(tiles))
+        n = tiles["pix_count"].astype(np.int64)
+        idx = np.repeat(np.arange(len(tiles)), n)
+        first = np.cumsum(n) - n
+        pix = np.arange(n.sum()) - first[idx] + tiles["pix_start"].astype(np.int64)[idx]
+        count = np.maximum(n, 1)
+        cx = np.bincount(idx, self.xpix[pix].astype(np.float64), len(tiles)) / count
+        cy = np.bincount(idx, self.ypix[pix].astype(np.float64), len(tiles)) / count
+        c = np.asarray(self.crpix, dtype=np.float64).reshape(int(self.n_exp), -1, 2)
+        c = c[tiles["exposure"].astype(np.int64)]                 # (n_tiles, n_src, 2)
+        d = np.hypot(c[..., 0] - cx[:, None], c[..., 1] - cy[:, None]).min(axis=1)
+        return np.argsort(d, kind="stable")
+
     def device_tiles(self, pix_per_tile):
-        """Device FpxTile table of this patch for a tile size (cached);
-        returns (tensor, n_tiles)."""
+        """Device FpxTile table of this patch for a tile size (cached), ordered
+        by ``tile_order``; returns (tensor, n_tiles)."""
         if pix_per_tile not in self._tiles:
             L = lib()
             args = [np.ascontiguousarray(self.exposure_start, dtype=np.int32),
@@ -205,6 +224,7 @@ class HIPPatchMixin(DevicePatchMixin):
             tiles = np.zeros(max(n, 1), dtype=TILE_DTYPE)
             check(L.fpx_plan_tiles(*ptrs, int(self.n_bands), int(pix_per_tile),
                                    tiles.ctypes.data, n), "fpx_plan_tiles")
+            tiles[:n] = tiles[:n][self.tile_order(tiles[:n])]
             self._tiles[pix_per_tile] = (device_bytes(tiles, self._dev()), n)
         return self._tiles[pix_per_tile]
 

@@ -67,20 +67,18 @@ class HIPProposer(ProposerBase):
     patch : HIPPatchMixin patch, optional default patch
     config : int, kernel configuration (``fpx_config_info``); default
         DEFAULT_CONFIG
+    persistent : launch a resident grid that pulls tiles from a queue
+        (default) or one workgroup per tile
     """
 
-    def __init__(self, patch=None, config=None, chi_dtype=np.float64):
+    def __init__(self, patch=None, config=None, persistent=True, chi_dtype=np.float64):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("HIPProposer needs a ROCm device")
         self.patch = patch
         self.chi_dtype = chi_dtype
-        self.config = DEFAULT_CONFIG if config is None else int(config)
-        L = lib()
-        self.waves, self.ppl, self.pix_per_tile = (np.zeros(1, np.int32) for _ in range(3))
-        check(L.fpx_config_info(self.config, self.waves.ctypes.data, self.ppl.ctypes.data,
-                                self.pix_per_tile.ctypes.data), "fpx_config_info")
-        self.pix_per_tile = int(self.pix_per_tile[0])
+        self.persistent = bool(persistent)
+        self.set_config(DEFAULT_CONFIG if config is None else config)
         self.device = torch.device("cuda", torch.cuda.current_device())
         dev = self.device
         n_acc = L.fpx_accum_doubles(MAXBANDS)
@@ -91,12 +89,49 @@ class HIPProposer(ProposerBase):
         self._prop = torch.zeros(MAXSOURCES * source_struct_dtype.itemsize, dtype=torch.uint8,
                                  device=dev)
         self._slot = torch.zeros(SLOT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+        self._queue = torch.zeros(2, dtype=torch.int32, device=dev)
         pin = dict(pin_memory=True)
         self._prop_h = torch.zeros(self._prop.numel(), dtype=torch.uint8, **pin)
         self._chi2_h = torch.zeros(MAXBANDS, dtype=torch.float64, **pin)
         self._resp_h = torch.zeros(self._resp.numel(), dtype=torch.float32, **pin)
         self._slot_key = None
         self._h2d_done = None   # event of the last copy out of the pinned proposal buffer
+
+    def set_config(self, config):
+        """Select the kernel configuration (tile shape, ``fpx_config_info``)."""
+        import ctypes
+        ppt = ctypes.c_int32()
+        check(lib().fpx_config_info(int(config), None, None, ctypes.byref(ppt)), "fpx_config_info")
+        self.config = int(config)
+        self.pix_per_tile = ppt.value
+        self._slot_key = None
+
+    def autotune(self, proposal, patch=None, repeats=5, configs=None, persistent=(True, False)):
+        """Time every (configuration, persistent) pair on ``proposal`` and keep
+        the fastest; returns {(config, persistent): seconds per evaluation}."""
+        import torch
+        if patch is None:
+            patch = self.patch
+        configs = range(lib().fpx_n_configs()) if configs is None else configs
+        timings = {}
+        for cfg in configs:
+            self.set_config(cfg)
+            for pers in persistent:
+                self.persistent = pers
+                self.launch(proposal, patch)
+                torch.cuda.synchronize()
+                a = torch.cuda.Event(enable_timing=True)
+                b = torch.cuda.Event(enable_timing=True)
+                a.record(self._stream())
+                for _ in range(repeats):
+                    self.launch(proposal, patch)
+                b.record(self._stream())
+                torch.cuda.synchronize()
+                timings[(cfg, pers)] = a.elapsed_time(b) * 1e-3 / repeats
+        best = min(timings, key=timings.get)
+        self.set_config(best[0])
+        self.persistent = best[1]
+        return timings
 
     def _stream(self):
         import torch
@@ -146,9 +181,15 @@ class HIPProposer(ProposerBase):
             self._prop[:proposal.numel()].copy_(proposal, non_blocking=True)
         n_tiles = self._slot_for(patch, pixel_weight)
         stream = self._stream()
-        check(lib().fpx_evaluate_proposal(self._slot.data_ptr(), n_tiles, patch.max_gauss(),
-                                          self.config, stream.cuda_stream),
-              "fpx_evaluate_proposal")
+        if self.persistent:
+            check(lib().fpx_evaluate_batch(self._slot.data_ptr(), 1, None, n_tiles,
+                                           patch.max_gauss(), self.config, 1,
+                                           self._queue.data_ptr(), stream.cuda_stream),
+                  "fpx_evaluate_batch")
+        else:
+            check(lib().fpx_evaluate_proposal(self._slot.data_ptr(), n_tiles, patch.max_gauss(),
+                                              self.config, stream.cuda_stream),
+                  "fpx_evaluate_proposal")
         return n_tiles
 
     def evaluate_proposal(self, proposal, patch=None, verbose=False, unpack=True):

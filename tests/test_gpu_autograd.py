@@ -45,10 +45,11 @@ def test_autograd_matches_kernel_gradient_and_fp64():
     g_kernel = stack_grad(d, model.n_sources, model.n_bands)
     np.testing.assert_allclose(chi2.item(), c_band.sum(), rtol=1e-6)
     # float64 reference of both
-    c64, g64, _, mass = fp64.evaluate(model.patch, model.proposal(th), mass=True)
+    c64, g64, mass, smass = U.fp64_reference(model.patch, model.proposal(th))
     g64s = stack_grad(g64, model.n_sources, model.n_bands)
     ms = stack_grad(mass, model.n_sources, model.n_bands)
-    tol = U.GRAD_RTOL * np.abs(g64s) + U.GRAD_FLOOR * ms
+    ss = stack_grad(smass, model.n_sources, model.n_bands)
+    tol = U.GRAD_RTOL * np.abs(g64s) + U.GRAD_FLOOR * ms + U.MODEL_EPS * ss
     assert np.all(np.abs(g_auto - g64s) <= tol)
     assert np.all(np.abs(g_kernel - g64s) <= tol)
 

@@ -46,17 +46,17 @@ def batch_setup():
     for (patch, scene), nc in zip(pats, chains):
         for c in range(nc):
             c32, _, _ = oracle.evaluate_cpu(patch, proposals[k])
-            c64, g64, _, mass = fp64.evaluate(patch, proposals[k], mass=True)
-            refs.append((c32, g64, mass))
+            c64, g64, mass, smass = U.fp64_reference(patch, proposals[k])
+            refs.append((c32, g64, mass, smass))
             k += 1
     return pats, chains, proposals, refs
 
 
 def _check(results, refs):
-    for (chi2, d), (c32, g64, mass) in zip(results, refs):
+    for (chi2, d), (c32, g64, mass, smass) in zip(results, refs):
         rel = np.abs(chi2 - c32) / np.abs(c32)
         assert rel.max() <= U.CHI2_RTOL, rel
-        assert U.grad_errors(d, g64, mass).max() <= 1.0
+        assert U.grad_errors(d, g64, mass, smass).max() <= 1.0
 
 
 @pytest.mark.parametrize("persistent", [True, False])

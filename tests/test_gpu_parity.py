@@ -20,7 +20,7 @@ def fp64_cache():
 
 def _fp64(name, patch, proposal, cache):
     if name not in cache:
-        cache[name] = fp64.evaluate(patch, proposal, mass=True)
+        cache[name] = U.fp64_reference(patch, proposal)
     return cache[name]
 
 
@@ -41,12 +41,12 @@ def test_fixture_parity(name, fp64_cache):
     rel = np.abs(chi2 - expected["chi2"]) / np.maximum(np.abs(expected["chi2"]), 1e-300)
     assert np.all(rel <= U.CHI2_RTOL), (name, rel)
     # gradients against the float64 oracle
-    c64, g64, r64, mass = _fp64(name, patch, proposal, fp64_cache)
-    err = U.grad_errors(dchi2, g64, mass)
+    c64, g64, mass, smass = _fp64(name, patch, proposal, fp64_cache)
+    err = U.grad_errors(dchi2, g64, mass, smass)
     assert err.max() <= 1.0, (name, err.max(), np.unravel_index(err.argmax(), err.shape))
     # and against the reference CPU kernel, whose own float32 error is included
-    err_ref = U.grad_errors(expected["dchi2"], g64, mass)
-    scale = U.GRAD_RTOL * np.abs(g64) + U.GRAD_FLOOR * mass
+    err_ref = U.grad_errors(expected["dchi2"], g64, mass, smass)
+    scale = U.GRAD_RTOL * np.abs(g64) + U.GRAD_FLOOR * mass + U.MODEL_EPS * smass
     both = np.abs(dchi2 - expected["dchi2"]) / np.where(scale > 0, scale, 1e-300)
     assert np.all(both <= err + err_ref + 1e-9), name
     # residual image against the reference, relative to the data scale
@@ -59,10 +59,10 @@ def test_all_configs_agree(config):
     name = "multi_small_s2"
     print(f"[gpu] config {config} on {name}", flush=True)
     patch, proposal, expected, (chi2, dchi2, resid) = _evaluate(name, config=config)
-    c64, g64, r64, mass = fp64.evaluate(patch, proposal, mass=True)
+    c64, g64, mass, smass = U.fp64_reference(patch, proposal)
     rel = np.abs(chi2 - expected["chi2"]) / np.abs(expected["chi2"])
     assert rel.max() <= U.CHI2_RTOL
-    assert U.grad_errors(dchi2, g64, mass).max() <= 1.0
+    assert U.grad_errors(dchi2, g64, mass, smass).max() <= 1.0
 
 
 def test_repeatable_and_no_residual():
