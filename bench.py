@@ -9,6 +9,10 @@ Workloads:
            sources, 9 radii x 4 PSF Gaussians = 1.416e8 pixel-Gaussian pairs
   single   1 band x 1 exposure x 64x64 px, 1 source, 9 x 3 Gaussians
 
+The line also carries a "batched" object: --batch-patches uneven patches
+evaluated in one persistent launch, next to the same patches evaluated by one
+plain launch each.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload multi]
 """
 
@@ -24,8 +28,10 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 METRIC = "pixel·Gaussian evals/s (chi2+grad), batched vs plain, % of FP32 VALU peak"
-FLOP_PER_EVAL = 117          # FP32 flop per pixel-Gaussian pair (FMA = 2)
+FLOP_PER_EVAL = 117          # FP32 flop per pair inside the cutoff, both passes (FMA = 2)
+FLOP_OUTSIDE = 22            # flop per pair beyond it: dx, dy, F dx, dx^T F dx in each pass
 PEAK_FP32 = 157.3e12         # MI355X FP32 vector peak, FLOP/s
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "chi2_grad_kernel_multi_band_traffic.txt")
 
 
 def parse():
@@ -39,6 +45,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (counter calibration)")
+    ap.add_argument("--batch-patches", type=int, default=512,
+                    help="uneven patches in the batched measurement (0 skips it)")
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--no-probe", action="store_true", help="skip the VALU rate probe")
     return ap.parse_args()
 
 
@@ -51,19 +61,6 @@ def build_workload(name, seed):
             "9 radii x 4 PSF Gaussians" if name == "multi" else
             "single plain: 1 band x 1 exposure x 64x64 px, 1 source, 9 radii x 3 PSF Gaussians")
     return patch, scene, desc
-
-
-def time_config(prop, patch, proposal_dev, n):
-    import torch
-    s = torch.cuda.current_stream()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(n)]
-    for a, b in ev:
-        a.record(s)
-        prop.launch(proposal_dev, patch)
-        b.record(s)
-    torch.cuda.synchronize()
-    return np.array([a.elapsed_time(b) for a, b in ev]) * 1e-3
 
 
 def valu_probe():
@@ -94,6 +91,135 @@ def valu_probe():
         res[name + "_cycles_per_wave_inst"] = float(cyc / (iters * 8))
         res[name + "_clock_ghz"] = float(cyc / ref * 0.1)
     return res
+
+
+def roofline(n_pairs, n_inside, seconds):
+    """FP32 VALU roofline of one evaluation: the reference algorithm's flops
+    (117 per pair inside the cutoff, 22 per pair beyond it, where the
+    per-pixel loops stop after the exponent) over the kernel time."""
+    flops = n_inside * FLOP_PER_EVAL + max(n_pairs - n_inside, 0) * FLOP_OUTSIDE
+    achieved = flops / seconds
+    return {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+            "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+            "kernel_ms": seconds * 1e3, "pairs": int(n_pairs), "pairs_inside_cutoff": int(n_inside),
+            "flop_per_pair_inside": FLOP_PER_EVAL, "flop_per_pair_outside": FLOP_OUTSIDE,
+            "frac_if_117_per_pair": n_pairs * FLOP_PER_EVAL / seconds / PEAK_FP32}
+
+
+def traffic(cfg):
+    """HBM bytes per launch of the multi-band workload for kernel configuration
+    cfg, from the committed FETCH_SIZE / WRITE_SIZE summary (None if that
+    configuration was not profiled)."""
+    import ctypes
+    from forcepho_amd._lib import lib, check
+    w, p = ctypes.c_int32(), ctypes.c_int32()
+    check(lib().fpx_config_info(cfg, ctypes.byref(w), ctypes.byref(p), None), "fpx_config_info")
+    key = f"{w.value}x{p.value}"
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rows = [json.loads(l) for l in f if l.startswith("{")]
+    except OSError:
+        return None
+    for r in rows:
+        if r.get("kernel") == key:
+            return r["hbm_bytes_per_launch"]
+    return None
+
+
+def build_uneven(n_patches, seed):
+    """n_patches independent uneven patches: 1-20 sources, 1-8 bands, 1-4
+    exposures per band, square stamps of side 48-256 px, 3-6 PSF Gaussians,
+    9 radii (scenes.spec_uneven); data = model + noise."""
+    from forcepho_amd import scenes
+    from forcepho_amd.patches import SimplePatch
+    rng = np.random.default_rng(seed)
+    patches, proposals = [], []
+    for _ in range(n_patches):
+        spec = scenes.spec_uneven(rng)
+        patch, scene, _ = scenes.build_patch(spec, SimplePatch, scenes.render_on_device)
+        patch.return_residual = False
+        patch.send_to_device()
+        prop = scene.get_proposal().copy()
+        prop["q"] *= np.float32(0.97)
+        patches.append(patch)
+        proposals.append(prop)
+    return patches, proposals
+
+
+def time_region(fn, steps):
+    """Seconds per call of ``fn`` over ``steps`` back-to-back calls,
+    bracketed by device synchronisations (HIP events on the current stream)."""
+    import torch
+    s = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(s)
+    for _ in range(steps):
+        fn()
+    b.record(s)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) * 1e-3 / steps
+
+
+def batched_measurement(args, plain):
+    """Batched persistent evaluation of --batch-patches uneven patches in one
+    launch, against the same patches evaluated one plain launch at a time
+    (device-resident launches, and the full evaluate_proposal API with host
+    proposals in and results out)."""
+    import torch
+    from forcepho_amd.batch import BatchEvaluator
+    from forcepho_amd.proposal import HIPProposer
+    t0 = time.perf_counter()
+    patches, proposals = build_uneven(args.batch_patches, args.seed + 1)
+    build_s = time.perf_counter() - t0
+    ev = BatchEvaluator(patches, chains=1)
+    ev.set_proposals(proposals)
+    tuned = ev.autotune(repeats=2)
+    n_evals = ev.n_evals()
+    n_inside = int(ev.count_pairs().sum())
+    ev.launch()
+    batch_s = time_region(ev.launch, args.batch_steps)
+    for (chi2, d) in ev.results():
+        assert np.all(np.isfinite(chi2)) and np.all(np.isfinite(d))
+
+    # one plain launch per patch, same kernel configuration as the plain bench
+    prop = HIPProposer(config=plain.config, persistent=plain.persistent)
+    pdevs = [torch.from_numpy(np.ascontiguousarray(p).view(np.uint8).reshape(-1).copy()).cuda()
+             for p in proposals]
+
+    def sequential():
+        for patch, pd in zip(patches, pdevs):
+            prop.launch(pd, patch)
+    sequential()
+    seq_s = time_region(sequential, 2)
+
+    def sequential_api():
+        for patch, pr in zip(patches, proposals):
+            prop.evaluate_proposal(pr, patch=patch)
+    sequential_api()
+    t1 = time.perf_counter()
+    sequential_api()
+    api_s = time.perf_counter() - t1
+    sides = [int(np.sqrt(p.original_sizes[0])) for p in patches]
+    desc = (f"{len(patches)} uneven patches (1-20 sources, 1-8 bands x 1-4 exposures, "
+            f"48-256 px stamps, 3-6 PSF Gaussians, 9 radii) in one persistent launch")
+    return {
+        "workload": desc,
+        "n_patches": len(patches),
+        "evals_per_step": n_evals,
+        "pixels": int(sum(int(p.npix) for p in patches)),
+        "stamp_side_range": [min(sides), max(sides)],
+        "evals_per_s": n_evals / batch_s,
+        "ms_per_step": batch_s * 1e3,
+        "roofline": roofline(n_evals, n_inside, batch_s),
+        "kernel_config": "{}{}".format(ev.config, "p" if ev.persistent else ""),
+        "config_ms": {f"{c}{'p' if p else ''}": t * 1e3 for (c, p), t in tuned.items()},
+        "sequential_plain_launches": {"evals_per_s": n_evals / seq_s, "ms": seq_s * 1e3},
+        "sequential_evaluate_proposal": {"evals_per_s": n_evals / api_s, "ms": api_s * 1e3},
+        "speedup_vs_sequential_plain": seq_s / batch_s,
+        "speedup_vs_sequential_evaluate_proposal": api_s / batch_s,
+        "host_build_s": build_s,
+    }
 
 
 def cpu_baseline(patch, proposal, n_evals, budget_s=12.0):
@@ -178,8 +304,9 @@ def main():
     assert np.all(np.isfinite(chi2)) and np.all(np.isfinite(derivs))
 
     value = n_evals * args.steps / region
-    achieved = n_evals * FLOP_PER_EVAL / kernel_s
-    probe = valu_probe()
+    n_inside = prop.count_pairs(pdev, patch)
+    probe = None if args.no_probe else valu_probe()
+    batched = batched_measurement(args, prop) if args.batch_patches > 0 else None
     line = {
         "metric": METRIC,
         "value": value,
@@ -196,9 +323,8 @@ def main():
         "config": {"workload": desc, "evals_per_step": n_evals, "path": "plain",
                    "kernel_config": best, "config_ms": {str(k): v * 1e3 for k, v in timings.items()},
                    "wall_s": wall},
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
-                     "kernel_ms": kernel_s * 1e3, "flop_per_eval": FLOP_PER_EVAL},
+        "roofline": dict(roofline(n_evals, n_inside, kernel_s), traffic=traffic(prop.config)),
+        "batched": batched,
         "valu_probe": probe,
     }
     if not args.no_cpu:

@@ -150,6 +150,21 @@ class BatchEvaluator:
             out.append((chi2[s, :nb].copy(), resp[s, :nb, :ns].astype(np.float64)))
         return out
 
+    def count_pairs(self):
+        """Per slot (slot order), the (pixel, ImageGaussian) pairs inside the
+        cutoff dx^T F dx <= MAX_EXP_ARG for the current proposals
+        (fpx_count_pairs)."""
+        torch = _torch()
+        out = torch.zeros(self.n_slots, dtype=torch.int64, device=self.device)
+        s = torch.cuda.current_stream(self.device)
+        check(lib().fpx_count_pairs(self.slots_dev.data_ptr(), self.n_slots,
+                                    self.offsets_dev.data_ptr(), self.n_items, self.max_gauss,
+                                    out.data_ptr(), None, s.cuda_stream), "fpx_count_pairs")
+        counts = out.cpu().numpy()
+        res = np.zeros(self.n_slots, dtype=np.int64)
+        res[self.order] = counts     # slot table position k holds slot order[k]
+        return res
+
     def n_evals(self):
         """Pixel x Gaussian pairs of one batched evaluation."""
         return sum(self.patches[p].n_evals() for p, _ in self.slot_map)

@@ -11,22 +11,33 @@
 //   amp = G a_psf sqrt(det F) / (2 pi) * flux_band * a_mix[bin]
 //   dF/dq = -F dSigma/dq F with dSigma/dq = rad2 (T dT^T + dT T^T)
 //   dA/dq = amp/2 tr(Sigma dF/dq)   (same for pa)
-// The record is laid out for the two pixel passes (fpx_kernel.hip), 24
-// floats (6 float4):
-//   [0..3]   xc yc hx hy      centre and the half-widths of the box that holds
-//                             the whole dx^T F dx <= MAX_EXP_ARG ellipse
-//   [4..8]   fxx fxy fyy h0 amp                       model pass
-//   [9..22]  amp/12 dA/dflux dA/dsersic dA/drh
-//            dA/dq kq dFxx/dq dFyy/dq dFxy/dq
-//            dA/dpa kp dFxx/dpa dFyy/dpa dFxy/dpa    gradient pass
-//   [23]     source index (int bits)
-// with h0 = 1 - (fxx + fyy)/24 and kq = -(dFxx/dq + dFyy/dq)/2 (kp
-// likewise), so that the pixel integral factor
-// H = 1 + (vx^2 + vy^2 - fxx - fyy)/24 of common_kernel.cc:77 is
-// H = h0 + (vx^2 + vy^2)/24.  The ellipse dx^T Sigma^-1 dx <= 36 spans
-// +-6 sqrt(Sigma_xx) in x (sqrt(Sigma_yy) in y); hx, hy add a relative
-// 1e-3 and an absolute 0.01 px margin so that no pixel the float32 test
-// would keep lies outside the box.
+// The record is laid out for the two pixel passes (fpx_kernel.hip), 36
+// floats (9 float4).  With dx = x - xc, dy = y - yc and the pixel sums
+//   a_k = sum_pix gw m_k,  b_k = sum_pix gw H m_k,  m = (1, dx, dy, dx^2, dy^2, dx dy)
+// (gw = G * pixel weight, G the Gaussian, H the pixel-integral factor) the
+// gradient of one Gaussian is a fixed linear map of (a, b); the record holds
+// that map's coefficients so the per-pixel work is the 12 sums only:
+//   [0..8]   xc yc  cxx cxy cyy  h0  sxx sxy syy
+//            exponent log2 G = cxx dx^2 + cxy dx dy + cyy dy^2
+//            (c = -log2(e)/2 (fxx, 2 fxy, fyy)); H = h0 + sxx dx^2 + sxy dx dy
+//            + syy dy^2 (h0 = 1 - (fxx + fyy)/24, s = ((fxx^2 + fxy^2),
+//            2 fxy (fxx + fyy), (fxy^2 + fyy^2))/24), i.e. the
+//            H = 1 + (vx^2 + vy^2 - fxx - fyy)/24, v = F dx of common_kernel.cc:77
+//   [9]      amp                                     model pass: amp G H
+//   [10]     source index (int bits)
+//   [11..13] dA/dflux dA/dsersic dA/drh              x b0
+//   [14..17] ex1 ex2 ex3 ex4, [18..19] ey2 ey4        d/dx: ex1 b1 + ex2 b2 + ex3 a1
+//            + ex4 a2;  d/dy: ex2 b1 + ey2 b2 + ex4 a1 + ey4 a2
+//   [20..27] dA/dq kq0 kq3 kq4 kq5 hq3 hq4 hq5       d/dq: dA/dq b0 + kq0 a0 + kq3 a3
+//            + kq4 a4 + kq5 a5 + hq3 b3 + hq4 b4 + hq5 b5
+//   [28..35] the same eight for pa
+// Expanding the per-pixel terms of common_kernel.cc:95-143 gives, with
+// amp12 = amp/12, dF/dq = [[qxx, qxy], [qxy, qyy]] and kq = -(qxx + qyy)/2:
+//   ex1 = amp fxx, ex2 = amp fxy, ey2 = amp fyy, ex3 = -2 amp sxx,
+//   ex4 = -amp sxy, ey4 = -2 amp syy,
+//   kq0 = amp12 kq, kq3 = amp12 (qxx fxx + qxy fxy), kq4 = amp12 (qyy fyy + qxy fxy),
+//   kq5 = amp12 ((qxx + qyy) fxy + qxy (fxx + fyy)),
+//   hq3 = -amp qxx / 2, hq4 = -amp qyy / 2, hq5 = -amp qxy.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -34,16 +45,18 @@
 
 namespace fpx {
 
-constexpr int kRec = 24;  // floats per ImageGaussian record
+constexpr int kRec = 36;  // floats per ImageGaussian record
+constexpr float kExp2Scale = -0.72134752044448170368f;  // -0.5 * log2(e)
 
 enum RecField {
-  kXc = 0, kYc, kHx, kHy,
-  kFxx, kFxy, kFyy, kH0, kAmp,
-  kAmp12, kDAF, kDAN, kDAR,
-  kDAQ, kKQ, kQxx, kQyy, kQxy,
-  kDAP, kKP, kPxx, kPyy, kPxy,
-  kSrc
+  kXc = 0, kYc, kCxx, kCxy, kCyy, kH0, kSxx, kSxy, kSyy,
+  kAmp, kSrc,
+  kDAF, kDAN, kDAR,
+  kEx1, kEx2, kEx3, kEx4, kEy2, kEy4,
+  kDAQ, kQ0, kQ3, kQ4, kQ5, kHQ3, kHQ4, kHQ5,
+  kDAP, kP0, kP3, kP4, kP5, kHP3, kHP4, kHP5
 };
+static_assert(kHP5 + 1 == kRec, "record layout");
 
 struct M22 {
   float a, b, c, d;  // [[a, b], [c, d]]
@@ -114,6 +127,30 @@ __device__ __forceinline__ void convolved_sigma(const M22 &T, float covar,
   Syy = covar * (T.c * T.c + T.d * T.d) + psf.Cyy;
 }
 
+// F = Sigma^-1 of a symmetric 2x2 Sigma.
+__device__ __forceinline__ void inverse_sigma(float Sxx, float Sxy, float Syy, float &fxx,
+                                              float &fxy, float &fyy) {
+  const float idet = 1.0f / (Sxx * Syy - Sxy * Sxy);
+  fxx = Syy * idet;
+  fxy = -Sxy * idet;
+  fyy = Sxx * idet;
+}
+
+// Centre and F of ImageGaussian (s, p) of exposure e.
+__device__ __forceinline__ void image_gaussian_shape(const FpxPatch *__restrict__ P,
+                                                     const FpxSource *__restrict__ props,
+                                                     int e, int s, int p, float &xc, float &yc,
+                                                     float &fxx, float &fxy, float &fyy) {
+  SourceGeometry g;
+  source_geometry<false>(P, props + s, e, s, g);
+  const FpxPSFSourceGaussian psf = P->psfgauss[P->psfgauss_start[e] + p];
+  float Sxx, Sxy, Syy;
+  convolved_sigma(g.T, P->rad2[psf.sersic_radius_bin], psf, Sxx, Sxy, Syy);
+  inverse_sigma(Sxx, Sxy, Syy, fxx, fxy, fyy);
+  xc = g.xs + psf.xcen;
+  yc = g.ys + psf.ycen;
+}
+
 // Half-width of the box holding dx^T Sigma^-1 dx <= MAX_EXP_ARG along an
 // axis of variance S, with a relative 1e-3 and an absolute 0.01 px margin.
 __device__ __forceinline__ float cutoff_halfwidth(float S) {
@@ -161,10 +198,8 @@ __device__ __forceinline__ void build_record(const FpxPatch *__restrict__ P,
 
   float Sxx, Sxy, Syy;
   convolved_sigma(T, covar, psf, Sxx, Sxy, Syy);
-  const float idet = 1.0f / (Sxx * Syy - Sxy * Sxy);
-  const float fxx = Syy * idet;
-  const float fxy = -Sxy * idet;
-  const float fyy = Sxx * idet;
+  float fxx, fxy, fyy;
+  inverse_sigma(Sxx, Sxy, Syy, fxx, fxy, fyy);
   const float detF = fxx * fyy - fxy * fxy;
   const float tmp = gcal * psf.amp * sqrtf(detF) * 0.15915494309189535f;  // 1/(2 pi)
   const float amp = tmp * flux * amp_s;
@@ -183,30 +218,47 @@ __device__ __forceinline__ void build_record(const FpxPatch *__restrict__ P,
   const float dAP = 0.5f * amp * (Sxx * pxx + 2.0f * Sxy * pxy + Syy * pyy);
 
   const float h0 = 1.0f - (fxx + fyy) * (1.0f / 24.0f);
+  const float tr = fxx + fyy;
+  const float sxx = (fxx * fxx + fxy * fxy) * (1.0f / 24.0f);
+  const float sxy = 2.0f * fxy * tr * (1.0f / 24.0f);
+  const float syy = (fxy * fxy + fyy * fyy) * (1.0f / 24.0f);
+  const float amp12 = amp * (1.0f / 12.0f);
   out[kXc] = g.xs + psf.xcen;
   out[kYc] = g.ys + psf.ycen;
-  out[kHx] = cutoff_halfwidth(Sxx);
-  out[kHy] = cutoff_halfwidth(Syy);
-  out[kFxx] = fxx;
-  out[kFxy] = fxy;
-  out[kFyy] = fyy;
+  out[kCxx] = kExp2Scale * fxx;
+  out[kCxy] = kExp2Scale * (2.0f * fxy);
+  out[kCyy] = kExp2Scale * fyy;
   out[kH0] = h0;
+  out[kSxx] = sxx;
+  out[kSxy] = sxy;
+  out[kSyy] = syy;
   out[kAmp] = amp;
-  out[kAmp12] = amp * (1.0f / 12.0f);
+  out[kSrc] = __builtin_bit_cast(float, s);
   out[kDAF] = tmp * amp_s;
   out[kDAN] = tmp * flux * da_dn;
   out[kDAR] = tmp * flux * da_dr;
+  out[kEx1] = amp * fxx;
+  out[kEx2] = amp * fxy;
+  out[kEx3] = -2.0f * amp * sxx;
+  out[kEx4] = -amp * sxy;
+  out[kEy2] = amp * fyy;
+  out[kEy4] = -2.0f * amp * syy;
   out[kDAQ] = dAQ;
-  out[kKQ] = -0.5f * (qxx + qyy);
-  out[kQxx] = qxx;
-  out[kQyy] = qyy;
-  out[kQxy] = qxy;
+  out[kQ0] = amp12 * (-0.5f * (qxx + qyy));
+  out[kQ3] = amp12 * (qxx * fxx + qxy * fxy);
+  out[kQ4] = amp12 * (qyy * fyy + qxy * fxy);
+  out[kQ5] = amp12 * ((qxx + qyy) * fxy + qxy * tr);
+  out[kHQ3] = -0.5f * amp * qxx;
+  out[kHQ4] = -0.5f * amp * qyy;
+  out[kHQ5] = -amp * qxy;
   out[kDAP] = dAP;
-  out[kKP] = -0.5f * (pxx + pyy);
-  out[kPxx] = pxx;
-  out[kPyy] = pyy;
-  out[kPxy] = pxy;
-  out[kSrc] = __builtin_bit_cast(float, s);
+  out[kP0] = amp12 * (-0.5f * (pxx + pyy));
+  out[kP3] = amp12 * (pxx * fxx + pxy * fxy);
+  out[kP4] = amp12 * (pyy * fyy + pxy * fxy);
+  out[kP5] = amp12 * ((pxx + pyy) * fxy + pxy * tr);
+  out[kHP3] = -0.5f * amp * pxx;
+  out[kHP4] = -0.5f * amp * pyy;
+  out[kHP5] = -amp * pxy;
 }
 
 }  // namespace fpx

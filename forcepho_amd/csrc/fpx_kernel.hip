@@ -5,24 +5,26 @@
 // exposure (fpx_plan_tiles).  One workgroup of W wavefronts owns one tile at a
 // time: every wavefront holds the same 64*PPL pixels (PPL per lane) and the
 // W wavefronts split the exposure's ImageGaussian list between them, so the
-// parallelism is pixels x Gaussian-slices rather than pixels alone (the
+// parallelism is pixels x Gaussians rather than pixels alone (the
 // reference runs one block per band: compute_gaussians_kernel.cu:209,262).
 //
 // Per tile:
 //   prologue  one lane per (source, PSF component) tests whether the
-//             Gaussian can reach the tile, the survivors are compacted in
-//             order and their ImageGaussian records built into LDS
-//             (fpx_gauss.h); if they exceed the LDS budget the sources are
-//             processed in chunks and rebuilt.
-//   pass 1    model = sum_g amp_g G_g H_g over the wave's Gaussian slice
+//             Gaussian can reach the tile's pixel box and the survivors are
+//             compacted in order; their ImageGaussian records (fpx_gauss.h)
+//             are built into LDS, in chunks when they exceed the LDS budget.
+//   pass 1    model = sum_g amp_g G_g H_g over the wave's Gaussians (every
+//             W-th record, so that costly and cheap Gaussians spread evenly)
 //             (common_kernel.cc:57-83), partial models combined through LDS,
 //             residual r = d - model, chi2 += (r ierr - d ierr)(r ierr + d ierr)
 //             in FP64 (compute_gaussians_kernel.cc:286-293), weight
 //             w2 = -2 r ierr^2.
-//   pass 2    per source, 7 accumulators of w2 * d model / d theta
-//             (common_kernel.cc:95-143 with the per-pixel factor -2 r ierr^2
-//             folded into G and the ra/dec Jacobian CW applied once per source
-//             after the Gaussian loop), wave-reduced into per-wave LDS rows.
+//   pass 2    per Gaussian, 12 sums over the lane's pixels of w2 G and
+//             w2 G H times (1, dx, dy, dx^2, dy^2, dx dy), mapped by the
+//             record's coefficients onto 7 per-source accumulators of
+//             w2 * d model / d theta (common_kernel.cc:95-143; the ra/dec
+//             Jacobian CW is applied once per source after the Gaussian
+//             loop), wave-reduced into per-wave LDS rows.
 //   combine   per-wave rows summed in fixed order, one FP64 global atomic per
 //             value into the slot's per-band accumulator; the last tile of the
 //             slot (ticket counter) writes pchi2 / Response and re-zeroes the
@@ -62,8 +64,8 @@ static_assert(sizeof(FpxSlot) == 80, "slot layout");
 namespace fpx {
 
 constexpr int kAcc = 1 + MAXSOURCES * NPARAMS;        // doubles per band accumulator
-constexpr float kExp2Scale = -0.72134752044448170368f;  // -0.5 * log2(e)
-constexpr float kMaxExpArg = (float)MAX_EXP_ARG;
+// dx^T F dx > MAX_EXP_ARG  <=>  log2 G < kLog2Min (log2 G = kExp2Scale dx^T F dx)
+constexpr float kLog2Min = kExp2Scale * (float)MAX_EXP_ARG;
 
 // ------------------------------------------------------------------------
 // wavefront helpers
@@ -108,21 +110,23 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask) {
 
 // ------------------------------------------------------------------------
 // LDS carve-up of one workgroup
-//   records  gauss_cap x kRec floats   ImageGaussians of the current chunk,
-//                                       only those that reach the tile, in
-//                                       (source, component) order
+//   records  gauss_cap x kRec floats   ImageGaussians of the current chunk
+//                                       of the tile's surviving Gaussians
 //   part     W x 64*PPL floats         per-wave partial models
 //   cwl      MAXSOURCES x 4 floats     CW of every source for the exposure
 //   rows     W x kRowBytes             per-wave accumulators: chi2 (double)
 //                                       then MAXSOURCES*NPARAMS gradient sums
-//   index    gauss_cap + 1 ints        compaction scratch, then the count
+//   index    index_cap + 1 ints        cull flags, then the survivors'
+//                                       (source, component) indices in order
+//                                       and their count
 // ------------------------------------------------------------------------
 constexpr int kRowBytes = 8 + MAXSOURCES * NPARAMS * 4;
 
-__host__ __device__ constexpr size_t lds_bytes_for(int waves, int ppl, int gauss_cap) {
+__host__ __device__ constexpr size_t lds_bytes_for(int waves, int ppl, int gauss_cap,
+                                                   int index_cap) {
   return (size_t)gauss_cap * kRec * sizeof(float) + (size_t)waves * ppl * 64 * sizeof(float) +
          (size_t)MAXSOURCES * 4 * sizeof(float) + (size_t)waves * kRowBytes +
-         (size_t)(gauss_cap + 1) * sizeof(int);
+         (size_t)(index_cap + 1) * sizeof(int);
 }
 
 struct Lds {
@@ -156,22 +160,21 @@ struct Box {
   float x0, x1, y0, y1;
 };
 
-// Build the records of the Gaussians of sources [s0, s0 + ns) that can reach
-// the box, compacted in (source, component) order; returns their count.
-// Phase 1 flags every Gaussian (one lane each), phase 2 compacts the flags
-// into an index list with one wavefront (ballot + prefix count), phase 3
-// builds the surviving records.
+// Cull the n_src x n_psf Gaussians of the exposure against the tile's box:
+// phase 1 flags every Gaussian (one lane each), phase 2 compacts the flags
+// into the list of survivors in (source, component) order with one
+// wavefront (ballot + prefix count).  Returns the number of survivors.
 template <int W>
-__device__ int build_chunk(const FpxPatch *__restrict__ P, const FpxSource *__restrict__ props,
-                           int e, int band, int s0, int ns, int n_psf, const Box &box,
-                           const Lds &l, int wave, int lane) {
-  const int n = ns * n_psf;
+__device__ int cull_tile(const FpxPatch *__restrict__ P, const FpxSource *__restrict__ props,
+                         int e, int n_src, int n_psf, const Box &box, const Lds &l, int wave,
+                         int lane) {
+  const int n = n_src * n_psf;
   int *index = l.index;
-  __syncthreads();  // previous users of the records and index are done
+  __syncthreads();  // previous users of the index are done
   for (int lg = threadIdx.x; lg < n; lg += W * 64) {
-    const int sl = lg / n_psf;
-    index[lg] = gauss_reaches(P, props, e, s0 + sl, lg - sl * n_psf, box.x0, box.x1,
-                              box.y0, box.y1) ? 1 : 0;
+    const int s = lg / n_psf;
+    index[lg] = gauss_reaches(P, props, e, s, lg - s * n_psf, box.x0, box.x1, box.y0, box.y1)
+                    ? 1 : 0;
   }
   __syncthreads();
   if (wave == 0) {
@@ -184,163 +187,190 @@ __device__ int build_chunk(const FpxPatch *__restrict__ P, const FpxSource *__re
       if (keep) index[count + lanes_below(m)] = lg;
       count += __builtin_popcountll(m);
     }
-    if (lane == 0) index[n > 0 ? n : 0] = count;
+    if (lane == 0) index[n] = count;
   }
   __syncthreads();
-  const int n_act = __builtin_amdgcn_readfirstlane(index[n]);
-  for (int i = threadIdx.x; i < n_act; i += W * 64) {
-    const int lg = index[i];
-    const int sl = lg / n_psf;
-    build_record(P, props, e, band, s0 + sl, lg - sl * n_psf, l.recs + i * kRec);
-  }
-  __syncthreads();
-  return n_act;
+  return __builtin_amdgcn_readfirstlane(l.index[n]);
 }
 
-// Pass 1: add this wave's slice [gb, ge) of the records to the lane's model.
+// Build the records of survivors [first, first + count) into LDS.
+template <int W>
+__device__ void build_records(const FpxPatch *__restrict__ P,
+                              const FpxSource *__restrict__ props, int e, int band, int n_psf,
+                              int first, int count, const Lds &l) {
+  __syncthreads();  // previous users of the records are done
+  for (int i = threadIdx.x; i < count; i += W * 64) {
+    const int lg = l.index[first + i];
+    const int s = lg / n_psf;
+    build_record(P, props, e, band, s, lg - s * n_psf, l.recs + i * kRec);
+  }
+  __syncthreads();
+}
+
+// The pixel-side quantities of one Gaussian at a lane's PPL pixels: offsets,
+// their products, log2 of the Gaussian and the cutoff flag.  Returns a bit
+// per pixel slot i (a 64-pixel block of the tile) that is set when some lane
+// of the wavefront has pixel i inside the cutoff.
 template <int PPL>
-__device__ __forceinline__ void model_pass(const float *__restrict__ recs, int gb, int ge,
-                                           const float (&xp)[PPL], const float (&yp)[PPL],
-                                           float (&m)[PPL]) {
+__device__ __forceinline__ unsigned gauss_at_pixels(const float4 r0, const float cyy,
+                                                const float (&xp)[PPL], const float (&yp)[PPL],
+                                                float (&dx)[PPL], float (&dy)[PPL],
+                                                float (&dxx)[PPL], float (&dyy)[PPL],
+                                                float (&dxy)[PPL], float (&lg)[PPL],
+                                                bool (&on)[PPL]) {
+  unsigned blocks = 0;
+#pragma unroll
+  for (int i = 0; i < PPL; ++i) {
+    dx[i] = xp[i] - r0.x;
+    dy[i] = yp[i] - r0.y;
+    dxx[i] = dx[i] * dx[i];
+    dyy[i] = dy[i] * dy[i];
+    dxy[i] = dx[i] * dy[i];
+    lg[i] = fmaf(r0.z, dxx[i], fmaf(r0.w, dxy[i], cyy * dyy[i]));
+    on[i] = !(lg[i] < kLog2Min);
+    blocks |= wave_any(on[i]) ? (1u << i) : 0u;
+  }
+  return blocks;
+}
+
+// Pass 1: add records g0, g0 + step, ... < n (this wave's share) to the
+// lane's model.
+template <int PPL>
+__device__ __forceinline__ void model_pass(const float *__restrict__ recs, int g0, int step,
+                                           int n, const float (&xp)[PPL],
+                                           const float (&yp)[PPL], float (&m)[PPL]) {
   const float4 *R4 = reinterpret_cast<const float4 *>(recs);
-  for (int g = gb; g < ge; ++g) {
-    const float4 r0 = R4[g * 6 + 0];
-    const float4 r1 = R4[g * 6 + 1];
-    const float amp = recs[g * kRec + kAmp];
-    const float xc = r0.x, yc = r0.y, fxx = r1.x, fxy = r1.y;
-    const float fyy = r1.z, h0 = r1.w;
-    float vx[PPL], vy[PPL], ea[PPL];
+  for (int g = g0; g < n; g += step) {
+    const float4 r0 = R4[g * (kRec / 4) + 0];  // xc yc cxx cxy
+    const float4 r1 = R4[g * (kRec / 4) + 1];  // cyy h0 sxx sxy
+    const float4 r2 = R4[g * (kRec / 4) + 2];  // syy amp src dA/dflux
+    float dx[PPL], dy[PPL], dxx[PPL], dyy[PPL], dxy[PPL], lg[PPL];
     bool on[PPL];
-    bool any = false;
+    const unsigned blocks = gauss_at_pixels<PPL>(r0, r1.x, xp, yp, dx, dy, dxx, dyy, dxy, lg, on);
+    if (blocks == 0) continue;
 #pragma unroll
     for (int i = 0; i < PPL; ++i) {
-      const float dx = xp[i] - xc;
-      const float dy = yp[i] - yc;
-      vx[i] = fxx * dx + fxy * dy;
-      vy[i] = fxy * dx + fyy * dy;
-      ea[i] = dx * vx[i] + dy * vy[i];
-      on[i] = !(ea[i] > kMaxExpArg);
-      any |= on[i];
-    }
-    if (!wave_any(any)) continue;
-#pragma unroll
-    for (int i = 0; i < PPL; ++i) {
-      const float G = on[i] ? __builtin_amdgcn_exp2f(kExp2Scale * ea[i]) : 0.0f;
-      const float s2 = vx[i] * vx[i] + vy[i] * vy[i];
-      m[i] = fmaf(G * amp, fmaf(s2, 1.0f / 24.0f, h0), m[i]);
+      if (!(blocks & (1u << i))) continue;  // no lane has pixel i inside
+      const float G = on[i] ? __builtin_amdgcn_exp2f(lg[i]) : 0.0f;
+      const float H = fmaf(r1.z, dxx[i], fmaf(r1.w, dxy[i], fmaf(r2.x, dyy[i], r1.y)));
+      m[i] = fmaf(G * r2.y, H, m[i]);
     }
   }
 }
 
-// Reduce one source's 7 accumulators over the wavefront and add them (with
-// the ra/dec Jacobian CW applied) to the wave's row.
+// Sum 8 per-lane values over the wavefront with halving exchanges (10
+// shuffles instead of 8 x 6): the xor-32, -16 and -8 steps each keep half of
+// the remaining values and add the partner's copy of that half, so that lane
+// l then holds value k(l) = 4 bit5(l) + 2 bit4(l) + bit3(l) summed over the 8
+// lanes that differ from l in bits 3-5; the xor-4, -2 and -1 steps finish the
+// sum.  Returns the wavefront total of value k(lane).
+__device__ __forceinline__ float wave_sum8(const float (&v)[8], int lane) {
+  const bool b5 = (lane & 32) != 0, b4 = (lane & 16) != 0, b3 = (lane & 8) != 0;
+  float w4[4], w2[2];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float send = b5 ? v[j] : v[4 + j];
+    const float keep = b5 ? v[4 + j] : v[j];
+    w4[j] = keep + __shfl_xor(send, 32, 64);
+  }
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const float send = b4 ? w4[j] : w4[2 + j];
+    const float keep = b4 ? w4[2 + j] : w4[j];
+    w2[j] = keep + __shfl_xor(send, 16, 64);
+  }
+  const float send = b3 ? w2[0] : w2[1];
+  const float keep = b3 ? w2[1] : w2[0];
+  float t = keep + __shfl_xor(send, 8, 64);
+  t += __shfl_xor(t, 4, 64);
+  t += __shfl_xor(t, 2, 64);
+  t += __shfl_xor(t, 1, 64);
+  return t;
+}
+
+// Reduce one source's accumulators over the wavefront and add them to the
+// wave's row: flux, ra, dec (the pixel-space x/y sums mapped through the
+// ra/dec Jacobian CW of the source), q, pa, sersic, rh.
 __device__ __forceinline__ void flush_source(int s, float aF, float aX, float aY, float aQ,
                                              float aP, float aN, float aR,
                                              const float *__restrict__ cwl,
                                              unsigned char *__restrict__ wrow, int lane) {
-  aF = wave_sum(aF);
-  aX = wave_sum(aX);
-  aY = wave_sum(aY);
-  aQ = wave_sum(aQ);
-  aP = wave_sum(aP);
-  aN = wave_sum(aN);
-  aR = wave_sum(aR);
-  if (lane == 0) {
-    const float *cw = cwl + 4 * s;  // [v11 v12 v21 v22] = dpix/dsky
-    float *row = row_grad(wrow) + NPARAMS * s;
-    row[0] += aF;
-    row[1] += cw[0] * aX + cw[2] * aY;
-    row[2] += cw[1] * aX + cw[3] * aY;
-    row[3] += aQ;
-    row[4] += aP;
-    row[5] += aN;
-    row[6] += aR;
-  }
+  const float *cw = cwl + 4 * s;  // [v11 v12 v21 v22] = dpix/dsky
+  const float v[8] = {aF, cw[0] * aX + cw[2] * aY, cw[1] * aX + cw[3] * aY, aQ,
+                      aP, aN, aR, 0.0f};
+  const float t = wave_sum8(v, lane);
+  const int k = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+  if ((lane & 7) == 0 && k < NPARAMS) row_grad(wrow)[NPARAMS * s + k] += t;
 }
 
-// Pass 2: gradient accumulation over the wave's slice [gb, ge) of the
-// records; a source's sums are flushed when the next record belongs to
-// another source.
+// Pass 2: gradient accumulation over records g0, g0 + step, ... < n (this
+// wave's share).  Per Gaussian the lane sums gw m_k and gw H m_k over its pixels
+// (m = 1, dx, dy, dx^2, dy^2, dx dy) and maps the 12 sums to the source's 7
+// accumulators with the record's coefficients; a source's sums are flushed
+// when the next record belongs to another source.
 template <int PPL>
-__device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int gb, int ge,
-                                          const float (&xp)[PPL], const float (&yp)[PPL],
+__device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int g0, int step,
+                                          int n, const float (&xp)[PPL], const float (&yp)[PPL],
                                           const float (&w2)[PPL], const float *__restrict__ cwl,
                                           unsigned char *__restrict__ wrow, int lane) {
   const float4 *R4 = reinterpret_cast<const float4 *>(recs);
   float aF = 0.f, aX = 0.f, aY = 0.f, aQ = 0.f, aP = 0.f, aN = 0.f, aR = 0.f;
   bool touched = false;
   int cur = -1;
-  for (int g = gb; g < ge; ++g) {
-    const float4 r0 = R4[g * 6 + 0];
-    const float4 r1 = R4[g * 6 + 1];
-    const float4 r2 = R4[g * 6 + 2];
-    const float4 r3 = R4[g * 6 + 3];
-    const float4 r4 = R4[g * 6 + 4];
-    const float4 r5 = R4[g * 6 + 5];
-    const int src = __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, r5.w));
+  for (int g = g0; g < n; g += step) {
+    const float4 *rg = R4 + g * (kRec / 4);
+    const float4 r0 = rg[0];  // xc yc cxx cxy
+    const float4 r1 = rg[1];  // cyy h0 sxx sxy
+    const float4 r2 = rg[2];  // syy amp src dA/dflux
+    const int src = __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, r2.z));
     if (src != cur) {
       if (touched) flush_source(cur, aF, aX, aY, aQ, aP, aN, aR, cwl, wrow, lane);
       aF = aX = aY = aQ = aP = aN = aR = 0.f;
       touched = false;
       cur = src;
     }
-    const float xc = r0.x, yc = r0.y, fxx = r1.x, fxy = r1.y;
-    const float fyy = r1.z, h0 = r1.w;
-    float dx[PPL], dy[PPL], vx[PPL], vy[PPL], px[PPL], py[PPL], ea[PPL];
+    float dx[PPL], dy[PPL], dxx[PPL], dyy[PPL], dxy[PPL], lg[PPL];
     bool on[PPL];
-    bool any = false;
-#pragma unroll
-    for (int i = 0; i < PPL; ++i) {
-      dx[i] = xp[i] - xc;
-      dy[i] = yp[i] - yc;
-      vx[i] = fxx * dx[i] + fxy * dy[i];
-      vy[i] = fxy * dx[i] + fyy * dy[i];
-      px[i] = dx[i] * vx[i];
-      py[i] = dy[i] * vy[i];
-      ea[i] = px[i] + py[i];
-      on[i] = !(ea[i] > kMaxExpArg);
-      any |= on[i];
-    }
-    if (!wave_any(any)) continue;
+    const unsigned blocks = gauss_at_pixels<PPL>(r0, r1.x, xp, yp, dx, dy, dxx, dyy, dxy, lg, on);
+    if (blocks == 0) continue;
     touched = true;
-    const float amp = r2.x, amp12 = r2.y, dAF = r2.z, dAN = r2.w;
-    const float dAR = r3.x, dAQ = r3.y, kq = r3.z, qxx = r3.w;
-    const float qyy = r4.x, qxy = r4.y, dAP = r4.z, kp = r4.w;
-    const float pxx = r5.x, pyy = r5.y, pxy = r5.z;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, a4 = 0.f, a5 = 0.f;
+    float b0 = 0.f, b1 = 0.f, b2 = 0.f, b3 = 0.f, b4 = 0.f, b5 = 0.f;
 #pragma unroll
     for (int i = 0; i < PPL; ++i) {
-      const float G = on[i] ? __builtin_amdgcn_exp2f(kExp2Scale * ea[i]) : 0.0f;
+      if (!(blocks & (1u << i))) continue;  // no lane has pixel i inside
+      const float G = on[i] ? __builtin_amdgcn_exp2f(lg[i]) : 0.0f;
       const float gw = G * w2[i];
-      const float s2 = vx[i] * vx[i] + vy[i] * vy[i];
-      const float H = fmaf(s2, 1.0f / 24.0f, h0);
-      const float gH = gw * H;
-      aF = fmaf(dAF, gH, aF);
-      aN = fmaf(dAN, gH, aN);
-      aR = fmaf(dAR, gH, aR);
-      const float C = amp * gH;
-      const float c12 = amp12 * gw;
-      const float ux = fxx * vx[i] + fxy * vy[i];
-      const float uy = fxy * vx[i] + fyy * vy[i];
-      aX = fmaf(C, vx[i], aX);
-      aX = fmaf(-c12, ux, aX);
-      aY = fmaf(C, vy[i], aY);
-      aY = fmaf(-c12, uy, aY);
-      const float dxx = dx[i] * dx[i];
-      const float dyy = dy[i] * dy[i];
-      const float dxy2 = (dx[i] + dx[i]) * dy[i];
-      const float cr = dy[i] * vx[i] + dx[i] * vy[i];
-      const float hC = -0.5f * C;
-      const float t1q = kq + qxx * px[i] + qyy * py[i] + qxy * cr;
-      const float t2q = qxx * dxx + qyy * dyy + qxy * dxy2;
-      aQ = fmaf(dAQ, gH, aQ);
-      aQ = fmaf(c12, t1q, aQ);
-      aQ = fmaf(hC, t2q, aQ);
-      const float t1p = kp + pxx * px[i] + pyy * py[i] + pxy * cr;
-      const float t2p = pxx * dxx + pyy * dyy + pxy * dxy2;
-      aP = fmaf(dAP, gH, aP);
-      aP = fmaf(c12, t1p, aP);
-      aP = fmaf(hC, t2p, aP);
+      const float H = fmaf(r1.z, dxx[i], fmaf(r1.w, dxy[i], fmaf(r2.x, dyy[i], r1.y)));
+      const float gh = gw * H;
+      a0 += gw;
+      a1 = fmaf(gw, dx[i], a1);
+      a2 = fmaf(gw, dy[i], a2);
+      a3 = fmaf(gw, dxx[i], a3);
+      a4 = fmaf(gw, dyy[i], a4);
+      a5 = fmaf(gw, dxy[i], a5);
+      b0 += gh;
+      b1 = fmaf(gh, dx[i], b1);
+      b2 = fmaf(gh, dy[i], b2);
+      b3 = fmaf(gh, dxx[i], b3);
+      b4 = fmaf(gh, dyy[i], b4);
+      b5 = fmaf(gh, dxy[i], b5);
     }
+    const float4 r3 = rg[3];  // dA/dsersic dA/drh ex1 ex2
+    const float4 r4 = rg[4];  // ex3 ex4 ey2 ey4
+    const float4 r5 = rg[5];  // dA/dq kq0 kq3 kq4
+    const float4 r6 = rg[6];  // kq5 hq3 hq4 hq5
+    const float4 r7 = rg[7];  // dA/dpa kp0 kp3 kp4
+    const float4 r8 = rg[8];  // kp5 hp3 hp4 hp5
+    aF = fmaf(r2.w, b0, aF);
+    aN = fmaf(r3.x, b0, aN);
+    aR = fmaf(r3.y, b0, aR);
+    aX = fmaf(r3.z, b1, fmaf(r3.w, b2, fmaf(r4.x, a1, fmaf(r4.y, a2, aX))));
+    aY = fmaf(r3.w, b1, fmaf(r4.z, b2, fmaf(r4.y, a1, fmaf(r4.w, a2, aY))));
+    aQ = fmaf(r5.x, b0, fmaf(r5.y, a0, fmaf(r5.z, a3, fmaf(r5.w, a4, aQ))));
+    aQ = fmaf(r6.x, a5, fmaf(r6.y, b3, fmaf(r6.z, b4, fmaf(r6.w, b5, aQ))));
+    aP = fmaf(r7.x, b0, fmaf(r7.y, a0, fmaf(r7.z, a3, fmaf(r7.w, a4, aP))));
+    aP = fmaf(r8.x, a5, fmaf(r8.y, b3, fmaf(r8.z, b4, fmaf(r8.w, b5, aP))));
   }
   if (touched) flush_source(cur, aF, aX, aY, aQ, aP, aN, aR, cwl, wrow, lane);
 }
@@ -349,6 +379,8 @@ template <int W, int PPL>
 __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPatch *P,
                           const FpxSource *props, int e, int band, int n_src, int n_psf,
                           int gauss_cap, unsigned char *smem) {
+  // The tile's Gaussians that reach it are processed in chunks of gauss_cap
+  // records (one chunk when they all fit).
   constexpr int kPix = 64 * PPL;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -356,8 +388,6 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
   // Gaussian loop bounds below in scalar registers
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const Lds l = carve(smem, W, PPL, gauss_cap);
-  const int spc = max(1, gauss_cap / n_psf);  // sources per LDS chunk
-  const int n_chunks = (n_src + spc - 1) / spc;
   const bool vjp = slot.pixel_weight != nullptr;
 
   for (int j = tid; j < W * kRowBytes / 4; j += W * 64) reinterpret_cast<float *>(l.rows)[j] = 0.0f;
@@ -396,16 +426,17 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
     box.y1 = uniform_f(wave_max(y1));
   }
 
-  int n_act = 0;
+  const int n_surv = cull_tile<W>(P, props, e, n_src, n_psf, box, l, wave, lane);
+  const int n_chunks = (n_surv + gauss_cap - 1) / gauss_cap;
+  auto chunk_size = [&](int c) { return min(gauss_cap, n_surv - c * gauss_cap); };
   if (!vjp) {
     float m[PPL];
 #pragma unroll
     for (int i = 0; i < PPL; ++i) m[i] = 0.0f;
     for (int c = 0; c < n_chunks; ++c) {
-      const int s0 = c * spc;
-      n_act = build_chunk<W>(P, props, e, band, s0, min(spc, n_src - s0), n_psf, box, l, wave,
-                             lane);
-      model_pass<PPL>(l.recs, (n_act * wave) / W, (n_act * (wave + 1)) / W, xp, yp, m);
+      const int n_act = chunk_size(c);
+      build_records<W>(P, props, e, band, n_psf, c * gauss_cap, n_act, l);
+      model_pass<PPL>(l.recs, wave, W, n_act, xp, yp, m);
     }
 #pragma unroll
     for (int i = 0; i < PPL; ++i) l.part[(wave * PPL + i) * 64 + lane] = m[i];
@@ -441,14 +472,13 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
   wave_has_weight = wave_any(wave_has_weight);
 
   unsigned char *wrow = l.rows + wave * kRowBytes;
-  for (int c = 0; c < n_chunks; ++c) {
-    const int s0 = c * spc;
-    if (n_chunks > 1 || vjp)
-      n_act = build_chunk<W>(P, props, e, band, s0, min(spc, n_src - s0), n_psf, box, l, wave,
-                             lane);
+  // last chunk first: after the model pass its records are still in LDS
+  for (int c = n_chunks - 1; c >= 0; --c) {
+    const int n_act = chunk_size(c);
+    if (c != n_chunks - 1 || vjp)
+      build_records<W>(P, props, e, band, n_psf, c * gauss_cap, n_act, l);
     if (wave_has_weight)
-      grad_pass<PPL>(l.recs, (n_act * wave) / W, (n_act * (wave + 1)) / W, xp, yp, w2, l.cwl,
-                     wrow, lane);
+      grad_pass<PPL>(l.recs, wave, W, n_act, xp, yp, w2, l.cwl, wrow, lane);
   }
 }
 
@@ -500,7 +530,7 @@ __device__ void finish_tile(const FpxSlot &slot, int band, int n_src, unsigned c
 }
 
 template <int W, int PPL>
-__device__ void process_tile(const FpxSlot &slot, int local, int gauss_cap,
+__device__ void process_tile(const FpxSlot &slot, int local, int gauss_cap, int index_cap,
                              unsigned char *smem, int *s_flag) {
   constexpr int kPix = 64 * PPL;
   // Scalars that every lane holds identically.  The loads below are vector
@@ -515,7 +545,7 @@ __device__ void process_tile(const FpxSlot &slot, int local, int gauss_cap,
   const int n_psf = __builtin_amdgcn_readfirstlane(P->n_psf_per_source[band]);
   // A tile whose sizes break the kernel's assumptions contributes nothing
   // (every lane sees the same values, so the whole workgroup agrees).
-  const bool ok = n_src >= 1 && n_src <= MAXSOURCES && n_psf >= 1 && n_psf <= gauss_cap &&
+  const bool ok = n_src >= 1 && n_src <= MAXSOURCES && n_psf >= 1 && n_src * n_psf <= index_cap &&
                   n_psf <= FPX_MAX_PSF_PER_SOURCE && band >= 0 && band < slot.n_bands &&
                   band < MAXBANDS && tile.pix_count >= 0 && tile.pix_count <= kPix;
   if (ok) tile_work<W, PPL>(slot, tile, P, props, e, band, n_src, n_psf, gauss_cap, smem);
@@ -526,7 +556,7 @@ __device__ void process_tile(const FpxSlot &slot, int local, int gauss_cap,
 template <int W, int PPL>
 __global__ __launch_bounds__(W * 64) void chi2_grad_kernel(
     const FpxSlot *__restrict__ slots, int n_slots, const int32_t *__restrict__ item_offsets,
-    int n_items, int gauss_cap, int persistent, uint32_t *__restrict__ queue) {
+    int n_items, int gauss_cap, int index_cap, int persistent, uint32_t *__restrict__ queue) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_item;
   __shared__ int s_flag;
@@ -554,7 +584,7 @@ __global__ __launch_bounds__(W * 64) void chi2_grad_kernel(
       local = item - item_offsets[si];
     }
     const FpxSlot slot = slots[si];
-    process_tile<W, PPL>(slot, local, gauss_cap, smem, &s_flag);
+    process_tile<W, PPL>(slot, local, gauss_cap, index_cap, smem, &s_flag);
   }
   if (persistent) {
     __syncthreads();
@@ -565,6 +595,73 @@ __global__ __launch_bounds__(W * 64) void chi2_grad_kernel(
         queue[1] = 0u;
       }
     }
+  }
+}
+
+// ------------------------------------------------------------------------
+// Census of the (pixel, ImageGaussian) pairs inside the cutoff, i.e. those
+// for which the reference's per-pixel loops go past their
+// `exparg > MAX_EXP_ARG` test (common_kernel.cc:67-73, 103-108), with the
+// reference's own exponent expression, over the packed pixels (padding
+// included, as the reference's loops run over it).  One workgroup per tile;
+// every Gaussian of the exposure is tested at every pixel (no culling).  Adds the
+// count of each slot to pairs_in[slot] and, if item_pairs is given, each
+// tile's count to item_pairs[item].  A measurement of the workload, used to
+// report the roofline and to balance tiles over compute units; not part of
+// the evaluation.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void census_kernel(
+    const FpxSlot *__restrict__ slots, int n_slots, const int32_t *__restrict__ item_offsets,
+    int n_items, int index_cap, unsigned long long *__restrict__ pairs_in,
+    unsigned *__restrict__ item_pairs) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float *shape = reinterpret_cast<float *>(smem);  // index_cap x (xc yc fxx fxy fyy)
+  const int item = blockIdx.x;
+  int si = 0, local = item;
+  if (n_slots > 1) {
+    int lo = 0, hi = n_slots - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (item_offsets[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    si = lo;
+    local = item - item_offsets[si];
+  }
+  const FpxSlot slot = slots[si];
+  const FpxTile tile = slot.tiles[local];
+  const FpxPatch *P = slot.patch;
+  const int e = __builtin_amdgcn_readfirstlane(tile.exposure);
+  const int band = __builtin_amdgcn_readfirstlane(tile.band);
+  const int n_src = __builtin_amdgcn_readfirstlane(P->n_sources);
+  if (band < 0 || band >= slot.n_bands || band >= MAXBANDS) return;
+  const int n_psf = __builtin_amdgcn_readfirstlane(P->n_psf_per_source[band]);
+  if (n_src < 1 || n_src > MAXSOURCES || n_psf < 1 || n_src * n_psf > index_cap) return;
+  const int n = n_src * n_psf;
+  for (int g = threadIdx.x; g < n; g += blockDim.x) {
+    const int s = g / n_psf;
+    float *o = shape + 5 * g;
+    image_gaussian_shape(P, slot.proposal, e, s, g - s * n_psf, o[0], o[1], o[2], o[3], o[4]);
+  }
+  __syncthreads();
+  unsigned count = 0;
+  for (int k = threadIdx.x; k < tile.pix_count; k += blockDim.x) {
+    const float xp = P->xpix[tile.pix_start + k];
+    const float yp = P->ypix[tile.pix_start + k];
+    for (int g = 0; g < n; ++g) {
+      const float *o = shape + 5 * g;
+      const float dx = xp - o[0];
+      const float dy = yp - o[1];
+      const float vx = o[2] * dx + o[3] * dy;
+      const float vy = o[4] * dy + o[3] * dx;
+      const float exparg = dx * vx + dy * vy;
+      count += (exparg > (float)MAX_EXP_ARG) ? 0u : 1u;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) count += __shfl_xor(count, o, 64);
+  if ((threadIdx.x & 63) == 0 && count != 0) {
+    atomicAdd(pairs_in + si, (unsigned long long)count);
+    if (item_pairs) atomicAdd(item_pairs + item, count);
   }
 }
 
@@ -623,6 +720,14 @@ int hip_err(hipError_t e, const char *where) {
     if (_e != hipSuccess) return hip_err(_e, #call);   \
   } while (0)
 
+// LDS budget of a workgroup: 10 KiB per wavefront, so that four wavefronts
+// per SIMD (16 per CU) fit in the 160 KiB of a CU; a tile whose surviving
+// Gaussians exceed the records that fit is processed in chunks.
+constexpr int kLdsPerWave = 10 * 1024;
+// static LDS of the chi2 kernel (the queue item and the ticket flag), for the
+// host-only size query; launches use the kernel's reported size
+constexpr int kStaticLds = 16;
+
 struct KernelCfg {
   int waves;
   int ppl;
@@ -631,8 +736,8 @@ struct KernelCfg {
 
 #define FPX_CFG(W, P) {W, P, reinterpret_cast<const void *>(&fpx::chi2_grad_kernel<W, P>)}
 const KernelCfg kConfigs[] = {
-    FPX_CFG(4, 1), FPX_CFG(4, 2), FPX_CFG(4, 4), FPX_CFG(8, 1),
-    FPX_CFG(8, 2), FPX_CFG(8, 4), FPX_CFG(2, 4), FPX_CFG(16, 1),
+    FPX_CFG(4, 1), FPX_CFG(4, 2), FPX_CFG(4, 4), FPX_CFG(8, 8),
+    FPX_CFG(16, 4), FPX_CFG(8, 4), FPX_CFG(2, 4), FPX_CFG(4, 8),
 };
 #undef FPX_CFG
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
@@ -648,6 +753,11 @@ struct DeviceState {
   // a launch recorded into a graph issues no occupancy query
   size_t occ_lds[kNumConfigs] = {};
   int occ_per_cu[kNumConfigs] = {};
+  size_t census_lds = 0;
+  int static_lds[kNumConfigs];  // static LDS of each kernel, -1 until queried
+  DeviceState() {
+    for (int &v : static_lds) v = -1;
+  }
 };
 
 std::mutex g_mu;
@@ -682,12 +792,60 @@ int check_cfg(int32_t cfg) {
   return FPX_OK;
 }
 
-// Largest Gaussian count per LDS chunk: all of them if they fit.
-int pick_gauss_cap(const KernelCfg &kc, int max_gauss, int lds_max) {
-  const size_t fixed = fpx::lds_bytes_for(kc.waves, kc.ppl, 0);
-  const size_t per = fpx::lds_bytes_for(kc.waves, kc.ppl, 1) - fixed;  // record + index slot
-  const long fit = ((long)lds_max - (long)fixed) / (long)per;
-  return (int)std::min<long>(max_gauss, fit);
+// Records per LDS chunk: all max_gauss if they fit the workgroup's budget
+// (kLdsPerWave per wavefront, less the kernel's static LDS), else what fits,
+// but at least 64 when the device's per-workgroup limit allows it.  0 if not
+// even one fits.
+int pick_gauss_cap(const KernelCfg &kc, int max_gauss, int lds_max, int static_lds) {
+  const long fixed = (long)fpx::lds_bytes_for(kc.waves, kc.ppl, 0, max_gauss);
+  const long per = (long)(fpx::kRec * sizeof(float));
+  lds_max -= static_lds;
+  const long budget = std::min<long>(lds_max, (long)kc.waves * kLdsPerWave - static_lds);
+  long fit = (budget - fixed) / per;
+  if (fit < std::min<long>(max_gauss, 64)) fit = ((long)lds_max - fixed) / per;
+  return (int)std::max<long>(0, std::min<long>(max_gauss, fit));
+}
+
+// Workgroups of configuration cfg with `lds` bytes of dynamic LDS that fit
+// one CU at once (cached, so that a launch recorded into a graph makes no
+// occupancy query).
+int groups_per_cu(DeviceState *st, int cfg, size_t lds, int *per_cu) {
+  {
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (st->occ_lds[cfg] == lds && st->occ_per_cu[cfg] > 0) {
+      *per_cu = st->occ_per_cu[cfg];
+      return FPX_OK;
+    }
+  }
+  int n = 0;
+  FPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kConfigs[cfg].fn,
+                                                       kConfigs[cfg].waves * 64, lds));
+  if (n < 1) n = 1;
+  std::lock_guard<std::mutex> lock(g_mu);
+  st->occ_lds[cfg] = lds;
+  st->occ_per_cu[cfg] = n;
+  *per_cu = n;
+  return FPX_OK;
+}
+
+// Dynamic LDS bytes and records per chunk of a launch of configuration cfg.
+int launch_lds(DeviceState *st, int cfg, int max_gauss, size_t *lds, int *cap, int *index_cap) {
+  const KernelCfg &kc = kConfigs[cfg];
+  int static_lds = 0;
+  {
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (st->static_lds[cfg] < 0) {
+      hipFuncAttributes attr;
+      FPX_HIP(hipFuncGetAttributes(&attr, kc.fn));
+      st->static_lds[cfg] = (int)attr.sharedSizeBytes;
+    }
+    static_lds = st->static_lds[cfg];
+  }
+  *cap = pick_gauss_cap(kc, max_gauss, st->lds_max, static_lds);
+  if (*cap < 1) return set_err(FPX_E_ARG, "max_gauss too large for the LDS of one workgroup");
+  *index_cap = max_gauss;
+  *lds = fpx::lds_bytes_for(kc.waves, kc.ppl, *cap, *index_cap);
+  return FPX_OK;
 }
 
 int launch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
@@ -705,10 +863,10 @@ int launch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
   rc = current_device_state(&st);
   if (rc) return rc;
   const KernelCfg &kc = kConfigs[cfg];
-  const int cap = pick_gauss_cap(kc, max_gauss, st->lds_max);
-  if (cap < FPX_MAX_PSF_PER_SOURCE && cap < max_gauss)
-    return set_err(FPX_E_ARG, "LDS too small for one source's Gaussians");
-  const size_t lds = fpx::lds_bytes_for(kc.waves, kc.ppl, cap);
+  int cap = 0, index_cap = 0;
+  size_t lds = 0;
+  rc = launch_lds(st, cfg, max_gauss, &lds, &cap, &index_cap);
+  if (rc) return rc;
   {
     std::lock_guard<std::mutex> lock(g_mu);
     if (st->lds_set[cfg] < lds) {
@@ -720,21 +878,13 @@ int launch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
   int grid = n_items;
   if (persistent) {
     int per_cu = 0;
-    {
-      std::lock_guard<std::mutex> lock(g_mu);
-      if (st->occ_lds[cfg] == lds) per_cu = st->occ_per_cu[cfg];
-    }
-    if (per_cu == 0) {
-      FPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kc.fn, threads, lds));
-      if (per_cu < 1) per_cu = 1;
-      std::lock_guard<std::mutex> lock(g_mu);
-      st->occ_lds[cfg] = lds;
-      st->occ_per_cu[cfg] = per_cu;
-    }
+    rc = groups_per_cu(st, cfg, lds, &per_cu);
+    if (rc) return rc;
     grid = std::min<long>((long)per_cu * st->n_cu, (long)n_items);
   }
-  void *args[] = {(void *)&slots,   (void *)&n_slots, (void *)&item_offsets, (void *)&n_items,
-                  (void *)&cap,     (void *)&persistent, (void *)&queue};
+  void *args[] = {(void *)&slots,     (void *)&n_slots,    (void *)&item_offsets,
+                  (void *)&n_items,   (void *)&cap,        (void *)&index_cap,
+                  (void *)&persistent, (void *)&queue};
   FPX_HIP(hipLaunchKernel(kc.fn, dim3(grid), dim3(threads), args, lds, (hipStream_t)stream));
   return FPX_OK;
 }
@@ -811,7 +961,28 @@ int fpx_lds_bytes(int32_t cfg, int32_t max_gauss, size_t *bytes) {
   int rc = check_cfg(cfg);
   if (rc) return rc;
   if (!bytes || max_gauss < 0) return set_err(FPX_E_ARG, "bad argument");
-  *bytes = fpx::lds_bytes_for(kConfigs[cfg].waves, kConfigs[cfg].ppl, max_gauss);
+  const KernelCfg &kc = kConfigs[cfg];
+  const int cap = pick_gauss_cap(kc, max_gauss, 160 * 1024, kStaticLds);
+  *bytes = fpx::lds_bytes_for(kc.waves, kc.ppl, cap, max_gauss);
+  return FPX_OK;
+}
+
+int fpx_resident_groups(int32_t cfg, int32_t max_gauss, int32_t *groups) {
+  int rc = check_cfg(cfg);
+  if (rc) return rc;
+  if (!groups) return set_err(FPX_E_ARG, "null output");
+  if (max_gauss <= 0 || max_gauss > MAXSOURCES * FPX_MAX_PSF_PER_SOURCE)
+    return set_err(FPX_E_ARG, "max_gauss out of range [1, MAXSOURCES*FPX_MAX_PSF_PER_SOURCE]");
+  DeviceState *st = nullptr;
+  rc = current_device_state(&st);
+  if (rc) return rc;
+  int cap = 0, index_cap = 0, per_cu = 0;
+  size_t lds = 0;
+  rc = launch_lds(st, cfg, max_gauss, &lds, &cap, &index_cap);
+  if (rc) return rc;
+  rc = groups_per_cu(st, cfg, lds, &per_cu);
+  if (rc) return rc;
+  *groups = per_cu * st->n_cu;
   return FPX_OK;
 }
 
@@ -844,6 +1015,36 @@ int fpx_evaluate_batch(const FpxSlot *slots, int32_t n_slots, const int32_t *ite
                        uint32_t *queue, void *stream) {
   return launch(slots, n_slots, item_offsets, n_items, max_gauss, cfg, persistent ? 1 : 0,
                 queue, stream);
+}
+
+int fpx_count_pairs(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
+                    int32_t n_items, int32_t max_gauss, unsigned long long *pairs_in,
+                    uint32_t *item_pairs, void *stream) {
+  if (!slots || n_slots <= 0 || n_items < 0 || !pairs_in)
+    return set_err(FPX_E_ARG, "empty or invalid slot list");
+  if (n_slots > 1 && !item_offsets) return set_err(FPX_E_ARG, "item_offsets required for n_slots > 1");
+  if (max_gauss <= 0 || max_gauss > MAXSOURCES * FPX_MAX_PSF_PER_SOURCE)
+    return set_err(FPX_E_ARG, "max_gauss out of range [1, MAXSOURCES*FPX_MAX_PSF_PER_SOURCE]");
+  if (n_items == 0) return FPX_OK;
+  DeviceState *st = nullptr;
+  int rc = current_device_state(&st);
+  if (rc) return rc;
+  const size_t lds = (size_t)max_gauss * 5 * sizeof(float);
+  if ((long)lds > st->lds_max) return set_err(FPX_E_ARG, "max_gauss too large for the LDS");
+  const void *fn = reinterpret_cast<const void *>(&fpx::census_kernel);
+  {
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (st->census_lds < lds) {
+      FPX_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      st->census_lds = lds;
+    }
+  }
+  int index_cap = max_gauss;
+  void *args[] = {(void *)&slots,     (void *)&n_slots,  (void *)&item_offsets,
+                  (void *)&n_items,   (void *)&index_cap, (void *)&pairs_in,
+                  (void *)&item_pairs};
+  FPX_HIP(hipLaunchKernel(fn, dim3(n_items), dim3(256), args, lds, (hipStream_t)stream));
+  return FPX_OK;
 }
 
 int fpx_probe_valu(int32_t kind, int32_t iters, float *out, int32_t n_threads, double *ops,

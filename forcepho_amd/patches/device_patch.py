@@ -144,6 +144,7 @@ class HIPPatchMixin(DevicePatchMixin):
             self.device_ptrs["residual"] = torch.zeros(self.xpix.nbytes, dtype=torch.uint8,
                                                        device=dev)
         self._tiles = {}
+        self._tiles_host = {}
         self.send_patchstruct_to_device()
         self._dirty_data = False
         return self.device_patch
@@ -226,6 +227,23 @@ class HIPPatchMixin(DevicePatchMixin):
                                    tiles.ctypes.data, n), "fpx_plan_tiles")
             tiles[:n] = tiles[:n][self.tile_order(tiles[:n])]
             self._tiles[pix_per_tile] = (device_bytes(tiles, self._dev()), n)
+            self._tiles_host[pix_per_tile] = tiles
+        return self._tiles[pix_per_tile]
+
+    def reorder_tiles(self, pix_per_tile, perm):
+        """Replace the cached tile table for pix_per_tile by its permutation
+        ``perm`` (new position k holds old tile perm[k]); returns (tensor,
+        n_tiles)."""
+        self.device_tiles(pix_per_tile)
+        tiles = self._tiles_host[pix_per_tile]
+        n = self._tiles[pix_per_tile][1]
+        perm = np.asarray(perm, dtype=np.int64)
+        if sorted(perm.tolist()) != list(range(n)):
+            raise ValueError("perm is not a permutation of the tile table")
+        tiles = tiles.copy()
+        tiles[:n] = tiles[:n][perm]
+        self._tiles[pix_per_tile] = (device_bytes(tiles, self._dev()), n)
+        self._tiles_host[pix_per_tile] = tiles
         return self._tiles[pix_per_tile]
 
     def max_gauss(self):
@@ -235,3 +253,4 @@ class HIPPatchMixin(DevicePatchMixin):
         self.device_ptrs = {}
         self.device_patch = None
         self._tiles = {}
+        self._tiles_host = {}

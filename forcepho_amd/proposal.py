@@ -81,7 +81,7 @@ class HIPProposer(ProposerBase):
         self.set_config(DEFAULT_CONFIG if config is None else config)
         self.device = torch.device("cuda", torch.cuda.current_device())
         dev = self.device
-        n_acc = L.fpx_accum_doubles(MAXBANDS)
+        n_acc = lib().fpx_accum_doubles(MAXBANDS)
         self._accum = torch.zeros(n_acc, dtype=torch.float64, device=dev)
         self._ticket = torch.zeros(2, dtype=torch.int32, device=dev)
         self._chi2 = torch.zeros(MAXBANDS, dtype=torch.float64, device=dev)
@@ -89,6 +89,7 @@ class HIPProposer(ProposerBase):
         self._prop = torch.zeros(MAXSOURCES * source_struct_dtype.itemsize, dtype=torch.uint8,
                                  device=dev)
         self._slot = torch.zeros(SLOT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+        self._slots = {}        # device FpxSlot per (patch struct, tiles, weights)
         self._queue = torch.zeros(2, dtype=torch.int32, device=dev)
         pin = dict(pin_memory=True)
         self._prop_h = torch.zeros(self._prop.numel(), dtype=torch.uint8, **pin)
@@ -106,9 +107,12 @@ class HIPProposer(ProposerBase):
         self.pix_per_tile = ppt.value
         self._slot_key = None
 
-    def autotune(self, proposal, patch=None, repeats=5, configs=None, persistent=(True, False)):
+    def autotune(self, proposal, patch=None, repeats=5, configs=None, persistent=(True, False),
+                 balance=True):
         """Time every (configuration, persistent) pair on ``proposal`` and keep
-        the fastest; returns {(config, persistent): seconds per evaluation}."""
+        the fastest; with ``balance`` the tiles are first reordered for each
+        pair (``balance``).  Returns {(config, persistent): seconds per
+        evaluation}."""
         import torch
         if patch is None:
             patch = self.patch
@@ -118,6 +122,8 @@ class HIPProposer(ProposerBase):
             self.set_config(cfg)
             for pers in persistent:
                 self.persistent = pers
+                if balance:
+                    self.balance(proposal, patch)
                 self.launch(proposal, patch)
                 torch.cuda.synchronize()
                 a = torch.cuda.Event(enable_timing=True)
@@ -131,6 +137,8 @@ class HIPProposer(ProposerBase):
         best = min(timings, key=timings.get)
         self.set_config(best[0])
         self.persistent = best[1]
+        if balance:
+            self.balance(proposal, patch)
         return timings
 
     def _stream(self):
@@ -142,7 +150,14 @@ class HIPProposer(ProposerBase):
         tiles, n_tiles = patch.device_tiles(self.pix_per_tile)
         pw = 0 if pixel_weight is None else pixel_weight.data_ptr()
         key = (patch.device_patch.data_ptr(), tiles.data_ptr(), n_tiles, int(patch.n_bands), pw)
-        if key != self._slot_key:
+        if key != self._slot_key and key in self._slots:
+            self._slot, self._slot_key = self._slots[key], key
+        elif key != self._slot_key:
+            import torch
+            if len(self._slots) >= 4096:
+                self._slots.clear()
+            self._slot = torch.empty(SLOT_DTYPE.itemsize, dtype=torch.uint8, device=self.device)
+            self._slots[key] = self._slot
             slot = np.zeros(1, dtype=SLOT_DTYPE)
             slot["patch"] = patch.device_patch.data_ptr()
             slot["proposal"] = self._prop.data_ptr()
@@ -154,7 +169,6 @@ class HIPProposer(ProposerBase):
             slot["ticket"] = self._ticket.data_ptr()
             slot["n_bands"] = int(patch.n_bands)
             slot["n_tiles"] = n_tiles
-            import torch
             self._slot.copy_(torch.from_numpy(slot.view(np.uint8).reshape(-1).copy()))
             self._slot_key = key
         return n_tiles
@@ -163,6 +177,86 @@ class HIPProposer(ProposerBase):
         """Queue one evaluation on the current stream (no synchronisation).
         ``proposal`` is a numpy source_struct_dtype array or a device uint8
         tensor already holding the records."""
+        n_tiles = self._load(proposal, patch, pixel_weight)
+        stream = self._stream()
+        if self.persistent:
+            check(lib().fpx_evaluate_batch(self._slot.data_ptr(), 1, None, n_tiles,
+                                           patch.max_gauss(), self.config, 1,
+                                           self._queue.data_ptr(), stream.cuda_stream),
+                  "fpx_evaluate_batch")
+        else:
+            check(lib().fpx_evaluate_proposal(self._slot.data_ptr(), n_tiles, patch.max_gauss(),
+                                              self.config, stream.cuda_stream),
+                  "fpx_evaluate_proposal")
+        return n_tiles
+
+    def count_pairs(self, proposal, patch=None, per_tile=False):
+        """Number of (pixel, ImageGaussian) pairs of the patch inside the
+        cutoff dx^T F dx <= MAX_EXP_ARG for this proposal (fpx_count_pairs):
+        the pairs that carry the full per-pair work of common_kernel.cc.
+        With per_tile, also the counts of the tiles of the current tile table
+        (numpy, tile order)."""
+        import torch
+        if patch is None:
+            patch = self.patch
+        n_tiles = self._load(proposal, patch, None)
+        out = torch.zeros(1, dtype=torch.int64, device=self.device)
+        tiles = torch.zeros(max(n_tiles, 1), dtype=torch.int32, device=self.device)
+        check(lib().fpx_count_pairs(self._slot.data_ptr(), 1, None, n_tiles, patch.max_gauss(),
+                                    out.data_ptr(), tiles.data_ptr() if per_tile else None,
+                                    self._stream().cuda_stream),
+              "fpx_count_pairs")
+        if per_tile:
+            return int(out.item()), tiles[:n_tiles].cpu().numpy().view(np.uint32).astype(np.int64)
+        return int(out.item())
+
+    def balance(self, proposal, patch=None, tile_overhead=5000):
+        """Reorder the patch's tiles for the current configuration so that the
+        compute units get equal work, the work of a tile being its pairs
+        inside the cutoff for ``proposal`` (fpx_count_pairs) plus
+        ``tile_overhead`` pair-equivalents of fixed cost.
+
+        When the whole table is resident at once (fpx_resident_groups) and the
+        launch is not persistent, tile i runs on compute unit i mod n_cu: the
+        tiles are dealt heaviest first to the least-loaded unit that still
+        has a free slot.  Otherwise (a queue) they are sorted heaviest first.
+        Returns the permutation applied (new position -> old position)."""
+        import ctypes
+        import heapq
+        if patch is None:
+            patch = self.patch
+        _, cost = self.count_pairs(proposal, patch, per_tile=True)
+        cost = cost.astype(np.float64) + float(tile_overhead)
+        n = len(cost)
+        L = lib()
+        n_cu, resident = ctypes.c_int32(), ctypes.c_int32()
+        check(L.fpx_device_info(ctypes.byref(n_cu), None, None), "fpx_device_info")
+        check(L.fpx_resident_groups(self.config, patch.max_gauss(), ctypes.byref(resident)),
+              "fpx_resident_groups")
+        order = np.argsort(-cost, kind="stable")
+        if self.persistent or n > resident.value or n_cu.value < 1:
+            perm = order
+        else:
+            ncu = n_cu.value
+            rounds = -(-n // ncu)
+            # position r * ncu + u runs on unit u; the last round is partial,
+            # so units u >= n - (rounds - 1) * ncu hold one tile fewer
+            cap = np.where(np.arange(ncu) < n - (rounds - 1) * ncu, rounds, rounds - 1)
+            filled = np.zeros(ncu, dtype=np.int64)
+            load = [(0.0, u) for u in range(ncu) if cap[u] > 0]
+            heapq.heapify(load)
+            perm = np.full(n, -1, dtype=np.int64)
+            for t in order:
+                w, u = heapq.heappop(load)
+                perm[filled[u] * ncu + u] = t
+                filled[u] += 1
+                if filled[u] < cap[u]:
+                    heapq.heappush(load, (w + cost[t], u))
+        patch.reorder_tiles(self.pix_per_tile, perm)
+        return perm
+
+    def _load(self, proposal, patch, pixel_weight):
+        """Stage the proposal and the slot for ``patch``; returns the tile count."""
         import torch
         n_src = int(patch.n_sources)
         if n_src > MAXSOURCES or int(patch.n_bands) > MAXBANDS:
@@ -179,18 +273,7 @@ class HIPProposer(ProposerBase):
             self._h2d_done.record(self._stream())
         else:
             self._prop[:proposal.numel()].copy_(proposal, non_blocking=True)
-        n_tiles = self._slot_for(patch, pixel_weight)
-        stream = self._stream()
-        if self.persistent:
-            check(lib().fpx_evaluate_batch(self._slot.data_ptr(), 1, None, n_tiles,
-                                           patch.max_gauss(), self.config, 1,
-                                           self._queue.data_ptr(), stream.cuda_stream),
-                  "fpx_evaluate_batch")
-        else:
-            check(lib().fpx_evaluate_proposal(self._slot.data_ptr(), n_tiles, patch.max_gauss(),
-                                              self.config, stream.cuda_stream),
-                  "fpx_evaluate_proposal")
-        return n_tiles
+        return self._slot_for(patch, pixel_weight)
 
     def evaluate_proposal(self, proposal, patch=None, verbose=False, unpack=True):
         """Evaluate chi2 and dchi2/dparam of ``proposal`` on ``patch``."""

@@ -177,9 +177,11 @@ int fpx_n_configs(void);
 int fpx_config_info(int32_t cfg, int32_t *waves_per_group,
                     int32_t *pixels_per_lane, int32_t *pixels_per_tile);
 
-/* Bytes of LDS one workgroup of configuration cfg uses when the largest
- * exposure of a launch holds max_gauss ImageGaussians (n_sources x
- * n_psf_per_source).  Returns FPX_E_ARG if cfg is invalid. */
+/* Bytes of LDS one workgroup of configuration cfg uses on gfx950 when the
+ * largest exposure of a launch holds max_gauss ImageGaussians (n_sources x
+ * n_psf_per_source); the Gaussians that reach a tile are held in chunks of
+ * what fits a budget of 10 KiB per wavefront.  Returns FPX_E_ARG if cfg is
+ * invalid. */
 int fpx_lds_bytes(int32_t cfg, int32_t max_gauss, size_t *bytes);
 
 /* ------------------------------------------------------------------------ */
@@ -224,6 +226,27 @@ int fpx_evaluate_batch(const FpxSlot *slots, int32_t n_slots,
                        const int32_t *item_offsets, int32_t n_items,
                        int32_t max_gauss, int32_t cfg, int32_t persistent,
                        uint32_t *queue, void *stream);
+
+/* Measurement helper with no counterpart in the reference: counts, per
+ * slot, the (pixel, ImageGaussian) pairs inside the cutoff (dx^T F dx <=
+ * MAX_EXP_ARG, the test of common_kernel.cc:73,108) for the slots' current
+ * proposals, testing every Gaussian at every pixel.  Slots, item_offsets and
+ * n_items as for fpx_evaluate_batch; adds each slot's count to
+ * pairs_in[slot] (device, n_slots values) and, if item_pairs is not NULL,
+ * each tile's count to item_pairs[item] (device, n_items values); the caller
+ * zeroes both. */
+int fpx_count_pairs(const FpxSlot *slots, int32_t n_slots,
+                    const int32_t *item_offsets, int32_t n_items,
+                    int32_t max_gauss, unsigned long long *pairs_in,
+                    uint32_t *item_pairs, void *stream);
+
+/* Host-side planning helper with no counterpart in the reference: the
+ * number of workgroups of configuration cfg that are resident on the whole
+ * current device at once for a launch with max_gauss (the persistent grid
+ * size).  The dispatcher hands out the workgroups of a launch round-robin
+ * over the compute units, so a non-persistent launch of at most that many
+ * tiles normally runs tile i on compute unit i mod n_cu (fpx_device_info). */
+int fpx_resident_groups(int32_t cfg, int32_t max_gauss, int32_t *groups);
 
 /* Throughput probe of the VALU pipes used by the hot loop: kind 0 = FP32
  * FMA, kind 1 = v_exp_f32, 8 independent chains of `iters` steps per lane.

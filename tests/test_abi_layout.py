@@ -114,9 +114,13 @@ def test_configs_and_lds():
     for cfg in range(n):
         assert L.fpx_config_info(cfg, ctypes.byref(w), ctypes.byref(p), ctypes.byref(t)) == 0
         assert t.value == 64 * p.value and w.value >= 1
-        size = ctypes.c_size_t()
+        size, small = ctypes.c_size_t(), ctypes.c_size_t()
         assert L.fpx_lds_bytes(cfg, 540, ctypes.byref(size)) == 0
-        assert size.value >= 540 * 24 * 4
+        assert L.fpx_lds_bytes(cfg, 27, ctypes.byref(small)) == 0
+        # a workgroup stays within a CU's 160 KiB; few Gaussians need less
+        assert 27 * 36 * 4 < small.value <= size.value <= 160 * 1024
+        # the budget is 10 KiB per wavefront unless the records must spill over it
+        assert size.value <= max(10 * 1024 * w.value, small.value + 64 * 36 * 4)
     assert L.fpx_config_info(n, None, None, None) < 0
     assert L.fpx_accum_doubles(8) == 8 * (1 + 30 * 7)
 

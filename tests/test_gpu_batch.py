@@ -104,3 +104,43 @@ def test_vjp_mode_matches_fp64():
     chi2, dchi2 = prop.evaluate_proposal(proposal, patch=patch)
     c32, _, _ = oracle.evaluate_cpu(patch, proposal)
     assert np.max(np.abs(chi2 - c32) / np.abs(c32)) <= U.CHI2_RTOL
+
+
+def _pairs_inside_fp64(patch, proposal):
+    """(pixel, Gaussian) pairs with dx^T F dx <= MAX_EXP_ARG over the packed
+    pixels (padding included), from the float64 oracle's ImageGaussians."""
+    n = 0
+    for e in range(int(patch.n_exp)):
+        band = int(np.searchsorted(np.cumsum(patch.band_N), e, side="right"))
+        ig = fp64.image_gaussians(patch, proposal, e, band)
+        s, c = int(patch.exposure_start[e]), int(patch.exposure_N[e])
+        xp = patch.xpix[s:s + c].astype(np.float64)
+        yp = patch.ypix[s:s + c].astype(np.float64)
+        for g in range(len(ig["xcen"])):
+            dx, dy = xp - ig["xcen"][g], yp - ig["ycen"][g]
+            ea = ig["fxx"][g] * dx * dx + 2 * ig["fxy"][g] * dx * dy + ig["fyy"][g] * dy * dy
+            n += int(np.count_nonzero(ea <= fp64.MAX_EXP_ARG))
+    return n
+
+
+def test_pair_census(batch_setup):
+    """fpx_count_pairs: per-slot counts of the batch equal the single-patch
+    counts, and agree with a float64 count up to pairs on the boundary."""
+    from forcepho_amd.batch import BatchEvaluator
+    from forcepho_amd.proposal import HIPProposer
+    pats, chains, proposals, refs = batch_setup
+    ev = BatchEvaluator([p for p, _ in pats], chains=chains)
+    ev.set_proposals(proposals)
+    counts = ev.count_pairs()
+    prop = HIPProposer()
+    for s, (pi, _) in enumerate(ev.slot_map):
+        patch = pats[pi][0]
+        single = prop.count_pairs(proposals[s], patch)
+        assert counts[s] == single
+        n_gauss = int(patch.n_sources) * np.asarray(patch.n_psf_per_source)[
+            np.repeat(np.arange(int(patch.n_bands)), patch.band_N)]
+        assert 0 < single <= int(np.sum(patch.exposure_N * n_gauss))
+    for k in (0, 3):
+        s = ev.slot_map.index((k, 0))
+        ref = _pairs_inside_fp64(pats[k][0], proposals[s])
+        assert abs(int(counts[s]) - ref) <= max(5, 1e-5 * ref), (k, counts[s], ref)

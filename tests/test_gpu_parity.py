@@ -24,11 +24,11 @@ def _fp64(name, patch, proposal, cache):
     return cache[name]
 
 
-def _evaluate(name, config=None, return_residual=True):
+def _evaluate(name, config=None, return_residual=True, persistent=True):
     from forcepho_amd.proposal import HIPProposer
     patch, proposal, expected = U.load_golden(name, return_residual=return_residual)
     patch.send_to_device()
-    prop = HIPProposer(patch=patch, config=config)
+    prop = HIPProposer(patch=patch, config=config, persistent=persistent)
     out = prop.evaluate_proposal(proposal, patch=patch, unpack=False)
     return patch, proposal, expected, out
 
@@ -54,15 +54,57 @@ def test_fixture_parity(name, fp64_cache):
     assert np.max(np.abs(resid - expected["residual"])) <= 2e-6 * dscale, name
 
 
+@pytest.mark.parametrize("persistent", [True, False])
 @pytest.mark.parametrize("config", range(8))
-def test_all_configs_agree(config):
+def test_all_configs_agree(config, persistent):
     name = "multi_small_s2"
-    print(f"[gpu] config {config} on {name}", flush=True)
-    patch, proposal, expected, (chi2, dchi2, resid) = _evaluate(name, config=config)
+    print(f"[gpu] config {config} persistent {persistent} on {name}", flush=True)
+    patch, proposal, expected, (chi2, dchi2, resid) = _evaluate(name, config=config,
+                                                                persistent=persistent)
     c64, g64, mass, smass = U.fp64_reference(patch, proposal)
     rel = np.abs(chi2 - expected["chi2"]) / np.abs(expected["chi2"])
     assert rel.max() <= U.CHI2_RTOL
     assert U.grad_errors(dchi2, g64, mass, smass).max() <= 1.0
+    dscale = np.max(np.abs(patch.data)) + 1e-30
+    assert np.max(np.abs(resid - expected["residual"])) <= 2e-6 * dscale
+
+
+def test_autotune_keeps_results():
+    """The plain proposer's autotuner leaves a (config, persistent) choice
+    whose results still match the reference."""
+    from forcepho_amd.proposal import HIPProposer
+    name = "multi_small_s2"
+    patch, proposal, expected = U.load_golden(name, return_residual=False)
+    patch.send_to_device()
+    prop = HIPProposer(patch=patch)
+    timings = prop.autotune(proposal, patch, repeats=2, configs=[0, 2, 5])
+    assert len(timings) == 6 and all(t > 0 for t in timings.values())
+    chi2, dchi2 = prop.evaluate_proposal(proposal, patch=patch)
+    c64, g64, mass, smass = U.fp64_reference(patch, proposal)
+    assert np.max(np.abs(chi2 - expected["chi2"]) / np.abs(expected["chi2"])) <= U.CHI2_RTOL
+    assert U.grad_errors(dchi2, g64, mass, smass).max() <= 1.0
+
+
+@pytest.mark.parametrize("persistent", [True, False])
+def test_balanced_tiles_keep_results(persistent):
+    """Reordering the tile table for load balance changes no result beyond
+    summation order."""
+    from forcepho_amd.proposal import HIPProposer
+    name = "multi_small_s2"
+    patch, proposal, expected = U.load_golden(name, return_residual=True)
+    patch.send_to_device()
+    prop = HIPProposer(patch=patch, config=2, persistent=persistent)
+    before = prop.evaluate_proposal(proposal, patch=patch, unpack=False)
+    perm = prop.balance(proposal, patch)
+    n_tiles = patch.device_tiles(prop.pix_per_tile)[1]
+    assert sorted(perm.tolist()) == list(range(n_tiles))
+    after = prop.evaluate_proposal(proposal, patch=patch, unpack=False)
+    np.testing.assert_allclose(after[0], before[0], rtol=1e-12)
+    np.testing.assert_allclose(after[1], before[1], rtol=1e-5,
+                               atol=1e-6 * np.abs(before[1]).max())
+    np.testing.assert_array_equal(after[2], before[2])
+    rel = np.abs(after[0] - expected["chi2"]) / np.abs(expected["chi2"])
+    assert rel.max() <= U.CHI2_RTOL
 
 
 def test_repeatable_and_no_residual():
