@@ -49,6 +49,12 @@ def parse():
                     help="uneven patches in the batched measurement (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--no-probe", action="store_true", help="skip the VALU rate probe")
+    ap.add_argument("--no-autograd", action="store_true",
+                    help="skip the autograd + leapfrog measurement")
+    ap.add_argument("--ensemble-patches", type=int, default=64,
+                    help="patches of the HMC ensemble measurement (0 skips it)")
+    ap.add_argument("--ensemble-chains", type=int, default=64, help="chains per patch")
+    ap.add_argument("--leapfrog-steps", type=int, default=32)
     return ap.parse_args()
 
 
@@ -126,14 +132,14 @@ def traffic(cfg):
     return None
 
 
-def build_uneven(n_patches, seed):
+def build_uneven(n_patches, seed, with_scenes=False):
     """n_patches independent uneven patches: 1-20 sources, 1-8 bands, 1-4
     exposures per band, square stamps of side 48-256 px, 3-6 PSF Gaussians,
     9 radii (scenes.spec_uneven); data = model + noise."""
     from forcepho_amd import scenes
     from forcepho_amd.patches import SimplePatch
     rng = np.random.default_rng(seed)
-    patches, proposals = [], []
+    patches, proposals, scns = [], [], []
     for _ in range(n_patches):
         spec = scenes.spec_uneven(rng)
         patch, scene, _ = scenes.build_patch(spec, SimplePatch, scenes.render_on_device)
@@ -143,7 +149,117 @@ def build_uneven(n_patches, seed):
         prop["q"] *= np.float32(0.97)
         patches.append(patch)
         proposals.append(prop)
+        scns.append(scene)
+    if with_scenes:
+        return patches, proposals, scns
     return patches, proposals
+
+
+def autograd_measurement(args, patch, scene):
+    """The multi-band patch behind torch.autograd (PatchModel): chi2 value +
+    gradient calls per second, and one leapfrog trajectory of
+    --leapfrog-steps steps driven by the autograd gradient (hmc.leapfrog, the
+    inner integrator of forcepho's HMC), in unit-scaled parameters."""
+    import torch
+    from forcepho_amd.autograd import PatchModel
+    from forcepho_amd.hmc import leapfrog
+    model = PatchModel(patch, scene)
+    theta = scene.get_all_source_params().copy()
+    nb = int(patch.n_bands)
+    pix_deg = 0.03 / 3600.0
+    units = np.tile(np.concatenate([np.abs(theta.reshape(-1, nb + 6)[:, :nb]).mean(0).clip(1e-3)
+                                    * 0.01, [0.1 * pix_deg, 0.1 * pix_deg, 0.01, 0.01, 0.01,
+                                             0.001]]), model.n_sources)
+    q0 = theta / units
+    q0.reshape(-1, nb + 6)[:, nb + 2] *= 0.99
+
+    def U_and_grad(q):
+        t = torch.tensor(q * units, dtype=torch.float64, device=model.proposer.device,
+                         requires_grad=True)
+        u = 0.5 * model.chi2(t)
+        u.backward()
+        return u.item(), t.grad.cpu().numpy() * units
+
+    for _ in range(3):
+        U_and_grad(q0)
+    n = 20
+    t0 = time.perf_counter()
+    for _ in range(n):
+        U_and_grad(q0)
+    call_s = (time.perf_counter() - t0) / n
+    p0 = np.random.default_rng(args.seed).normal(size=len(q0))
+    leapfrog(q0, p0, U_and_grad, eps=0.01, n_steps=2)
+    t0 = time.perf_counter()
+    _, _, _, calls = leapfrog(q0, p0, U_and_grad, eps=0.01, n_steps=args.leapfrog_steps)
+    traj_s = time.perf_counter() - t0
+    n_evals = patch.n_evals()
+    return {"workload": "torch.autograd chi2 of the multi-band patch (forward: residual image, "
+                        "backward: hand-written gradient pass)",
+            "value_and_grad_per_s": 1.0 / call_s, "ms_per_value_and_grad": call_s * 1e3,
+            "evals_per_s": n_evals / call_s,
+            "leapfrog_steps": args.leapfrog_steps, "gradient_calls": calls,
+            "trajectory_s": traj_s, "trajectories_per_s": 1.0 / traj_s}
+
+
+def ensemble_bounds(theta, nb):
+    """Prior box of one patch's parameters around theta (fluxes x0.5-1.5,
+    +-3 pixels, q in [0.2, 1], pa in [-2, 2], sersic in [0.8, 5.5], rh in
+    [0.03, 0.4] arcsec)."""
+    t = theta.reshape(-1, nb + 6)
+    lo, hi = t.copy(), t.copy()
+    lo[:, :nb], hi[:, :nb] = t[:, :nb] * 0.5, t[:, :nb] * 1.5 + 1e-3
+    pix = 0.03 / 3600.0
+    lo[:, nb:nb + 2] -= 3 * pix
+    hi[:, nb:nb + 2] += 3 * pix
+    lo[:, nb + 2], hi[:, nb + 2] = 0.2, 1.0
+    lo[:, nb + 3], hi[:, nb + 3] = -2.0, 2.0
+    lo[:, nb + 4], hi[:, nb + 4] = 0.8, 5.5
+    lo[:, nb + 5], hi[:, nb + 5] = 0.03, 0.4
+    return lo.reshape(-1), hi.reshape(-1)
+
+
+def ensemble_measurement(args):
+    """--ensemble-chains chains on each of --ensemble-patches uneven patches,
+    all positions on the device, one --leapfrog-steps leapfrog trajectory of
+    every chain recorded once into a graph (batched persistent chi2 +
+    gradient launches, bounded transform, mixture amplitudes, momentum and
+    position updates) and replayed."""
+    import torch
+    from forcepho_amd.ensemble import HMCEnsemble
+    t0 = time.perf_counter()
+    patches, _, scns = build_uneven(args.ensemble_patches, args.seed + 2, with_scenes=True)
+    bounds, start = [], []
+    thetas = [sc.get_all_source_params().copy() for sc in scns]
+    for patch, th in zip(patches, thetas):
+        bounds.append(ensemble_bounds(th, int(patch.n_bands)))
+    ens = HMCEnsemble(patches, scns, args.ensemble_chains, bounds)
+    for pi, c in ens.batch.slot_map:
+        t = thetas[pi].copy()
+        nb = int(patches[pi].n_bands)
+        t.reshape(-1, nb + 6)[:, nb + 2] *= 0.98 - 0.001 * c
+        start.append(t)
+    ens.set_theta(start)
+    ens.potential()                    # source records of the start positions
+    build_s = time.perf_counter() - t0
+    ens.batch.autotune(repeats=1, configs=[2, 6, 7], persistent=(True,))
+    gen = torch.Generator(device=ens.device)
+    gen.manual_seed(args.seed)
+    ens.eps.fill_(0.005)
+    ens.p.copy_(torch.randn(ens.p.shape, generator=gen, dtype=torch.float64,
+                            device=ens.device) * ens.fmask)
+    t1 = time.perf_counter()
+    ens.capture(args.leapfrog_steps)
+    capture_s = time.perf_counter() - t1
+    traj_s = time_region(ens.graph.replay, 1)
+    n_evals = ens.batch.n_evals() * (args.leapfrog_steps + 1)
+    return {"workload": f"{ens.n_slots} chains ({args.ensemble_chains} per patch) on "
+                        f"{len(patches)} uneven patches, graph-captured {args.leapfrog_steps}-step "
+                        "leapfrog, state on the device",
+            "chains": ens.n_slots, "trajectory_s": traj_s,
+            "trajectories_per_s": ens.n_slots / traj_s,
+            "evals_per_s": n_evals / traj_s,
+            "kernel_config": "{}{}".format(ens.batch.config, "p" if ens.batch.persistent else ""),
+            "host_build_s": build_s, "warmup_and_capture_s": capture_s}
 
 
 def time_region(fn, steps):
@@ -307,6 +423,8 @@ def main():
     n_inside = prop.count_pairs(pdev, patch)
     probe = None if args.no_probe else valu_probe()
     batched = batched_measurement(args, prop) if args.batch_patches > 0 else None
+    ensemble = ensemble_measurement(args) if args.ensemble_patches > 0 else None
+    autograd = None if args.no_autograd else autograd_measurement(args, patch, scene)
     line = {
         "metric": METRIC,
         "value": value,
@@ -325,6 +443,8 @@ def main():
                    "wall_s": wall},
         "roofline": dict(roofline(n_evals, n_inside, kernel_s), traffic=traffic(prop.config)),
         "batched": batched,
+        "autograd": autograd,
+        "ensemble": ensemble,
         "valu_probe": probe,
     }
     if not args.no_cpu:

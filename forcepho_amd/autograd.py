@@ -57,13 +57,16 @@ class PatchModel:
         self.scene.set_all_source_params(np.asarray(theta, dtype=np.float64))
         return self.scene.get_proposal()
 
-    def forward_residual(self, proposal):
+    def forward_residual(self, proposal, outputs=True):
         """Launch the chi2 kernel for ``proposal``; returns (residual float32
-        device tensor [npix] (a copy), chi2 per band, dchi2 [n_bands, n_src, 7])."""
+        device tensor [npix] (a copy), chi2 per band, dchi2 [n_bands, n_src, 7]),
+        or the residual alone (no host synchronisation) without ``outputs``."""
         torch = _torch()
         p = self.proposer
         p.launch(proposal, self.patch)
         resid = self.patch.device_ptrs["residual"].view(torch.float32)[:self.patch.npix].clone()
+        if not outputs:
+            return resid
         nb = self.n_bands
         chi2 = p._chi2[:nb].cpu().numpy()
         resp = p._resp[:nb * 210].view(nb, 30, 7)[:, :self.n_sources].cpu().numpy()
@@ -100,7 +103,7 @@ def _make_function():
         @staticmethod
         def forward(ctx, theta, model):
             proposal = model.proposal(theta.detach().cpu().numpy())
-            resid, _, _ = model.forward_residual(proposal)
+            resid = model.forward_residual(proposal, outputs=False)
             ctx.model = model
             ctx.proposal = proposal
             ctx.theta_meta = (theta.dtype, theta.device)
