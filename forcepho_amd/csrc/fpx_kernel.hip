@@ -26,9 +26,10 @@
 //             Jacobian CW is applied once per source after the Gaussian
 //             loop), wave-reduced into per-wave LDS rows.
 //   combine   per-wave rows summed in fixed order, one FP64 global atomic per
-//             value into the slot's per-band accumulator; the last tile of the
-//             slot (ticket counter) writes pchi2 / Response and re-zeroes the
-//             accumulator.
+//             value into the slot's per-band accumulator.  A second kernel
+//             (finalize_kernel, same stream) writes pchi2 / Response from the
+//             accumulators and re-zeroes them: the kernel boundary orders it
+//             after every tile, with no per-tile fence or completion count.
 // A Gaussian is skipped at a pixel when dx^T F dx > MAX_EXP_ARG
 // (common_kernel.cc:73,108).  Two shortcuts skip work that this test would
 // discard anyway: the prologue builds records only for Gaussians whose
@@ -483,10 +484,11 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
 }
 
 // Add the tile's per-wave rows to the slot accumulator (band < 0: nothing
-// to add) and, for the slot's last tile, write the outputs.
+// to add).  No fence or completion count: the outputs are written by
+// finalize_kernel, launched after this kernel on the same stream.
 template <int W>
 __device__ void finish_tile(const FpxSlot &slot, int band, int n_src, unsigned char *smem,
-                            int gauss_cap, int ppl, int *s_flag) {
+                            int gauss_cap, int ppl) {
   const int tid = threadIdx.x;
   const Lds l = carve(smem, W, ppl, gauss_cap);
   if (band >= 0) {
@@ -502,36 +504,31 @@ __device__ void finish_tile(const FpxSlot &slot, int band, int n_src, unsigned c
       if (v != 0.0) unsafeAtomicAdd(acc + j, v);
     }
   }
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) {
-    const unsigned t = atomicAdd(slot.ticket, 1u);
-    *s_flag = (t == (unsigned)(slot.n_tiles - 1)) ? 1 : 0;
-  }
-  __syncthreads();
-  if (*s_flag) {
-    __threadfence();
-    const int nb = min(slot.n_bands, MAXBANDS);
-    for (int j = tid; j < nb * kAcc; j += W * 64) {
-      const double v = __hip_atomic_load(slot.accum + j, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-      const int b = j / kAcc;
-      const int k = j - b * kAcc;
-      if (k == 0) {
-        slot.pchi2[b] = v;
-      } else {
-        slot.pdchi2_dp[b * (MAXSOURCES * NPARAMS) + (k - 1)] = (float)v;
-      }
-      slot.accum[j] = 0.0;
+  __syncthreads();  // the rows are reused by the next tile
+}
+
+// Outputs of every slot from its accumulator, which is left zero: pchi2[b]
+// and the float Response of each band.  One workgroup per slot.
+__global__ __launch_bounds__(256) void finalize_kernel(const FpxSlot *__restrict__ slots,
+                                                       int n_slots) {
+  const FpxSlot slot = slots[blockIdx.x];
+  const int nb = min(slot.n_bands, MAXBANDS);
+  for (int j = threadIdx.x; j < nb * kAcc; j += blockDim.x) {
+    const double v = __hip_atomic_load(slot.accum + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int b = j / kAcc;
+    const int k = j - b * kAcc;
+    if (k == 0) {
+      slot.pchi2[b] = v;
+    } else {
+      slot.pdchi2_dp[b * (MAXSOURCES * NPARAMS) + (k - 1)] = (float)v;
     }
-    if (tid == 0) *slot.ticket = 0u;
+    __hip_atomic_store(slot.accum + j, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  __syncthreads();
 }
 
 template <int W, int PPL>
 __device__ void process_tile(const FpxSlot &slot, int local, int gauss_cap, int index_cap,
-                             unsigned char *smem, int *s_flag) {
+                             unsigned char *smem) {
   constexpr int kPix = 64 * PPL;
   // Scalars that every lane holds identically.  The loads below are vector
   // loads (the kernel writes global memory, so they cannot be proven constant),
@@ -550,7 +547,7 @@ __device__ void process_tile(const FpxSlot &slot, int local, int gauss_cap, int 
                   band < MAXBANDS && tile.pix_count >= 0 && tile.pix_count <= kPix;
   if (ok) tile_work<W, PPL>(slot, tile, P, props, e, band, n_src, n_psf, gauss_cap, smem);
   __syncthreads();
-  finish_tile<W>(slot, ok ? band : -1, ok ? n_src : 0, smem, gauss_cap, PPL, s_flag);
+  finish_tile<W>(slot, ok ? band : -1, ok ? n_src : 0, smem, gauss_cap, PPL);
 }
 
 template <int W, int PPL>
@@ -559,7 +556,6 @@ __global__ __launch_bounds__(W * 64) void chi2_grad_kernel(
     int n_items, int gauss_cap, int index_cap, int persistent, uint32_t *__restrict__ queue) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_item;
-  __shared__ int s_flag;
   for (int it = 0;; ++it) {
     int item;
     if (persistent) {
@@ -584,7 +580,7 @@ __global__ __launch_bounds__(W * 64) void chi2_grad_kernel(
       local = item - item_offsets[si];
     }
     const FpxSlot slot = slots[si];
-    process_tile<W, PPL>(slot, local, gauss_cap, index_cap, smem, &s_flag);
+    process_tile<W, PPL>(slot, local, gauss_cap, index_cap, smem);
   }
   if (persistent) {
     __syncthreads();
@@ -724,8 +720,8 @@ int hip_err(hipError_t e, const char *where) {
 // per SIMD (16 per CU) fit in the 160 KiB of a CU; a tile whose surviving
 // Gaussians exceed the records that fit is processed in chunks.
 constexpr int kLdsPerWave = 10 * 1024;
-// static LDS of the chi2 kernel (the queue item and the ticket flag), for the
-// host-only size query; launches use the kernel's reported size
+// static LDS of the chi2 kernel (the queue item), for the host-only size
+// query; launches use the kernel's reported size
 constexpr int kStaticLds = 16;
 
 struct KernelCfg {
@@ -886,6 +882,9 @@ int launch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
                   (void *)&n_items,   (void *)&cap,        (void *)&index_cap,
                   (void *)&persistent, (void *)&queue};
   FPX_HIP(hipLaunchKernel(kc.fn, dim3(grid), dim3(threads), args, lds, (hipStream_t)stream));
+  void *fargs[] = {(void *)&slots, (void *)&n_slots};
+  FPX_HIP(hipLaunchKernel(reinterpret_cast<const void *>(&fpx::finalize_kernel), dim3(n_slots),
+                          dim3(256), fargs, 0, (hipStream_t)stream));
   return FPX_OK;
 }
 

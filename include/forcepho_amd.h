@@ -114,7 +114,7 @@ typedef struct FpxTile {
  * many (many patches, many HMC chains on one patch, ...).  80 bytes.
  *   accum  : double[fpx_accum_doubles(n_bands)], zero before the first launch;
  *            the kernel leaves it zero again after each launch
- *   ticket : uint32, zero before the first launch; left zero after each launch
+ *   ticket : reserved (unused by this version; may be NULL)
  *   pixel_weight : NULL for chi2 mode.  Non-NULL selects vector-Jacobian mode:
  *            no model pass, pchi2 is written as 0 and pdchi2_dp receives
  *            sum_pix pixel_weight[pix] * d model[pix] / d param.
@@ -197,7 +197,8 @@ int fpx_init(int32_t device);
  * the current device. */
 int fpx_device_info(int32_t *n_cu, int32_t *lds_per_group, int32_t *clock_khz);
 
-/* chi2 and dchi2/dparam for one proposal on one patch.
+/* chi2 and dchi2/dparam for one proposal on one patch (two kernels on
+ * `stream`: the tiles, then the per-slot output pass).
  * Replaces the PyCUDA launch of EvaluateProposal<<<n_bands, 1024, 48000>>>
  * (forcepho/proposal.py:133-136, forcepho/src/compute_gaussians_kernel.cu:213-312)
  * and, per band, the C++ EvaluateProposal(band, patch, proposal)
