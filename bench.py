@@ -155,6 +155,33 @@ def build_uneven(n_patches, seed, with_scenes=False):
     return patches, proposals
 
 
+def single_plain_measurement(args):
+    """The single-patch plain launch path: one 64x64 exposure, one source,
+    9 radii x 3 PSF Gaussians, evaluated back-to-back with device-resident
+    inputs (launch-bound), and through evaluate_proposal (host proposal in,
+    results out)."""
+    import torch
+    from forcepho_amd.proposal import HIPProposer
+    patch, scene, desc = build_workload("single", args.seed)
+    patch.return_residual = False
+    patch.send_to_device()
+    proposal = scene.get_proposal()
+    raw = np.ascontiguousarray(proposal).view(np.uint8).reshape(-1)
+    pdev = torch.from_numpy(raw.copy()).cuda()
+    prop = HIPProposer(patch=patch)
+    prop.autotune(pdev, patch, repeats=20)
+    n = patch.n_evals()
+    launch_s = time_region(lambda: prop.launch(pdev, patch), 200)
+    prop.evaluate_proposal(proposal, patch=patch)
+    t0 = time.perf_counter()
+    for _ in range(200):
+        prop.evaluate_proposal(proposal, patch=patch)
+    api_s = (time.perf_counter() - t0) / 200
+    return {"workload": desc, "evals_per_step": n,
+            "launch_evals_per_s": n / launch_s, "launch_us": launch_s * 1e6,
+            "evaluate_proposal_evals_per_s": n / api_s, "evaluate_proposal_us": api_s * 1e6}
+
+
 def autograd_measurement(args, patch, scene):
     """The multi-band patch behind torch.autograd (PatchModel): chi2 value +
     gradient calls per second, and one leapfrog trajectory of
@@ -422,7 +449,11 @@ def main():
     value = n_evals * args.steps / region
     n_inside = prop.count_pairs(pdev, patch)
     probe = None if args.no_probe else valu_probe()
+    single = single_plain_measurement(args)
     batched = batched_measurement(args, prop) if args.batch_patches > 0 else None
+    if batched is not None:
+        batched["speedup_vs_single_patch_plain_launches"] = (
+            batched["evals_per_s"] / single["launch_evals_per_s"])
     ensemble = ensemble_measurement(args) if args.ensemble_patches > 0 else None
     autograd = None if args.no_autograd else autograd_measurement(args, patch, scene)
     line = {
@@ -442,6 +473,7 @@ def main():
                    "kernel_config": best, "config_ms": {str(k): v * 1e3 for k, v in timings.items()},
                    "wall_s": wall},
         "roofline": dict(roofline(n_evals, n_inside, kernel_s), traffic=traffic(prop.config)),
+        "single_plain": single,
         "batched": batched,
         "autograd": autograd,
         "ensemble": ensemble,

@@ -13,8 +13,8 @@
 //             Gaussian can reach the tile's pixel box and the survivors are
 //             compacted in order; their ImageGaussian records (fpx_gauss.h)
 //             are built into LDS, in chunks when they exceed the LDS budget.
-//   pass 1    model = sum_g amp_g G_g H_g over the wave's Gaussians (every
-//             W-th record, so that costly and cheap Gaussians spread evenly)
+//   pass 1    model = sum_g amp_g G_g H_g over the wave's Gaussians (runs of
+//             8 records dealt round-robin to the wavefronts)
 //             (common_kernel.cc:57-83), partial models combined through LDS,
 //             residual r = d - model, chi2 += (r ierr - d ierr)(r ierr + d ierr)
 //             in FP64 (compute_gaussians_kernel.cc:286-293), weight
@@ -234,14 +234,20 @@ __device__ __forceinline__ unsigned gauss_at_pixels(const float4 r0, const float
   return blocks;
 }
 
-// Pass 1: add records g0, g0 + step, ... < n (this wave's share) to the
-// lane's model.
+// A wavefront's share of a chunk's n records: runs of kRun consecutive
+// records dealt round-robin over the W wavefronts (run r to wavefront r mod W),
+// which spreads costly and cheap Gaussians evenly while keeping a wave's
+// records mostly within one source (few per-source flushes).
+constexpr int kRun = 8;
+
+// Pass 1: add this wave's share of records [0, n) to the lane's model.
 template <int PPL>
-__device__ __forceinline__ void model_pass(const float *__restrict__ recs, int g0, int step,
+__device__ __forceinline__ void model_pass(const float *__restrict__ recs, int wave, int W,
                                            int n, const float (&xp)[PPL],
                                            const float (&yp)[PPL], float (&m)[PPL]) {
   const float4 *R4 = reinterpret_cast<const float4 *>(recs);
-  for (int g = g0; g < n; g += step) {
+  for (int r0 = wave * kRun; r0 < n; r0 += W * kRun)
+  for (int g = r0; g < min(r0 + kRun, n); ++g) {
     const float4 r0 = R4[g * (kRec / 4) + 0];  // xc yc cxx cxy
     const float4 r1 = R4[g * (kRec / 4) + 1];  // cyy h0 sxx sxy
     const float4 r2 = R4[g * (kRec / 4) + 2];  // syy amp src dA/dflux
@@ -304,13 +310,12 @@ __device__ __forceinline__ void flush_source(int s, float aF, float aX, float aY
   if ((lane & 7) == 0 && k < NPARAMS) row_grad(wrow)[NPARAMS * s + k] += t;
 }
 
-// Pass 2: gradient accumulation over records g0, g0 + step, ... < n (this
-// wave's share).  Per Gaussian the lane sums gw m_k and gw H m_k over its pixels
+// Pass 2: gradient accumulation over this wave's share of records [0, n).  Per Gaussian the lane sums gw m_k and gw H m_k over its pixels
 // (m = 1, dx, dy, dx^2, dy^2, dx dy) and maps the 12 sums to the source's 7
 // accumulators with the record's coefficients; a source's sums are flushed
 // when the next record belongs to another source.
 template <int PPL>
-__device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int g0, int step,
+__device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int wave, int W,
                                           int n, const float (&xp)[PPL], const float (&yp)[PPL],
                                           const float (&w2)[PPL], const float *__restrict__ cwl,
                                           unsigned char *__restrict__ wrow, int lane) {
@@ -318,7 +323,8 @@ __device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int g0
   float aF = 0.f, aX = 0.f, aY = 0.f, aQ = 0.f, aP = 0.f, aN = 0.f, aR = 0.f;
   bool touched = false;
   int cur = -1;
-  for (int g = g0; g < n; g += step) {
+  for (int r0 = wave * kRun; r0 < n; r0 += W * kRun)
+  for (int g = r0; g < min(r0 + kRun, n); ++g) {
     const float4 *rg = R4 + g * (kRec / 4);
     const float4 r0 = rg[0];  // xc yc cxx cxy
     const float4 r1 = rg[1];  // cyy h0 sxx sxy
@@ -854,10 +860,15 @@ int launch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
   if (max_gauss <= 0 || max_gauss > MAXSOURCES * FPX_MAX_PSF_PER_SOURCE)
     return set_err(FPX_E_ARG, "max_gauss out of range [1, MAXSOURCES*FPX_MAX_PSF_PER_SOURCE]");
   if (persistent && !queue) return set_err(FPX_E_ARG, "persistent launch needs a queue counter");
-  if (n_items == 0) return FPX_OK;
   DeviceState *st = nullptr;
   rc = current_device_state(&st);
   if (rc) return rc;
+  void *fargs[] = {(void *)&slots, (void *)&n_slots};
+  const void *finalize = reinterpret_cast<const void *>(&fpx::finalize_kernel);
+  if (n_items == 0) {  // no pixels: chi2 and gradients are zero
+    FPX_HIP(hipLaunchKernel(finalize, dim3(n_slots), dim3(256), fargs, 0, (hipStream_t)stream));
+    return FPX_OK;
+  }
   const KernelCfg &kc = kConfigs[cfg];
   int cap = 0, index_cap = 0;
   size_t lds = 0;
@@ -882,9 +893,7 @@ int launch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
                   (void *)&n_items,   (void *)&cap,        (void *)&index_cap,
                   (void *)&persistent, (void *)&queue};
   FPX_HIP(hipLaunchKernel(kc.fn, dim3(grid), dim3(threads), args, lds, (hipStream_t)stream));
-  void *fargs[] = {(void *)&slots, (void *)&n_slots};
-  FPX_HIP(hipLaunchKernel(reinterpret_cast<const void *>(&fpx::finalize_kernel), dim3(n_slots),
-                          dim3(256), fargs, 0, (hipStream_t)stream));
+  FPX_HIP(hipLaunchKernel(finalize, dim3(n_slots), dim3(256), fargs, 0, (hipStream_t)stream));
   return FPX_OK;
 }
 

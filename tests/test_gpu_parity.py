@@ -134,3 +134,56 @@ def test_chi2_convention():
     per_pix = (r - d) * (r + d)
     bands = np.split(per_pix, np.cumsum(patch.band_N_pix[:-1]))
     np.testing.assert_allclose(chi2, [x.sum() for x in bands], rtol=1e-6)
+
+
+@pytest.mark.parametrize("config", range(8))
+def test_chunked_records_every_config(config):
+    """30 sources: the surviving records exceed one LDS chunk for the small
+    configurations, so they are built and consumed in several chunks."""
+    name = "max_sources_s3"
+    print(f"[gpu] chunks config {config} on {name}", flush=True)
+    patch, proposal, expected, (chi2, dchi2, resid) = _evaluate(name, config=config)
+    c64, g64, mass, smass = _fp64(name, patch, proposal, {})
+    rel = np.abs(chi2 - expected["chi2"]) / np.abs(expected["chi2"])
+    assert rel.max() <= U.CHI2_RTOL
+    assert U.grad_errors(dchi2, g64, mass, smass).max() <= 1.0
+    dscale = np.max(np.abs(patch.data)) + 1e-30
+    assert np.max(np.abs(resid - expected["residual"])) <= 2e-6 * dscale
+
+
+def test_thirty_bands():
+    """MAXBANDS bands of one small exposure each: band indexing and the
+    per-band accumulators at their limit, against the C restatement."""
+    from forcepho_amd import scenes
+    from forcepho_amd.patches import SimplePatch
+    from forcepho_amd.proposal import HIPProposer
+    rng = np.random.default_rng(30)
+    spec = scenes.draw_patch_spec(rng, 30, [1] * 30, [16] * 30, 2, [3] * 30)
+    patch, scene, theta = scenes.build_patch(spec, SimplePatch, oracle.render_model)
+    patch.send_to_device()
+    proposal = scene.get_proposal()
+    proposal["q"] *= np.float32(0.95)
+    chi2, dchi2 = HIPProposer(patch=patch).evaluate_proposal(proposal, patch=patch)
+    c32, _, _ = oracle.evaluate_cpu(patch, proposal)
+    assert chi2.shape == (30,) and dchi2.shape == (30, 2, 7)
+    assert np.max(np.abs(chi2 - c32) / np.abs(c32)) <= U.CHI2_RTOL
+    c64, g64, mass, smass = U.fp64_reference(patch, proposal)
+    assert U.grad_errors(dchi2, g64, mass, smass).max() <= 1.0
+
+
+def test_launch_without_tiles_writes_zeros():
+    """A launch with no tiles (no pixels) still writes the outputs: chi2 and
+    gradients of an empty pixel set are zero."""
+    import torch
+    from forcepho_amd._lib import lib, check
+    from forcepho_amd.proposal import HIPProposer
+    patch, proposal, expected = U.load_golden("single_s1", return_residual=False)
+    patch.send_to_device()
+    prop = HIPProposer(patch=patch, persistent=False)
+    chi2, _ = prop.evaluate_proposal(proposal, patch=patch)
+    assert np.all(chi2 != 0)
+    check(lib().fpx_evaluate_proposal(prop._slot.data_ptr(), 0, patch.max_gauss(), prop.config,
+                                      torch.cuda.current_stream().cuda_stream), "empty launch")
+    torch.cuda.synchronize()
+    assert np.all(prop._chi2[:int(patch.n_bands)].cpu().numpy() == 0)
+    assert np.all(prop._resp.cpu().numpy() == 0)
