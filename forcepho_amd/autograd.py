@@ -13,11 +13,18 @@ of forcepho/model.py:515-551 played the same role for pymc3.
 sum (r ierr)^2 - (d ierr)^2 (compute_gaussians_kernel.cc:286-293) on top,
 so ``-0.5 * chi2`` is forcepho's log-likelihood and its autograd gradient
 matches FastPosterior.lnprob_grad in the untransformed space.
+
+When every source uses LogNormalMixtureAmplitudes the Source records are
+built on the device from theta (device_scene.DeviceRecords) and the gradient
+is stacked there too: forward and backward then make no host round trip.
+Other amplitude models build the records on the host (Galaxy.proposal).
 """
 
 import numpy as np
 
 from .model import stack_grad
+from .device_scene import DeviceRecords, device_amplitude_models_ok, stack_grad_device
+from .kernel_limits import MAXSOURCES
 
 __all__ = ["PatchModel", "PatchResidual"]
 
@@ -52,6 +59,18 @@ class PatchModel:
         self.ierr = torch.from_numpy(np.asarray(patch.ierr, np.float64)).to(dev)
         self.n_sources, self.n_bands = int(patch.n_sources), int(patch.n_bands)
         self.ndim = self.n_sources * (self.n_bands + 6)
+        self.records = None
+        if device_amplitude_models_ok(scene):
+            buf = torch.zeros(MAXSOURCES * 264, dtype=torch.uint8, device=dev)
+            self.records = DeviceRecords([scene], [self.n_bands], buf, self.n_sources,
+                                         self.n_bands)
+
+    def device_proposal(self, theta):
+        """Source records of theta (any device, float64 [ndim]) as a uint8
+        device tensor (a copy, owned by the caller)."""
+        t = theta.detach().to(self.records.records.device, dtype=self.records.lnr.dtype)
+        self.records.write(t.reshape(1, self.n_sources, self.n_bands + 6))
+        return self.records.records.clone()
 
     def proposal(self, theta):
         self.scene.set_all_source_params(np.asarray(theta, dtype=np.float64))
@@ -102,7 +121,10 @@ def _make_function():
 
         @staticmethod
         def forward(ctx, theta, model):
-            proposal = model.proposal(theta.detach().cpu().numpy())
+            if model.records is not None:
+                proposal = model.device_proposal(theta)
+            else:
+                proposal = model.proposal(theta.detach().cpu().numpy())
             resid = model.forward_residual(proposal, outputs=False)
             ctx.model = model
             ctx.proposal = proposal
@@ -112,11 +134,15 @@ def _make_function():
         @staticmethod
         def backward(ctx, grad_resid):
             model = ctx.model
+            dtype, device = ctx.theta_meta
             # residual = data - model, so dL/dtheta = -sum_pix g * dmodel/dtheta
             pw = (-grad_resid).to(torch.float32).contiguous()
+            if model.records is not None:
+                d = model.proposer.vjp_device(ctx.proposal, pw, model.patch)
+                g = stack_grad_device(d[None], model.n_sources, model.n_bands)[0]
+                return g.reshape(-1).to(dtype=dtype, device=device), None
             d = model.proposer.vjp(ctx.proposal, pw, model.patch)
             g = stack_grad(d, model.n_sources, model.n_bands)
-            dtype, device = ctx.theta_meta
             return torch.from_numpy(g).to(dtype=dtype, device=device), None
 
     return _Residual

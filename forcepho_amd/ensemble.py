@@ -12,25 +12,18 @@ Every chain's position lives on the device; one leapfrog step is
       -> momentum / position updates,
 all as device operations with no host round trip, so a whole trajectory can
 be recorded once into a graph and replayed.  The mixture amplitudes use the
-analytic LogNormalMixtureAmplitudes model (sources.py), evaluated here on the
-device for every (chain, source).  Momentum draws and the Metropolis test run
+analytic LogNormalMixtureAmplitudes model (sources.py), evaluated on the
+device for every (chain, source) by device_scene.DeviceRecords.  Momentum draws and the Metropolis test run
 outside the graph.
 """
 
 import numpy as np
 
-from .kernel_limits import MAXBANDS, MAXRADII, MAXSOURCES, NPARAMS
+from .kernel_limits import MAXBANDS, MAXSOURCES, NPARAMS
 from .batch import BatchEvaluator
-from .sources import LogNormalMixtureAmplitudes
+from .device_scene import DeviceRecords, stack_grad_device
 
 __all__ = ["HMCEnsemble"]
-
-# float offsets of the Source record fields (include/forcepho_amd.h FpxSource)
-_F_SHAPE, _F_FLUX = 0, 6
-_F_AMP = 6 + MAXBANDS
-_F_DRH = _F_AMP + MAXRADII
-_F_DN = _F_DRH + MAXRADII
-_REC = _F_DN + MAXRADII   # 66 floats
 
 
 def _torch():
@@ -64,8 +57,6 @@ class HMCEnsemble:
         lo = np.zeros((n, self.S, self.K))
         hi = np.ones((n, self.S, self.K))
         mask = np.zeros((n, self.S, self.K), dtype=bool)
-        lnr = np.full((n, MAXRADII), np.log(1e-4))
-        rmask = np.zeros((n, MAXRADII), dtype=bool)
         for s, (pi, _) in enumerate(b.slot_map):
             patch = patches[pi]
             ns, nb = int(patch.n_sources), int(patch.n_bands)
@@ -74,26 +65,19 @@ class HMCEnsemble:
             lo[s, :ns, self.NB:], hi[s, :ns, self.NB:] = l[:, nb:], h[:, nb:]
             mask[s, :ns, :nb] = True
             mask[s, :ns, self.NB:] = True
-            model = scenes[pi].sources[0].amplitude_model
-            if not isinstance(model, LogNormalMixtureAmplitudes):
-                raise TypeError("device amplitudes implement LogNormalMixtureAmplitudes only")
-            r = len(model.radii)
-            lnr[s, :r] = model._lnr
-            rmask[s, :r] = True
         f64 = dict(dtype=torch.float64, device=dev)
         self.lo = torch.tensor(lo, **f64)
         self.hi = torch.tensor(hi, **f64)
         self.mask = torch.tensor(mask, device=dev)
         self.fmask = self.mask.to(torch.float64)
-        self.lnr = torch.tensor(lnr, **f64)
-        self.rmask = torch.tensor(rmask, device=dev)
+        self.records = DeviceRecords([scenes[pi] for pi, _ in b.slot_map],
+                                     [int(patches[pi].n_bands) for pi, _ in b.slot_map],
+                                     b.proposals, self.S, self.NB)
         self.z = torch.zeros((n, self.S, self.K), **f64)
         self.p = torch.zeros_like(self.z)
         self.g = torch.zeros_like(self.z)
         self.U = torch.zeros(n, **f64)
         self.eps = torch.full((n, 1, 1), 0.01, **f64)
-        self.props = b.proposals.view(torch.float32).view(n, MAXSOURCES, _REC)
-        self.props.zero_()
         self.resp = b.resp.view(n, MAXBANDS, MAXSOURCES, NPARAMS)
         self.chi2 = b.chi2.view(n, MAXBANDS)
         self.graph = None
@@ -128,42 +112,16 @@ class HMCEnsemble:
         return out
 
     # ------------------------------------------------------------------
-    def _amplitudes(self, n, rh):
-        """A, dA/dn, dA/drh [slot, source, MAXRADII] of the log-normal model."""
-        torch = _torch()
-        sig = 0.6 + 0.25 * n
-        mu = torch.log(rh) - 0.15 * (n - 1.0)
-        z = (self.lnr[:, None, :] - mu[..., None]) / sig[..., None]
-        z2 = torch.where(self.rmask[:, None, :], z * z, torch.full_like(z, float("inf")))
-        w = torch.exp(-0.5 * (z2 - z2.min(dim=-1, keepdim=True).values))
-        A = w / w.sum(dim=-1, keepdim=True)
-        zz = torch.where(self.rmask[:, None, :], z, torch.zeros_like(z))
-        dl_drh = zz / (sig * rh)[..., None]
-        dl_dn = zz / sig[..., None] * (-0.15) + zz * zz * 0.25 / sig[..., None]
-        dA_dn = A * (dl_dn - (A * dl_dn).sum(-1, keepdim=True))
-        dA_drh = A * (dl_drh - (A * dl_drh).sum(-1, keepdim=True))
-        return A, dA_dn, dA_drh
-
     def _potential_and_grad(self):
         """U(z) and dU/dz for the current z into self.U / self.g (device)."""
         torch = _torch()
-        S, NB = self.S, self.NB
         sz = torch_sigmoid(self.z)
         rng = self.hi - self.lo
         theta = self.lo + sz * rng
         jac = rng * sz * (1.0 - sz)
-        # Source records
-        props = self.props
-        props[:, :S, _F_FLUX:_F_FLUX + NB] = (theta[:, :, :NB] * self.fmask[:, :, :NB]).float()
-        props[:, :S, _F_SHAPE:_F_SHAPE + 6] = theta[:, :, NB:].float()
-        A, dA_dn, dA_drh = self._amplitudes(theta[:, :, NB + 4], theta[:, :, NB + 5])
-        props[:, :S, _F_AMP:_F_AMP + MAXRADII] = A.float()
-        props[:, :S, _F_DRH:_F_DRH + MAXRADII] = dA_drh.float()
-        props[:, :S, _F_DN:_F_DN + MAXRADII] = dA_dn.float()
+        self.records.write(theta)
         self.batch.launch()
-        # stack_grad on the device
-        d = self.resp[:, :, :S, :].double()
-        g_theta = torch.cat([d[:, :NB, :, 0].permute(0, 2, 1), d[:, :, :, 1:].sum(dim=1)], dim=2)
+        g_theta = stack_grad_device(self.resp, self.S, self.NB)
         chi2 = self.chi2.sum(dim=1)
         lndet = (torch.log(jac) * self.fmask).sum(dim=(1, 2))
         self.U.copy_(0.5 * chi2 - lndet)

@@ -120,3 +120,35 @@ def test_leapfrog_trajectory_conserves_energy():
     assert abs(H1 - H0) < 0.05 * max(1.0, abs(np.sum(p0 ** 2)))
     q2, p2, U2, _ = leapfrog(q1, -p1, U_and_grad, eps=0.02, n_steps=32)
     np.testing.assert_allclose(q2, q0, atol=1e-3)
+
+
+def test_device_records_match_host_proposal():
+    """The device-built Source records (DeviceRecords) equal Galaxy.proposal's
+    float32 fields, and autograd through them matches the host-record path."""
+    import torch
+    from forcepho_amd.kernel_limits import MAXSOURCES
+    from forcepho_amd.sources import source_struct_dtype
+    model, theta = _model(side=32, n_bands=3, n_sources=4)
+    assert model.records is not None
+    th = _offset(theta, 3)
+    host = model.proposal(th)
+    dev = model.device_proposal(torch.tensor(th, dtype=torch.float64))
+    rec = dev.cpu().numpy().view(source_struct_dtype)[:model.n_sources]
+    for f in ("ra", "dec", "q", "pa", "nsersic", "rh"):
+        np.testing.assert_array_equal(rec[f], host[f])
+    np.testing.assert_array_equal(rec["fluxes"], host["fluxes"])
+    for f in ("mixture_amplitudes", "damplitude_drh", "damplitude_dnsersic"):
+        np.testing.assert_allclose(rec[f], host[f], rtol=2e-6, atol=1e-7 * np.abs(host[f]).max())
+    grads = []
+    for device_path in (True, False):
+        saved = model.records
+        if not device_path:
+            model.records = None
+        t = torch.tensor(th, dtype=torch.float64, device="cuda", requires_grad=True)
+        c = model.chi2(t)
+        c.backward()
+        grads.append((c.item(), t.grad.cpu().numpy()))
+        model.records = saved
+    np.testing.assert_allclose(grads[0][0], grads[1][0], rtol=1e-9)
+    g = grads[1][1]
+    np.testing.assert_allclose(grads[0][1], g, rtol=1e-4, atol=1e-5 * np.abs(g).max())
