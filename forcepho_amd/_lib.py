@@ -43,6 +43,8 @@ _SIGNATURES = {
     "fpx_device_info": (_c_int, [_p, _p, _p]),
     "fpx_evaluate_proposal": (_c_int, [_p, _c_int, _c_int, _c_int, _p]),
     "fpx_evaluate_batch": (_c_int, [_p, _c_int, _p, _c_int, _c_int, _c_int, _c_int, _p, _p]),
+    "fpx_evaluate_tiles": (_c_int, [_p, _c_int, _p, _c_int, _c_int, _c_int, _c_int, _p, _p]),
+    "fpx_finalize": (_c_int, [_p, _c_int, _p]),
     "fpx_count_pairs": (_c_int, [_p, _c_int, _p, _c_int, _c_int, _p, _p, _p]),
     "fpx_resident_groups": (_c_int, [_c_int, _c_int, _p]),
     "fpx_probe_valu": (_c_int, [_c_int, _c_int, _p, _c_int, _p, _p]),

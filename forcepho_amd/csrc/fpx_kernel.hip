@@ -850,9 +850,10 @@ int launch_lds(DeviceState *st, int cfg, int max_gauss, size_t *lds, int *cap, i
   return FPX_OK;
 }
 
+// Launch the tile kernel (and, with finalize, the output pass after it).
 int launch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
            int32_t n_items, int32_t max_gauss, int32_t cfg, int32_t persistent,
-           uint32_t *queue, void *stream) {
+           uint32_t *queue, void *stream, bool finalize_outputs = true) {
   int rc = check_cfg(cfg);
   if (rc) return rc;
   if (!slots || n_slots <= 0 || n_items < 0) return set_err(FPX_E_ARG, "empty or invalid slot list");
@@ -866,7 +867,8 @@ int launch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
   void *fargs[] = {(void *)&slots, (void *)&n_slots};
   const void *finalize = reinterpret_cast<const void *>(&fpx::finalize_kernel);
   if (n_items == 0) {  // no pixels: chi2 and gradients are zero
-    FPX_HIP(hipLaunchKernel(finalize, dim3(n_slots), dim3(256), fargs, 0, (hipStream_t)stream));
+    if (finalize_outputs)
+      FPX_HIP(hipLaunchKernel(finalize, dim3(n_slots), dim3(256), fargs, 0, (hipStream_t)stream));
     return FPX_OK;
   }
   const KernelCfg &kc = kConfigs[cfg];
@@ -893,7 +895,8 @@ int launch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
                   (void *)&n_items,   (void *)&cap,        (void *)&index_cap,
                   (void *)&persistent, (void *)&queue};
   FPX_HIP(hipLaunchKernel(kc.fn, dim3(grid), dim3(threads), args, lds, (hipStream_t)stream));
-  FPX_HIP(hipLaunchKernel(finalize, dim3(n_slots), dim3(256), fargs, 0, (hipStream_t)stream));
+  if (finalize_outputs)
+    FPX_HIP(hipLaunchKernel(finalize, dim3(n_slots), dim3(256), fargs, 0, (hipStream_t)stream));
   return FPX_OK;
 }
 
@@ -1023,6 +1026,24 @@ int fpx_evaluate_batch(const FpxSlot *slots, int32_t n_slots, const int32_t *ite
                        uint32_t *queue, void *stream) {
   return launch(slots, n_slots, item_offsets, n_items, max_gauss, cfg, persistent ? 1 : 0,
                 queue, stream);
+}
+
+int fpx_evaluate_tiles(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
+                       int32_t n_items, int32_t max_gauss, int32_t cfg, int32_t persistent,
+                       uint32_t *queue, void *stream) {
+  return launch(slots, n_slots, item_offsets, n_items, max_gauss, cfg, persistent ? 1 : 0,
+                queue, stream, false);
+}
+
+int fpx_finalize(const FpxSlot *slots, int32_t n_slots, void *stream) {
+  if (!slots || n_slots <= 0) return set_err(FPX_E_ARG, "empty or invalid slot list");
+  DeviceState *st = nullptr;
+  int rc = current_device_state(&st);
+  if (rc) return rc;
+  void *fargs[] = {(void *)&slots, (void *)&n_slots};
+  FPX_HIP(hipLaunchKernel(reinterpret_cast<const void *>(&fpx::finalize_kernel), dim3(n_slots),
+                          dim3(256), fargs, 0, (hipStream_t)stream));
+  return FPX_OK;
 }
 
 int fpx_count_pairs(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,

@@ -173,13 +173,20 @@ class HIPProposer(ProposerBase):
             self._slot_key = key
         return n_tiles
 
-    def launch(self, proposal, patch, pixel_weight=None):
+    def launch(self, proposal, patch, pixel_weight=None, outputs=True):
         """Queue one evaluation on the current stream (no synchronisation).
         ``proposal`` is a numpy source_struct_dtype array or a device uint8
-        tensor already holding the records."""
+        tensor already holding the records.  ``outputs=False`` launches only
+        the tile kernel (fpx_evaluate_tiles): its sums stay in the
+        accumulator until a launch with outputs (or ``finalize``)."""
         n_tiles = self._load(proposal, patch, pixel_weight)
         stream = self._stream()
-        if self.persistent:
+        if not outputs:
+            check(lib().fpx_evaluate_tiles(self._slot.data_ptr(), 1, None, n_tiles,
+                                           patch.max_gauss(), self.config, int(self.persistent),
+                                           self._queue.data_ptr(), stream.cuda_stream),
+                  "fpx_evaluate_tiles")
+        elif self.persistent:
             check(lib().fpx_evaluate_batch(self._slot.data_ptr(), 1, None, n_tiles,
                                            patch.max_gauss(), self.config, 1,
                                            self._queue.data_ptr(), stream.cuda_stream),
@@ -189,6 +196,12 @@ class HIPProposer(ProposerBase):
                                               self.config, stream.cuda_stream),
                   "fpx_evaluate_proposal")
         return n_tiles
+
+    def finalize(self):
+        """Write the outputs of the accumulated tile launches and zero the
+        accumulator (fpx_finalize)."""
+        check(lib().fpx_finalize(self._slot.data_ptr(), 1, self._stream().cuda_stream),
+              "fpx_finalize")
 
     def count_pairs(self, proposal, patch=None, per_tile=False):
         """Number of (pixel, ImageGaussian) pairs of the patch inside the

@@ -228,6 +228,18 @@ int fpx_evaluate_batch(const FpxSlot *slots, int32_t n_slots,
                        int32_t max_gauss, int32_t cfg, int32_t persistent,
                        uint32_t *queue, void *stream);
 
+/* The two halves of fpx_evaluate_batch, for callers that time or overlap
+ * them (no counterpart in the reference, whose single kernel does both):
+ * fpx_evaluate_tiles launches only the tile kernel, which adds every tile's
+ * chi2 and gradient sums into the slots' FP64 accumulators; fpx_finalize
+ * writes pchi2 / Response of every slot from its accumulator and zeroes it.
+ * Launches of fpx_evaluate_tiles before one fpx_finalize accumulate. */
+int fpx_evaluate_tiles(const FpxSlot *slots, int32_t n_slots,
+                       const int32_t *item_offsets, int32_t n_items,
+                       int32_t max_gauss, int32_t cfg, int32_t persistent,
+                       uint32_t *queue, void *stream);
+int fpx_finalize(const FpxSlot *slots, int32_t n_slots, void *stream);
+
 /* Measurement helper with no counterpart in the reference: counts, per
  * slot, the (pixel, ImageGaussian) pairs inside the cutoff (dx^T F dx <=
  * MAX_EXP_ARG, the test of common_kernel.cc:73,108) for the slots' current

@@ -187,3 +187,27 @@ def test_launch_without_tiles_writes_zeros():
     torch.cuda.synchronize()
     assert np.all(prop._chi2[:int(patch.n_bands)].cpu().numpy() == 0)
     assert np.all(prop._resp.cpu().numpy() == 0)
+
+
+@pytest.mark.parametrize("persistent", [True, False])
+def test_tiles_then_finalize(persistent):
+    """fpx_evaluate_tiles accumulates, fpx_finalize writes and clears: two tile
+    launches then one output pass give twice one evaluation, and the next
+    full evaluation is back to one."""
+    import torch
+    from forcepho_amd.proposal import HIPProposer
+    patch, proposal, expected = U.load_golden("multi_small_s2", return_residual=False)
+    patch.send_to_device()
+    prop = HIPProposer(patch=patch, persistent=persistent)
+    chi2, d = prop.evaluate_proposal(proposal, patch=patch)
+    prop.launch(proposal, patch, outputs=False)
+    prop.launch(proposal, patch, outputs=False)
+    prop.finalize()
+    torch.cuda.synchronize()
+    nb, ns = int(patch.n_bands), int(patch.n_sources)
+    chi2_2 = prop._chi2[:nb].cpu().numpy()
+    d_2 = prop._resp.view(-1, 30, 7)[:nb, :ns].cpu().numpy()
+    np.testing.assert_allclose(chi2_2, 2 * chi2, rtol=1e-12)
+    np.testing.assert_allclose(d_2, 2 * d, rtol=1e-5, atol=1e-6 * np.abs(d).max())
+    again, _ = prop.evaluate_proposal(proposal, patch=patch)
+    np.testing.assert_allclose(again, chi2, rtol=1e-12)
