@@ -64,13 +64,17 @@ class PatchModel:
             buf = torch.zeros(MAXSOURCES * 264, dtype=torch.uint8, device=dev)
             self.records = DeviceRecords([scene], [self.n_bands], buf, self.n_sources,
                                          self.n_bands)
+            # fixed buffers that the launches read in place (their device
+            # slots are then built once)
+            self.backward_records = torch.zeros_like(buf)
 
-    def device_proposal(self, theta):
+    def device_proposal(self, theta, copy=True):
         """Source records of theta (any device, float64 [ndim]) as a uint8
-        device tensor (a copy, owned by the caller)."""
+        device tensor: a copy owned by the caller, or (copy=False) the
+        builder's own buffer, overwritten by the next call."""
         t = theta.detach().to(self.records.records.device, dtype=self.records.lnr.dtype)
         self.records.write(t.reshape(1, self.n_sources, self.n_bands + 6))
-        return self.records.records.clone()
+        return self.records.records.clone() if copy else self.records.records
 
     def proposal(self, theta):
         self.scene.set_all_source_params(np.asarray(theta, dtype=np.float64))
@@ -122,10 +126,12 @@ def _make_function():
         @staticmethod
         def forward(ctx, theta, model):
             if model.records is not None:
-                proposal = model.device_proposal(theta)
+                records = model.device_proposal(theta, copy=False)
+                resid = model.forward_residual(records, outputs=False)
+                proposal = records.clone()          # kept for backward
             else:
                 proposal = model.proposal(theta.detach().cpu().numpy())
-            resid = model.forward_residual(proposal, outputs=False)
+                resid = model.forward_residual(proposal, outputs=False)
             ctx.model = model
             ctx.proposal = proposal
             ctx.theta_meta = (theta.dtype, theta.device)
@@ -138,7 +144,8 @@ def _make_function():
             # residual = data - model, so dL/dtheta = -sum_pix g * dmodel/dtheta
             pw = (-grad_resid).to(torch.float32).contiguous()
             if model.records is not None:
-                d = model.proposer.vjp_device(ctx.proposal, pw, model.patch)
+                model.backward_records.copy_(ctx.proposal)
+                d = model.proposer.vjp_device(model.backward_records, pw, model.patch)
                 g = stack_grad_device(d[None], model.n_sources, model.n_bands)[0]
                 return g.reshape(-1).to(dtype=dtype, device=device), None
             d = model.proposer.vjp(ctx.proposal, pw, model.patch)

@@ -145,11 +145,14 @@ class HIPProposer(ProposerBase):
         import torch
         return torch.cuda.current_stream(self.device)
 
-    def _slot_for(self, patch, pixel_weight=None):
-        """(Re)build the device FpxSlot when the patch struct or tiles change."""
+    def _slot_for(self, patch, pixel_weight=None, prop_ptr=None):
+        """(Re)build the device FpxSlot when the patch struct, tiles, weights
+        or proposal buffer change."""
         tiles, n_tiles = patch.device_tiles(self.pix_per_tile)
         pw = 0 if pixel_weight is None else pixel_weight.data_ptr()
-        key = (patch.device_patch.data_ptr(), tiles.data_ptr(), n_tiles, int(patch.n_bands), pw)
+        prop_ptr = self._prop.data_ptr() if prop_ptr is None else prop_ptr
+        key = (patch.device_patch.data_ptr(), tiles.data_ptr(), n_tiles, int(patch.n_bands), pw,
+               prop_ptr)
         if key != self._slot_key and key in self._slots:
             self._slot, self._slot_key = self._slots[key], key
         elif key != self._slot_key:
@@ -160,7 +163,7 @@ class HIPProposer(ProposerBase):
             self._slots[key] = self._slot
             slot = np.zeros(1, dtype=SLOT_DTYPE)
             slot["patch"] = patch.device_patch.data_ptr()
-            slot["proposal"] = self._prop.data_ptr()
+            slot["proposal"] = prop_ptr
             slot["tiles"] = tiles.data_ptr()
             slot["pixel_weight"] = pw
             slot["pchi2"] = self._chi2.data_ptr()
@@ -269,7 +272,10 @@ class HIPProposer(ProposerBase):
         return perm
 
     def _load(self, proposal, patch, pixel_weight):
-        """Stage the proposal and the slot for ``patch``; returns the tile count."""
+        """Stage the proposal and the slot for ``patch``; returns the tile
+        count.  A numpy proposal is copied through pinned memory into the
+        proposer's buffer; a device tensor is read in place by the launch (it
+        must stay alive and unchanged until the launch has run)."""
         import torch
         n_src = int(patch.n_sources)
         if n_src > MAXSOURCES or int(patch.n_bands) > MAXBANDS:
@@ -284,9 +290,12 @@ class HIPProposer(ProposerBase):
             self._prop[:raw.size].copy_(self._prop_h[:raw.size], non_blocking=True)
             self._h2d_done = torch.cuda.Event()
             self._h2d_done.record(self._stream())
-        else:
-            self._prop[:proposal.numel()].copy_(proposal, non_blocking=True)
-        return self._slot_for(patch, pixel_weight)
+            return self._slot_for(patch, pixel_weight)
+        if proposal.device != self.device or not proposal.is_contiguous():
+            raise ValueError("device proposals must be contiguous tensors on the proposer's device")
+        if proposal.numel() * proposal.element_size() < n_src * source_struct_dtype.itemsize:
+            raise ValueError("device proposal holds fewer records than the patch has sources")
+        return self._slot_for(patch, pixel_weight, prop_ptr=proposal.data_ptr())
 
     def evaluate_proposal(self, proposal, patch=None, verbose=False, unpack=True):
         """Evaluate chi2 and dchi2/dparam of ``proposal`` on ``patch``."""
