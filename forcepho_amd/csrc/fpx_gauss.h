@@ -118,6 +118,32 @@ __device__ __forceinline__ void source_geometry(const FpxPatch *__restrict__ P,
   g.ys = CWp[2] * sx + CWp[3] * sy + crpix[1];
 }
 
+// A SourceGeometry kept in kGeoFloats floats (LDS): T, dT/dq, dT/dpa, centre.
+constexpr int kGeoFloats = 16;
+
+__device__ __forceinline__ void store_geometry(float *__restrict__ dst, const SourceGeometry &g) {
+  const M22 *m[3] = {&g.T, &g.dTq, &g.dTp};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    dst[4 * k + 0] = m[k]->a;
+    dst[4 * k + 1] = m[k]->b;
+    dst[4 * k + 2] = m[k]->c;
+    dst[4 * k + 3] = m[k]->d;
+  }
+  dst[12] = g.xs;
+  dst[13] = g.ys;
+}
+
+__device__ __forceinline__ SourceGeometry load_geometry(const float *__restrict__ src) {
+  SourceGeometry g;
+  g.T = {src[0], src[1], src[2], src[3]};
+  g.dTq = {src[4], src[5], src[6], src[7]};
+  g.dTp = {src[8], src[9], src[10], src[11]};
+  g.xs = src[12];
+  g.ys = src[13];
+  return g;
+}
+
 // Sigma = covar T T^T + C_psf
 __device__ __forceinline__ void convolved_sigma(const M22 &T, float covar,
                                                 const FpxPSFSourceGaussian &psf, float &Sxx,
@@ -157,17 +183,14 @@ __device__ __forceinline__ float cutoff_halfwidth(float S) {
   return sqrtf((float)MAX_EXP_ARG) * sqrtf(fmaxf(S, 0.0f)) * 1.001f + 0.01f;
 }
 
-// True when the MAX_EXP_ARG box of Gaussian (s, p) of exposure e can reach
-// a pixel of [x0, x1] x [y0, y1].
-__device__ __forceinline__ bool gauss_reaches(const FpxPatch *__restrict__ P,
-                                              const FpxSource *__restrict__ props, int e,
-                                              int s, int p, float x0, float x1, float y0,
-                                              float y1) {
-  SourceGeometry g;
-  source_geometry<false>(P, props + s, e, s, g);
-  const FpxPSFSourceGaussian psf = P->psfgauss[P->psfgauss_start[e] + p];
+// True when the MAX_EXP_ARG box of the convolution of the source (geometry
+// g) with PSF Gaussian psf (radius bin variance covar) can reach a pixel of
+// [x0, x1] x [y0, y1].
+__device__ __forceinline__ bool gauss_reaches(const SourceGeometry &g,
+                                              const FpxPSFSourceGaussian &psf, float covar,
+                                              float x0, float x1, float y0, float y1) {
   float Sxx, Sxy, Syy;
-  convolved_sigma(g.T, P->rad2[psf.sersic_radius_bin], psf, Sxx, Sxy, Syy);
+  convolved_sigma(g.T, covar, psf, Sxx, Sxy, Syy);
   const float xc = g.xs + psf.xcen;
   const float yc = g.ys + psf.ycen;
   const float gx = fmaxf(fmaxf(x0 - xc, xc - x1), 0.0f);
@@ -175,22 +198,19 @@ __device__ __forceinline__ bool gauss_reaches(const FpxPatch *__restrict__ P,
   return gx <= cutoff_halfwidth(Sxx) && gy <= cutoff_halfwidth(Syy);
 }
 
-// Build the record of source s, PSF component p (0 <= p < n_psf_per_source[band])
-// for exposure e of band `band`.  PSF lookup ignores the source index, as
-// the reference does (compute_gaussians_kernel.cc:100, .cu:77).  The pad
-// slot holds the source index (as int bits).
-__device__ __forceinline__ void build_record(const FpxPatch *__restrict__ P,
-                                             const FpxSource *__restrict__ props,
-                                             int e, int band, int s, int p,
+// Build the record of source s (geometry g for the exposure, with dT/dq and
+// dT/dpa) convolved with PSF Gaussian psf of the exposure (radius bin
+// variance covar, exposure calibration gcal) in band `band`.  The caller
+// takes psf from the exposure's PSF block whatever the source, as the
+// reference does (compute_gaussians_kernel.cc:100, .cu:77).  kSrc holds the
+// source index (as int bits).
+__device__ __forceinline__ void build_record(const SourceGeometry &g,
+                                             const FpxSource *__restrict__ src,
+                                             const FpxPSFSourceGaussian &psf, float covar,
+                                             float gcal, int band, int s,
                                              float *__restrict__ out) {
-  const FpxSource *src = props + s;
-  SourceGeometry g;
-  source_geometry<true>(P, src, e, s, g);
   const M22 &T = g.T, &dTq = g.dTq, &dTp = g.dTp;
-  const float gcal = P->G[e];
-  const FpxPSFSourceGaussian psf = P->psfgauss[P->psfgauss_start[e] + p];
   const int rb = psf.sersic_radius_bin;
-  const float covar = P->rad2[rb];
   const float amp_s = src->mixture_amplitudes[rb];
   const float da_dn = src->damplitude_dnsersic[rb];
   const float da_dr = src->damplitude_drh[rb];

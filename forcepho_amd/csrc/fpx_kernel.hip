@@ -114,7 +114,10 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask) {
 //   records  gauss_cap x kRec floats   ImageGaussians of the current chunk
 //                                       of the tile's surviving Gaussians
 //   part     W x 64*PPL floats         per-wave partial models
+//   geo      MAXSOURCES x kGeoFloats   per-source geometry of the exposure
+//                                       (T, dT/dq, dT/dpa, centre)
 //   cwl      MAXSOURCES x 4 floats     CW of every source for the exposure
+//   rad2     kRad2 floats              the patch's radius-bin variances
 //   rows     W x kRowBytes             per-wave accumulators: chi2 (double)
 //                                       then MAXSOURCES*NPARAMS gradient sums
 //   index    index_cap + 1 ints        cull flags, then the survivors'
@@ -123,17 +126,22 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask) {
 // ------------------------------------------------------------------------
 constexpr int kRowBytes = 8 + MAXSOURCES * NPARAMS * 4;
 
+constexpr int kRad2 = 16;  // >= MAXRADII
+constexpr int kExpoFloats = MAXSOURCES * (kGeoFloats + 4) + kRad2;
+
 __host__ __device__ constexpr size_t lds_bytes_for(int waves, int ppl, int gauss_cap,
                                                    int index_cap) {
   return (size_t)gauss_cap * kRec * sizeof(float) + (size_t)waves * ppl * 64 * sizeof(float) +
-         (size_t)MAXSOURCES * 4 * sizeof(float) + (size_t)waves * kRowBytes +
+         (size_t)kExpoFloats * sizeof(float) + (size_t)waves * kRowBytes +
          (size_t)(index_cap + 1) * sizeof(int);
 }
 
 struct Lds {
   float *recs;
   float *part;
+  float *geo;
   float *cwl;
+  float *rad2;
   unsigned char *rows;
   int *index;
 };
@@ -142,8 +150,10 @@ __device__ __forceinline__ Lds carve(unsigned char *smem, int waves, int ppl, in
   Lds l;
   l.recs = reinterpret_cast<float *>(smem);
   l.part = l.recs + (size_t)gauss_cap * kRec;
-  l.cwl = l.part + waves * 64 * ppl;
-  l.rows = reinterpret_cast<unsigned char *>(l.cwl + MAXSOURCES * 4);
+  l.geo = l.part + waves * 64 * ppl;
+  l.cwl = l.geo + MAXSOURCES * kGeoFloats;
+  l.rad2 = l.cwl + MAXSOURCES * 4;
+  l.rows = reinterpret_cast<unsigned char *>(l.rad2 + kRad2);
   l.index = reinterpret_cast<int *>(l.rows + waves * kRowBytes);
   return l;
 }
@@ -165,17 +175,23 @@ struct Box {
 // phase 1 flags every Gaussian (one lane each), phase 2 compacts the flags
 // into the list of survivors in (source, component) order with one
 // wavefront (ballot + prefix count).  Returns the number of survivors.
+// Radius-bin variance of a PSF entry from the LDS copy (bins outside the
+// table, which valid input never has, read its last entry).
+__device__ __forceinline__ float bin_covar(const Lds &l, const FpxPSFSourceGaussian &psf) {
+  return l.rad2[min(max(psf.sersic_radius_bin, 0), MAXRADII - 1)];
+}
+
 template <int W>
-__device__ int cull_tile(const FpxPatch *__restrict__ P, const FpxSource *__restrict__ props,
-                         int e, int n_src, int n_psf, const Box &box, const Lds &l, int wave,
-                         int lane) {
+__device__ int cull_tile(const FpxPatch *__restrict__ P, int pstart, int n_src, int n_psf,
+                         const Box &box, const Lds &l, int wave, int lane) {
   const int n = n_src * n_psf;
   int *index = l.index;
-  __syncthreads();  // previous users of the index are done
+  __syncthreads();  // previous users of the index are done; geometry is in LDS
   for (int lg = threadIdx.x; lg < n; lg += W * 64) {
     const int s = lg / n_psf;
-    index[lg] = gauss_reaches(P, props, e, s, lg - s * n_psf, box.x0, box.x1, box.y0, box.y1)
-                    ? 1 : 0;
+    const FpxPSFSourceGaussian psf = P->psfgauss[pstart + lg - s * n_psf];
+    index[lg] = gauss_reaches(load_geometry(l.geo + s * kGeoFloats), psf, bin_covar(l, psf),
+                              box.x0, box.x1, box.y0, box.y1) ? 1 : 0;
   }
   __syncthreads();
   if (wave == 0) {
@@ -197,13 +213,15 @@ __device__ int cull_tile(const FpxPatch *__restrict__ P, const FpxSource *__rest
 // Build the records of survivors [first, first + count) into LDS.
 template <int W>
 __device__ void build_records(const FpxPatch *__restrict__ P,
-                              const FpxSource *__restrict__ props, int e, int band, int n_psf,
-                              int first, int count, const Lds &l) {
+                              const FpxSource *__restrict__ props, int pstart, float gcal,
+                              int band, int n_psf, int first, int count, const Lds &l) {
   __syncthreads();  // previous users of the records are done
   for (int i = threadIdx.x; i < count; i += W * 64) {
     const int lg = l.index[first + i];
     const int s = lg / n_psf;
-    build_record(P, props, e, band, s, lg - s * n_psf, l.recs + i * kRec);
+    const FpxPSFSourceGaussian psf = P->psfgauss[pstart + lg - s * n_psf];
+    build_record(load_geometry(l.geo + s * kGeoFloats), props + s, psf, bin_covar(l, psf), gcal,
+                 band, s, l.recs + i * kRec);
   }
   __syncthreads();
 }
@@ -398,10 +416,19 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
   const bool vjp = slot.pixel_weight != nullptr;
 
   for (int j = tid; j < W * kRowBytes / 4; j += W * 64) reinterpret_cast<float *>(l.rows)[j] = 0.0f;
-  for (int j = tid; j < 4 * n_src; j += W * 64) {
-    const int s = j >> 2;
-    l.cwl[j] = P->CW[4 * (n_src * e + s) + (j & 3)];
+  // per-source geometry of the exposure, its CW and the radius-bin variances,
+  // once per tile (the cull and the records read them from LDS)
+  for (int j = tid; j < n_src; j += W * 64) {
+    SourceGeometry g;
+    source_geometry<true>(P, props + j, e, j, g);
+    store_geometry(l.geo + j * kGeoFloats, g);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) l.cwl[4 * j + c] = P->CW[4 * (n_src * e + j) + c];
   }
+  const int n_radii = min(__builtin_amdgcn_readfirstlane(P->n_radii), MAXRADII);
+  for (int j = tid; j < n_radii; j += W * 64) l.rad2[j] = P->rad2[j];
+  const int pstart = __builtin_amdgcn_readfirstlane(P->psfgauss_start[e]);
+  const float gcal = uniform_f(P->G[e]);
 
   float xp[PPL], yp[PPL], w2[PPL];
   int pix[PPL];
@@ -433,7 +460,7 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
     box.y1 = uniform_f(wave_max(y1));
   }
 
-  const int n_surv = cull_tile<W>(P, props, e, n_src, n_psf, box, l, wave, lane);
+  const int n_surv = cull_tile<W>(P, pstart, n_src, n_psf, box, l, wave, lane);
   const int n_chunks = (n_surv + gauss_cap - 1) / gauss_cap;
   auto chunk_size = [&](int c) { return min(gauss_cap, n_surv - c * gauss_cap); };
   if (!vjp) {
@@ -442,7 +469,7 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
     for (int i = 0; i < PPL; ++i) m[i] = 0.0f;
     for (int c = 0; c < n_chunks; ++c) {
       const int n_act = chunk_size(c);
-      build_records<W>(P, props, e, band, n_psf, c * gauss_cap, n_act, l);
+      build_records<W>(P, props, pstart, gcal, band, n_psf, c * gauss_cap, n_act, l);
       model_pass<PPL>(l.recs, wave, W, n_act, xp, yp, m);
     }
 #pragma unroll
@@ -483,7 +510,7 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
   for (int c = n_chunks - 1; c >= 0; --c) {
     const int n_act = chunk_size(c);
     if (c != n_chunks - 1 || vjp)
-      build_records<W>(P, props, e, band, n_psf, c * gauss_cap, n_act, l);
+      build_records<W>(P, props, pstart, gcal, band, n_psf, c * gauss_cap, n_act, l);
     if (wave_has_weight)
       grad_pass<PPL>(l.recs, wave, W, n_act, xp, yp, w2, l.cwl, wrow, lane);
   }
