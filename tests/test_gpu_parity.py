@@ -211,3 +211,26 @@ def test_tiles_then_finalize(persistent):
     np.testing.assert_allclose(d_2, 2 * d, rtol=1e-5, atol=1e-6 * np.abs(d).max())
     again, _ = prop.evaluate_proposal(proposal, patch=patch)
     np.testing.assert_allclose(again, chi2, rtol=1e-12)
+
+
+def test_full_size_multi_band_against_fp64():
+    """The 8-band x 2-exposure x 128x128, 15-source benchmark patch at full
+    size (1.4e8 pairs): chi2 and gradients against the float64 oracle's
+    values committed in tests/golden/fullsize (tests/golden/make_fullsize.py),
+    with the same tolerance band as the fixtures."""
+    import sys
+    from forcepho_amd.proposal import HIPProposer
+    sys.path.insert(0, U.GOLDEN_DIR)
+    import make_fullsize as F
+    print("[gpu] full-size multi-band", flush=True)
+    ref = np.load(F.OUT)
+    patch, proposal = F.build()
+    np.testing.assert_allclose(F.data_digest(patch), ref["digest"], rtol=1e-12)
+    patch.send_to_device()
+    prop = HIPProposer(patch=patch)
+    prop.autotune(proposal, patch, repeats=1, configs=[2, 5])
+    chi2, d = prop.evaluate_proposal(proposal, patch=patch)
+    assert np.max(np.abs(chi2 - ref["chi2"]) / np.abs(ref["chi2"])) <= U.CHI2_RTOL
+    err = U.grad_errors(d, ref["dchi2"], ref["mass"], ref["smass"])
+    print("full-size gradient error / tolerance: max", err.max(), "median", np.median(err))
+    assert err.max() <= 1.0

@@ -223,3 +223,18 @@ def test_source_permutation_equivariance():
     c1, g1, _ = fp64.evaluate(pp, proposal[perm])
     np.testing.assert_allclose(c1, c0, rtol=1e-12)
     np.testing.assert_allclose(g1, g0[:, perm], rtol=1e-9, atol=1e-12 * np.abs(g0).max())
+
+
+def test_reference_cpu_kernel_inside_band_at_full_size():
+    """At the full multi-band benchmark size the C restatement of the
+    reference CPU kernel (bit-identical to it) stays inside the tolerance
+    band around the committed float64 values, as on the small fixtures."""
+    import sys
+    sys.path.insert(0, U.GOLDEN_DIR)
+    import make_fullsize as F
+    ref = np.load(F.OUT)
+    patch, proposal = F.build()
+    np.testing.assert_allclose(F.data_digest(patch), ref["digest"], rtol=1e-12)
+    c32, d32, _ = oracle.evaluate_cpu(patch, proposal)
+    assert np.max(np.abs(c32 - ref["chi2"]) / np.abs(ref["chi2"])) <= 1e-6
+    assert U.grad_errors(d32, ref["dchi2"], ref["mass"], ref["smass"]).max() <= 1.0
