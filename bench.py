@@ -114,6 +114,20 @@ def roofline(n_pairs, n_inside, seconds):
             "frac_if_117_per_pair": n_pairs * FLOP_PER_EVAL / seconds / PEAK_FP32}
 
 
+def roofline_line(base, traffic_bytes, seconds, n_inside, probe):
+    """The roofline object plus the exponential throughput (two v_exp_f32 per
+    pair inside the cutoff) against the probe's measured rate, and the HBM
+    bandwidth the committed traffic figure implies."""
+    line = dict(base, traffic=traffic_bytes)
+    line["exp_per_s"] = 2.0 * n_inside / seconds
+    if probe is not None:
+        line["exp_frac_of_measured_peak"] = line["exp_per_s"] / probe["exp_per_s"]
+    if traffic_bytes is not None:
+        line["hbm_gb_per_s"] = traffic_bytes / seconds / 1e9
+        line["hbm_frac"] = line["hbm_gb_per_s"] / 8000.0
+    return line
+
+
 def traffic(cfg):
     """HBM bytes per launch of the multi-band workload for kernel configuration
     cfg, from the committed FETCH_SIZE / WRITE_SIZE summary (None if that
@@ -476,7 +490,8 @@ def main():
         "config": {"workload": desc, "evals_per_step": n_evals, "path": "plain",
                    "kernel_config": best, "config_ms": {str(k): v * 1e3 for k, v in timings.items()},
                    "wall_s": wall},
-        "roofline": dict(roofline(n_evals, n_inside, kernel_s), traffic=traffic(prop.config)),
+        "roofline": roofline_line(roofline(n_evals, n_inside, kernel_s), traffic(prop.config),
+                                  kernel_s, n_inside, probe),
         "single_plain": single,
         "batched": batched,
         "autograd": autograd,

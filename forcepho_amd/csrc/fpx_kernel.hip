@@ -823,15 +823,16 @@ int check_cfg(int32_t cfg) {
 
 // Records per LDS chunk: all max_gauss if they fit the workgroup's budget
 // (kLdsPerWave per wavefront, less the kernel's static LDS), else what fits,
-// but at least 64 when the device's per-workgroup limit allows it.  0 if not
-// even one fits.
+// but at least kMinChunk when the device's per-workgroup limit allows it.  0
+// if not even one fits.
+constexpr int kMinChunk = 16;
 int pick_gauss_cap(const KernelCfg &kc, int max_gauss, int lds_max, int static_lds) {
   const long fixed = (long)fpx::lds_bytes_for(kc.waves, kc.ppl, 0, max_gauss);
   const long per = (long)(fpx::kRec * sizeof(float));
   lds_max -= static_lds;
   const long budget = std::min<long>(lds_max, (long)kc.waves * kLdsPerWave - static_lds);
   long fit = (budget - fixed) / per;
-  if (fit < std::min<long>(max_gauss, 64)) fit = ((long)lds_max - fixed) / per;
+  if (fit < std::min<long>(max_gauss, kMinChunk)) fit = ((long)lds_max - fixed) / per;
   return (int)std::max<long>(0, std::min<long>(max_gauss, fit));
 }
 

@@ -120,7 +120,7 @@ def test_configs_and_lds():
         # a workgroup stays within a CU's 160 KiB; few Gaussians need less
         assert 27 * 36 * 4 < small.value <= size.value <= 160 * 1024
         # the budget is 10 KiB per wavefront unless the records must spill over it
-        assert size.value <= max(10 * 1024 * w.value, small.value + 64 * 36 * 4)
+        assert size.value <= max(10 * 1024 * w.value, small.value + 16 * 36 * 4)
     assert L.fpx_config_info(n, None, None, None) < 0
     assert L.fpx_accum_doubles(8) == 8 * (1 + 30 * 7)
 
