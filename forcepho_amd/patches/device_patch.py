@@ -39,19 +39,23 @@ def device_bytes(arr, device):
 class DevicePatchMixin:
     """Device-independent logic shared by device mixins."""
 
-    def prepare_model(self, active=None, fixed=None, bounds=None, maxactive=15,
-                      model_kwargs={}):
-        """Subtract fixed sources, load the active scene and wrap the patch in
-        a FastPosterior (forcepho/patches/device_patch.py:28-72).
+    def prepare_model(self, active=None, fixed=None, big=None, bounds=None, maxactive=15,
+                      shapes=None, big_scene_kwargs={}, model_kwargs={}):
+        """Subtract big and fixed sources, load the active scene and wrap the
+        patch in a FastPosterior (forcepho/patches/device_patch.py:28-72).
 
         Parameters
         ----------
-        active : Scene
-        fixed : Scene or None
+        active : Scene, or catalog (structured array) of active sources
+        fixed, big : Scene or catalog of sources to subtract, or None
+            (catalogs are subtracted in blocks of ``maxactive`` sources)
         bounds : (lower, upper) arrays for a BoundedTransform, or None
+        shapes : accepted for interface compatibility (the Galaxy SHAPE_COLS)
         """
         from ..model import FastPosterior, Transform
-        proposer, scene = self.subtract_fixed(fixed, active=active)
+        proposer, scene = self.subtract_fixed(fixed, active=active, big=big,
+                                              maxactive=maxactive,
+                                              big_scene_kwargs=big_scene_kwargs)
         self.return_residual = False
         self.send_patchstruct_to_device()
         q = scene.get_all_source_params().copy()
@@ -64,15 +68,27 @@ class DevicePatchMixin:
                                   **model_kwargs)
         return model, q
 
-    def subtract_fixed(self, fixed, active=None):
-        """Subtract a fixed scene from the device-side data, then load the
-        active scene's metadata (forcepho/patches/device_patch.py:113-183).
-        After this the device ``data`` holds data - model(fixed)."""
-        scenes = []
-        if fixed is not None:
-            scenes.append(fixed)
+    def _scenes(self, sources, maxactive, **scene_kwargs):
+        """Scenes of a Scene (itself) or of a catalog (blocks of maxactive)."""
+        from ..sources import Scene
+        if sources is None:
+            return []
+        if isinstance(sources, Scene):
+            return [sources]
+        cat = np.atleast_1d(sources)
+        blocks = np.array_split(cat, np.arange(maxactive, len(cat), maxactive))
+        return [self.set_scene(block, **scene_kwargs) for block in blocks]
+
+    def subtract_fixed(self, fixed, active=None, big=None, maxactive=15,
+                       big_scene_kwargs={}, **scene_kwargs):
+        """Subtract big and fixed sources from the device-side data, then load
+        the active scene's metadata (forcepho/patches/device_patch.py:113-183).
+        After this the device ``data`` holds data - model(big + fixed)."""
+        scenes = self._scenes(big, maxactive, **big_scene_kwargs)
+        scenes += self._scenes(fixed, maxactive, **scene_kwargs)
         assert active is not None, "an active scene is required"
-        scenes.append(active)
+        scenes += self._scenes(active, max(1, len(np.atleast_1d(getattr(active, "sources", active)))),
+                               **scene_kwargs)
         proposer = self.get_proposer()
         self.return_residual = True
         self.pack_meta(scenes[0])

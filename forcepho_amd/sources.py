@@ -60,8 +60,12 @@ class LogNormalMixtureAmplitudes:
 
 
 class SplineMixtureAmplitudes:
-    """Amplitudes from per-radius RectBivariateSplines over an (n, rh) grid,
-    as forcepho/sources.py:623-681 builds them."""
+    """Amplitudes from per-radius scipy splines of a fitted table, as
+    forcepho/sources.py:623-681 builds them: RectBivariateSplines over the
+    (nsersic, rh) grid (``spline_type="rect"``, the reference's default) or
+    SmoothBivariateSplines of the flat table (``"smooth"``, :640-642)."""
+
+    _file_cache = {}
 
     def __init__(self, nsersic_grid, rh_grid, amplitudes, radii, smoothing=None):
         from scipy.interpolate import RectBivariateSpline
@@ -69,6 +73,45 @@ class SplineMixtureAmplitudes:
         self.radii = np.asarray(radii, dtype=np.float64)
         self.splines = [RectBivariateSpline(nsersic_grid, rh_grid, A[:, :, i], s=smoothing)
                         for i in range(A.shape[-1])]
+        self.sersic_range = (float(np.min(nsersic_grid)), float(np.max(nsersic_grid)))
+        self.rh_range = (float(np.min(rh_grid)), float(np.max(rh_grid)))
+
+    @classmethod
+    def from_table(cls, nsersic, rh, amplitudes, radii, smoothing=None, spline_type="rect"):
+        """Splines of a flat table of rows (nsersic[k], rh[k], amplitudes[k, :]).
+        "rect": rows ordered sersic-major over the grid of unique values
+        (forcepho/sources.py:633-638); "smooth": any scattered rows (:640-642)."""
+        A = np.asarray(amplitudes)
+        if spline_type == "rect":
+            ngrid, rgrid = np.unique(nsersic), np.unique(rh)
+            if A.shape[0] != len(ngrid) * len(rgrid):
+                raise ValueError("table rows do not form a (nsersic, rh) grid")
+            A = A.reshape(len(ngrid), len(rgrid), A.shape[-1])
+            return cls(ngrid, rgrid, A, radii, smoothing=smoothing)
+        if spline_type != "smooth":
+            raise ValueError("spline_type must be 'rect' or 'smooth'")
+        from scipy.interpolate import SmoothBivariateSpline
+        obj = cls.__new__(cls)
+        obj.radii = np.asarray(radii, dtype=np.float64)
+        obj.splines = [SmoothBivariateSpline(nsersic, rh, A[:, i], s=smoothing)
+                       for i in range(A.shape[-1])]
+        obj.sersic_range = (float(np.min(nsersic)), float(np.max(nsersic)))
+        obj.rh_range = (float(np.min(rh)), float(np.max(rh)))
+        return obj
+
+    @classmethod
+    def from_file(cls, splinedata, smoothing=None, spline_type="rect"):
+        """Splines of an HDF5 table with datasets nsersic, rh, amplitudes,
+        radii (forcepho/sources.py:623-645); cached per file and options."""
+        key = (str(splinedata), smoothing, spline_type)
+        if key not in cls._file_cache:
+            from .formats.hdf5 import File
+            with File(splinedata) as f:
+                n, r = f["nsersic"][:], f["rh"][:]
+                A, radii = f["amplitudes"][:], f["radii"][:]
+            cls._file_cache[key] = cls.from_table(n, r, A, radii, smoothing=smoothing,
+                                                  spline_type=spline_type)
+        return cls._file_cache[key]
 
     def amplitudes(self, n, rh):
         return np.squeeze(np.array([s(n, rh) for s in self.splines]))
@@ -126,6 +169,17 @@ class Galaxy:
         return np.concatenate([self.flux, [self.ra, self.dec, self.q, self.pa,
                                            self.sersic, self.rhalf]])
 
+    def from_catalog_row(self, row, extra_parameters=(), filternames=None):
+        """Parameters from a catalog row with SHAPE_COLS and one column per
+        filter name (forcepho/sources.py:458-468)."""
+        for p in list(self.SHAPE_COLS) + list(extra_parameters):
+            setattr(self, p, float(row[p]))
+        if filternames is not None:
+            self.filternames = list(filternames)
+            self.flux = np.zeros(self.n_bands)
+        for j, b in enumerate(self.filternames):
+            self.flux[j] = row[b]
+
     @property
     def amplitudes(self):
         return self.amplitude_model.amplitudes(self.sersic, self.rhalf)
@@ -155,11 +209,33 @@ class Scene:
     """An ordered collection of sources and the map to a flat parameter
     vector (forcepho/sources.py:28-215)."""
 
-    def __init__(self, sources=()):
+    def __init__(self, sources=(), catalog=None, **catkwargs):
         self.sources = list(sources)
         self.sky_zero = None  # set when coordinates are shifted to patch offsets
+        if catalog is not None:
+            self.sources = Scene.from_catalog(catalog, **catkwargs).sources
         for i, s in enumerate(self.sources):
             s.id = i
+
+    @classmethod
+    def from_catalog(cls, catalog, filternames=("band",), splinedata=None,
+                     spline_smoothing=None, spline_type="rect", amplitude_model=None,
+                     extra_parameters=()):
+        """A Scene of Galaxies, one per catalog row (forcepho/sources.py:154-182).
+        Mixture amplitudes come from ``amplitude_model``, else from
+        ``splinedata`` (an HDF5 table path, or an amplitude model object),
+        else the default log-normal model."""
+        if amplitude_model is None and splinedata is not None and not isinstance(splinedata, str):
+            amplitude_model = splinedata
+        elif amplitude_model is None and splinedata:
+            amplitude_model = SplineMixtureAmplitudes.from_file(splinedata, spline_smoothing,
+                                                                spline_type)
+        sources = []
+        for row in np.atleast_1d(catalog):
+            g = Galaxy(filters=filternames, amplitude_model=amplitude_model)
+            g.from_catalog_row(row, extra_parameters=extra_parameters)
+            sources.append(g)
+        return cls(sources)
 
     def __len__(self):
         return len(self.sources)

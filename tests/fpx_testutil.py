@@ -79,3 +79,48 @@ def grad_errors(g, g64, mass, smass=None):
         tol = tol + MODEL_EPS * smass
     tol = np.where(tol > 0, tol, 1e-300)
     return np.abs(g - g64) / tol
+
+
+# ---------------------------------------------------------------------------
+# The reference's end-to-end verification case
+# (tests/verification/verify_reference.py), fixtures written by
+# tests/golden/make_verification.py.
+VERIF_DIR = os.path.join(GOLDEN_DIR, "verification")
+VERIF_IMAGE = os.path.join(VERIF_DIR, "reference_f200w_sersic2.2_rhalf0.10.fits")
+TEST_IMAGE = os.path.join(VERIF_DIR, "test_image_data.fits")
+TEST_IMAGE_PSF = os.path.join(VERIF_DIR, "test_image_psf.h5")
+
+
+def verification_amplitudes(spline_type="smooth"):
+    """Mixture amplitude model of the verification image: splines of the
+    sersic_mog_model.smooth=0.0150 table.  The image was rendered with the
+    SmoothBivariateSpline form (forcepho/sources.py:640-642)."""
+    from forcepho_amd.sources import SplineMixtureAmplitudes
+    t = np.load(os.path.join(VERIF_DIR, "sersic_mog_smooth0.0150.npz"))
+    return SplineMixtureAmplitudes.from_table(t["nsersic"], t["rh"], t["amplitudes"],
+                                              t["radii"], spline_type=spline_type)
+
+
+def verification_psfstore(n_radii=9):
+    """The F200W 4-Gaussian PSF the image header names, for every radius bin."""
+    from forcepho_amd.patches import PSFStore
+    t = np.load(os.path.join(VERIF_DIR, "psf_f200w_ng4m0.npz"))
+    mix = np.zeros(len(t["amp"]), dtype=[(k, np.float64) for k in
+                                         ("amp", "xcr", "ycr", "Cxx", "Cyy", "Cxy")])
+    for k in mix.dtype.names:
+        mix[k] = t[k]
+    return PSFStore.from_mixtures({"F200W": mix}, n_radii=n_radii)
+
+
+def verification_patch(cls, spline_type="smooth", pixel_order="superpixel", **kwargs):
+    """(patch, catalog) of the verification image packed by ``cls`` (a
+    FITSPatch subclass): whole image, pixel inverse errors 1/(dFLUX/dF200W)
+    from extension 1, as verify_reference.py:73-79 configures it."""
+    from forcepho_amd.formats import fits
+    amps = verification_amplitudes(spline_type)
+    patch = cls(fitsfiles=[VERIF_IMAGE], psfstore=verification_psfstore(len(amps.radii)),
+                splinedata=amps, pixel_order=pixel_order, **kwargs)
+    catalog = fits.getdata(VERIF_IMAGE, -1)
+    bands = [b.upper() for b in fits.getheader(VERIF_IMAGE, -1)["FILTERS"].split(",")]
+    patch.build_patch(region=None, allbands=bands)
+    return patch, catalog

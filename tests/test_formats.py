@@ -1,0 +1,164 @@
+"""FITS, HDF5 and TAN WCS readers against the reference's own data files.
+
+Pins: the image statistics and catalog row of the verification image
+(tests/verification/data/reference-*.fits: sum 0.935400442635408, max
+0.0217633531574183, catalog ra 53.00028993, dec -26.99987083, sersic 2.2,
+rhalf 0.1); the catalog position, which make_reference.py:81-83 computed with
+astropy's ``all_pix2world((nx-1)/2, (ny-1)/2, 0)`` -- a golden vector of the
+TAN projection; the PSF store of tests/data/test_image (one Gaussian, amp 1,
+Cxx = Cyy = 6.25, one row per radius bin); the spline table's grid layout.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from fpx_testutil import VERIF_DIR, VERIF_IMAGE, TEST_IMAGE, TEST_IMAGE_PSF
+from forcepho_amd.formats import fits
+from forcepho_amd.formats.hdf5 import File, HDF5FormatError
+from forcepho_amd.formats.wcs import TanWCS, WCSError
+
+REF_TESTS = "/root/reference/tests"
+
+
+def test_fits_image_and_headers():
+    hdus = fits.open(VERIF_IMAGE)
+    assert [h.kind for h in hdus] == ["PRIMARY"] + ["IMAGE"] * 7 + ["BINTABLE"]
+    im = hdus[0].data
+    assert im.shape == (32, 63) and im.dtype == np.float64
+    assert im.sum() == pytest.approx(0.935400442635408, rel=1e-15)
+    assert im.max() == pytest.approx(0.0217633531574183, rel=1e-15)
+    hdr = hdus[0].header
+    assert hdr["NAXIS1"] == 63 and hdr["CTYPE1"] == "RA---TAN" and hdr["SIMPLE"] is True
+    assert hdr["PC1_1"] == pytest.approx(8.3333333333333e-06)
+    # a long string continued on a CONTINUE card
+    assert hdr["OUTNAME"].endswith("reference-2021Nov30_f200w_sersic=2.2_rhalf=0.10.fits")
+    units = [h.header["BUNIT"] for h in hdus[1:8]]
+    assert units == ["dFLUX/d%s_0" % p for p in
+                     ("F200W", "ra", "dec", "q", "pa", "sersic", "rhalf")]
+    # flux 1: the flux derivative image is the image
+    np.testing.assert_allclose(hdus[1].data, im, rtol=1e-12, atol=1e-17)
+
+
+def test_fits_catalog_row():
+    cat = fits.getdata(VERIF_IMAGE, -1)
+    assert len(cat) == 1
+    row = cat[0]
+    assert row["ra"] == pytest.approx(53.00028993, abs=1e-8)
+    assert row["dec"] == pytest.approx(-26.99987083, abs=1e-8)
+    assert (row["q"], row["pa"], row["sersic"], row["rhalf"], row["F200W"]) == (1.0, 0.0, 2.2, 0.1, 1.0)
+    assert cat.dtype["id"] == np.dtype(np.int32) and cat.dtype["roi"] == np.dtype(np.float32)
+    assert fits.getheader(VERIF_IMAGE, -1)["FILTERS"] == "F200W"
+
+
+def test_tan_wcs_against_astropy_catalog_position():
+    """make_reference.py:81-83 placed the source at pixel ((nx-1)/2, (ny-1)/2)
+    = (31, 15.5) with astropy; the catalog keeps its ra, dec in float64."""
+    hdr = fits.getheader(VERIF_IMAGE, 0)
+    w = TanWCS(hdr)
+    row = fits.getdata(VERIF_IMAGE, -1)[0]
+    x, y = w.all_world2pix(row["ra"], row["dec"], 0)
+    assert abs(x - 31.0) < 1e-7 and abs(y - 15.5) < 1e-7
+    ra, dec = w.all_pix2world(31.0, 15.5, 0)
+    assert abs(ra - row["ra"]) < 1e-12 and abs(dec - row["dec"]) < 1e-12
+    # the FITS (origin 1) convention is the same point shifted by one pixel
+    x1, y1 = w.all_world2pix(row["ra"], row["dec"], 1)
+    assert abs(x1 - 32.0) < 1e-7 and abs(y1 - 16.5) < 1e-7
+    # the reference pixel maps to CRVAL
+    ra0, dec0 = w.all_pix2world(0.0, 0.0, 0)
+    assert abs(ra0 - 53.0) < 1e-12 and abs(dec0 + 27.0) < 1e-12
+    np.testing.assert_allclose(w.pixel_scale_matrix * 3600, 0.03 * np.eye(2), rtol=1e-12)
+
+
+def test_tan_wcs_closed_forms_and_round_trip():
+    """Along the reference meridian the gnomonic projection gives
+    eta = tan(dec - dec0); along the equator through (0, 0),
+    xi = tan(ra - ra0).  Round trips hold far from the tangent point."""
+    hdr = {"CTYPE1": "RA---TAN", "CTYPE2": "DEC--TAN", "CRVAL1": 10.0, "CRVAL2": 0.0,
+           "CRPIX1": 1.0, "CRPIX2": 1.0, "CD1_1": 1e-3, "CD2_2": 1e-3}
+    w = TanWCS(hdr)
+    for d in (0.01, 0.5, 5.0):
+        x, y = w.all_world2pix(10.0, d, 0)
+        assert abs(x) < 1e-9
+        assert y == pytest.approx(np.rad2deg(np.tan(np.deg2rad(d))) / 1e-3, rel=1e-12)
+        x, y = w.all_world2pix(10.0 + d, 0.0, 0)
+        assert x == pytest.approx(np.rad2deg(np.tan(np.deg2rad(d))) / 1e-3, rel=1e-12)
+    rot = dict(hdr, CRVAL2=-60.0, CD1_2=2e-4, CD2_1=-1e-4)
+    w = TanWCS(rot)
+    px = np.array([-4000.0, 0.0, 123.4, 5000.0])
+    py = np.array([3000.0, 0.0, -77.7, 2500.0])
+    ra, dec = w.all_pix2world(px, py, 0)
+    qx, qy = w.all_world2pix(ra, dec, 0)
+    np.testing.assert_allclose(qx, px, atol=1e-7)
+    np.testing.assert_allclose(qy, py, atol=1e-7)
+
+
+def test_tan_wcs_refuses_distortion():
+    hdr = dict(fits.getheader(VERIF_IMAGE, 0))
+    with pytest.raises(WCSError):
+        TanWCS(dict(hdr, A_ORDER=2))
+    with pytest.raises(WCSError):
+        TanWCS(dict(hdr, CTYPE1="RA---SIN"))
+
+
+def test_hdf5_psf_store_file():
+    """tests/data/test_image PSF store: group CLEAR, compound dataset
+    parameters (1, 9), attribute n_psf_per_source = 9."""
+    with File(TEST_IMAGE_PSF) as f:
+        assert f.keys() == ["CLEAR"]
+        g = f["CLEAR"]
+        assert int(g.attrs["n_psf_per_source"]) == 9
+        pars = g["parameters"][:]
+    assert pars.shape == (1, 9)
+    assert pars.dtype.names == ("amp", "xcen", "ycen", "Cxx", "Cyy", "Cxy", "sersic_bin")
+    np.testing.assert_array_equal(pars["amp"], 1.0)
+    np.testing.assert_array_equal(pars["Cxx"], 6.25)
+    np.testing.assert_array_equal(pars["Cyy"], 6.25)
+    np.testing.assert_array_equal(pars["Cxy"], 0.0)
+    np.testing.assert_array_equal(pars["sersic_bin"][0], np.arange(9))
+
+
+def test_psf_store_lookup():
+    from forcepho_amd.patches import PSFStore
+    store = PSFStore(TEST_IMAGE_PSF)
+    assert store.n_psf_per_source("CLEAR") == 9
+    psf = store.get_local_psf(band="CLEAR")
+    assert len(psf) == 9 and (psf["sersic_bin"] == np.arange(9)).all()
+    with pytest.raises(KeyError):
+        store.lookup("F444W")
+
+
+def test_hdf5_rejects_non_hdf5():
+    with pytest.raises(HDF5FormatError):
+        File(VERIF_IMAGE)
+
+
+def test_spline_table_is_a_grid():
+    """The mixture table is sersic-major over a 5 x 10 grid, and each row's
+    amplitudes sum to one (float32)."""
+    t = np.load(os.path.join(VERIF_DIR, "sersic_mog_smooth0.0150.npz"))
+    n, r, A = t["nsersic"], t["rh"], t["amplitudes"]
+    assert A.shape == (50, 9) and t["radii"][0] == 0.0
+    np.testing.assert_array_equal(n.reshape(5, 10), np.repeat(np.unique(n), 10).reshape(5, 10))
+    np.testing.assert_array_equal(r.reshape(5, 10), np.tile(np.unique(r), 5).reshape(5, 10))
+    np.testing.assert_allclose(A.sum(axis=1), 1.0, atol=1e-6)
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_TESTS), reason="reference tree not present")
+def test_hdf5_reader_matches_committed_table():
+    """The committed table is what the reader reads from the reference file;
+    the file's other datasets read with their declared shapes."""
+    t = np.load(os.path.join(VERIF_DIR, "sersic_mog_smooth0.0150.npz"))
+    with File(os.path.join(REF_TESTS, "data/sersic_mog_model.smooth=0.0150.h5")) as f:
+        for k in ("nsersic", "rh", "amplitudes", "radii"):
+            np.testing.assert_array_equal(f[k][:], t[k])
+        assert f["model"].shape == (50, 6399) and f["x"].shape == (6399,)
+        assert f.attrs["smoothing"] == pytest.approx(0.015)
+    with File(os.path.join(REF_TESTS, "data/sersic_splinedata_large.h5")) as f:
+        A, n, r = f["amplitudes"][:], f["nsersic"][:], f["rh"][:]
+    # a 55 x 101 sersic-major grid of mixture amplitudes in [0, 1]
+    assert A.shape == (5555, 9) and A.dtype == np.float64
+    assert len(np.unique(n)) * len(np.unique(r)) == 5555
+    np.testing.assert_array_equal(n.reshape(55, 101), np.unique(n)[:, None] + 0 * np.unique(r))
+    assert A.min() >= 0.0 and A.max() <= 1.0
