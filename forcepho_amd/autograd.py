@@ -14,10 +14,12 @@ sum (r ierr)^2 - (d ierr)^2 (compute_gaussians_kernel.cc:286-293) on top,
 so ``-0.5 * chi2`` is forcepho's log-likelihood and its autograd gradient
 matches FastPosterior.lnprob_grad in the untransformed space.
 
-When every source uses LogNormalMixtureAmplitudes the Source records are
-built on the device from theta (device_scene.DeviceRecords) and the gradient
-is stacked there too: forward and backward then make no host round trip.
-Other amplitude models build the records on the host (Galaxy.proposal).
+When the scene's amplitude model can be evaluated on the device (the
+analytic LogNormalMixtureAmplitudes, or forcepho's spline tables shared by
+every source: device_scene.DeviceRecords) the Source records are built on the
+device from theta and the gradient is stacked there too: forward and backward
+then make no host round trip.  Other amplitude models build the records on
+the host (Galaxy.proposal).
 """
 
 import numpy as np

@@ -11,9 +11,11 @@ Every chain's position lives on the device; one leapfrog step is
                                                             forcepho/model.py:302-387)
       -> momentum / position updates,
 all as device operations with no host round trip, so a whole trajectory can
-be recorded once into a graph and replayed.  The mixture amplitudes use the
-analytic LogNormalMixtureAmplitudes model (sources.py), evaluated on the
-device for every (chain, source) by device_scene.DeviceRecords.  Momentum draws and the Metropolis test run
+be recorded once into a graph and replayed.  The mixture amplitudes and
+their derivatives are evaluated on the device for every (chain, source) by
+device_scene.DeviceRecords: forcepho's spline tables (one
+SplineMixtureAmplitudes shared by all scenes) or the analytic
+LogNormalMixtureAmplitudes model.  Momentum draws and the Metropolis test run
 outside the graph.
 """
 

@@ -112,12 +112,13 @@ def verification_psfstore(n_radii=9):
     return PSFStore.from_mixtures({"F200W": mix}, n_radii=n_radii)
 
 
-def verification_patch(cls, spline_type="smooth", pixel_order="superpixel", **kwargs):
+def verification_patch(cls, spline_type="smooth", pixel_order="superpixel", amplitudes=None,
+                       **kwargs):
     """(patch, catalog) of the verification image packed by ``cls`` (a
     FITSPatch subclass): whole image, pixel inverse errors 1/(dFLUX/dF200W)
     from extension 1, as verify_reference.py:73-79 configures it."""
     from forcepho_amd.formats import fits
-    amps = verification_amplitudes(spline_type)
+    amps = amplitudes if amplitudes is not None else verification_amplitudes(spline_type)
     patch = cls(fitsfiles=[VERIF_IMAGE], psfstore=verification_psfstore(len(amps.radii)),
                 splinedata=amps, pixel_order=pixel_order, **kwargs)
     catalog = fits.getdata(VERIF_IMAGE, -1)

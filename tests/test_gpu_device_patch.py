@@ -85,3 +85,23 @@ def test_subtract_fixed_and_design_matrix():
         flux = np.array([s.flux[b] for s in active.sources])
         np.testing.assert_allclose(flux @ Xes[b], split[b], rtol=0,
                                    atol=2e-6 * np.abs(m_active).max())
+
+
+def test_optimize_fluxes():
+    """forcepho/fitting.py:589-620: after subtracting a fixed source the ML
+    fluxes of the active sources solve the weighted normal equations of the
+    kernel-rendered design matrix, and lie within 5 sigma of the truth."""
+    from forcepho_amd.fitting import optimize_fluxes
+    patch, scene = _patch(seed=34)
+    fixed = scene.subscene([0])
+    active = scene.subscene([1, 2])
+    patch.subtract_fixed(fixed, active=active)
+    fluxes, precisions, Xes, ys, ws = optimize_fluxes(patch, active, return_all=True)
+    assert len(fluxes) == patch.n_bands
+    for b in range(patch.n_bands):
+        w = ws[b].astype(np.float64)
+        ref = np.linalg.lstsq((Xes[b] * w).T, ys[b].astype(np.float64) * w, rcond=None)[0]
+        np.testing.assert_allclose(fluxes[b], ref, rtol=1e-8)
+        truth = np.array([s.flux[b] for s in active.sources])
+        sigma = np.sqrt(np.diag(np.linalg.inv(precisions[b])))
+        assert (np.abs(fluxes[b] - truth) < 5 * sigma).all(), (fluxes[b], truth, sigma)

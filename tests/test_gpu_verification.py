@@ -111,3 +111,92 @@ def test_cswap_on_reference_test_image():
     red = np.sum((residual_original.astype(np.float64) * patcher.ierr)[valid] ** 2) / n
     assert abs(red - 1.0) < 5 * np.sqrt(2.0 / n)
     assert model[valid].sum() == pytest.approx(float(cat["CLEAR"][0]), abs=0.01)
+
+
+def _bounds(theta, nb, q_hi=1.2):
+    t = theta.reshape(-1, nb + 6)
+    lo, hi = t.copy(), t.copy()
+    lo[:, :nb], hi[:, :nb] = t[:, :nb] * 0.5, t[:, :nb] * 1.5 + 1e-3
+    pix = 0.03 / 3600.0
+    lo[:, nb:nb + 2] -= 3 * pix
+    hi[:, nb:nb + 2] += 3 * pix
+    lo[:, nb + 2], hi[:, nb + 2] = 0.3, q_hi
+    lo[:, nb + 3], hi[:, nb + 3] = -2.0, 2.0
+    lo[:, nb + 4], hi[:, nb + 4] = 0.8, 5.5
+    lo[:, nb + 5], hi[:, nb + 5] = 0.02, 0.35
+    return lo.reshape(-1), hi.reshape(-1)
+
+
+def test_ensemble_on_real_data_with_spline_amplitudes():
+    """HMCEnsemble builds Source records on the device from the reference's
+    spline table (DeviceSplineAmplitudes) for two FITS patches; its potential
+    and gradient equal the host FastPosterior, whose records come from scipy
+    (Galaxy.proposal)."""
+    from forcepho_amd.formats import fits
+    from forcepho_amd.patches import FITSDevicePatch
+    from forcepho_amd.ensemble import HMCEnsemble
+    from forcepho_amd.model import FastPosterior
+    from forcepho_amd.proposal import HIPProposer
+    amps = verification_amplitudes("smooth")
+    p1, cat1 = verification_patch(FITSDevicePatch, amplitudes=amps)
+    p2 = FITSDevicePatch(fitsfiles=[TEST_IMAGE], psfstore=TEST_IMAGE_PSF, splinedata=amps)
+    p2.build_patch(None, None, allbands=["CLEAR"])
+    pats, scs, bnds, start = [], [], [], []
+    for patch, cat in ((p1, cat1), (p2, fits.getdata(TEST_IMAGE, -1))):
+        scene = patch.set_scene(cat)
+        patch.pack_meta(scene)
+        patch.send_to_device()
+        theta = scene.get_all_source_params().copy()
+        pats.append(patch)
+        scs.append(scene)
+        bnds.append(_bounds(theta, 1))
+        start.append(theta)
+    ens = HMCEnsemble(pats, scs, [2, 2], bnds)
+    assert ens.records.spline is not None
+    thetas = []
+    for pi, c in ens.batch.slot_map:
+        t = start[pi].copy()
+        t[2] *= 0.95 - 0.05 * c          # q
+        t[4] *= 1.0 + 0.1 * c            # sersic
+        t[5] *= 1.1 - 0.2 * c            # rhalf
+        thetas.append(t)
+    ens.set_theta(thetas)
+    print("ensemble potential", flush=True)
+    Ud, gd = (x.cpu().numpy() for x in ens.potential())
+    for s, (pi, _) in enumerate(ens.batch.slot_map):
+        patch = pats[pi]
+        post = FastPosterior(HIPProposer(patch=patch), patch, scene=scs[pi],
+                             lower=bnds[pi][0], upper=bnds[pi][1])
+        lnp, grad = post.lnprob_and_grad(post.transform.inverse_transform(thetas[s]))
+        assert Ud[s] == pytest.approx(-lnp, rel=1e-6)
+        g = np.concatenate([gd[s, :, :1], gd[s, :, ens.NB:]], axis=1)[:patch.n_sources].reshape(-1)
+        np.testing.assert_allclose(g, -grad, rtol=1e-4, atol=1e-5 * np.abs(grad).max())
+
+
+def test_autograd_records_from_spline_table():
+    """PatchModel keeps real-data scenes on the device path: the records
+    come from the device spline evaluation, and chi2 and its autograd
+    gradient equal the host-record path."""
+    import torch
+    from forcepho_amd.patches import FITSDevicePatch
+    from forcepho_amd.autograd import PatchModel
+    patch, cat = verification_patch(FITSDevicePatch)
+    scene = patch.set_scene(cat)
+    patch.pack_meta(scene)
+    model = PatchModel(patch, scene)
+    assert model.records is not None and model.records.spline is not None
+    theta = scene.get_all_source_params().copy()
+    theta[2:6] = 0.85, 0.4, 2.6, 0.12
+    out = []
+    for device_path in (True, False):
+        saved = model.records
+        if not device_path:
+            model.records = None
+        t = torch.tensor(theta, dtype=torch.float64, device="cuda", requires_grad=True)
+        c = model.chi2(t)
+        c.backward()
+        out.append((c.item(), t.grad.cpu().numpy()))
+        model.records = saved
+    np.testing.assert_allclose(out[0][0], out[1][0], rtol=1e-9)
+    g = out[1][1]
+    np.testing.assert_allclose(out[0][1], g, rtol=1e-4, atol=1e-5 * np.abs(g).max())

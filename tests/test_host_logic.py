@@ -110,3 +110,24 @@ def test_posterior_needs_transform_or_bounds():
     with pytest.raises(ValueError):
         FastPosterior(CheckerProposer(), patch=None, scene=object())
     FastPosterior(CheckerProposer(), patch=None, scene=object(), transform=Transform(3))
+
+
+def test_optimize_one_band_matches_lstsq():
+    """The weighted normal equations give the least-squares solution of
+    (X w)^T f = y w, and recover exact fluxes from noise-free data."""
+    from forcepho_amd.fitting import optimize_one_band
+    rng = np.random.default_rng(5)
+    X = rng.random((3, 200))
+    w = 0.5 + rng.random(200)
+    f_true = np.array([2.0, -1.0, 0.5])
+    y = f_true @ X + 0.01 * rng.normal(size=200) / w
+    flux, prec = optimize_one_band(X, w, y)
+    ref = np.linalg.lstsq((X * w).T, y * w, rcond=None)[0]
+    np.testing.assert_allclose(flux[:, 0], ref, rtol=1e-10)
+    np.testing.assert_allclose(prec, (X * w) @ (X * w).T, rtol=1e-12)
+    flux0, _ = optimize_one_band(X, w, f_true @ X)
+    np.testing.assert_allclose(flux0[:, 0], f_true, rtol=1e-10)
+    # a fixed-source model is subtracted first
+    fixed = rng.random(200)
+    flux1, _ = optimize_one_band(X, w, f_true @ X + fixed, fixedX=fixed)
+    np.testing.assert_allclose(flux1[:, 0], f_true, rtol=1e-10)
