@@ -22,13 +22,14 @@ import builtins
 
 import numpy as np
 
-__all__ = ["Header", "HDU", "open", "getheader", "getdata", "FITSFormatError"]
+__all__ = ["Header", "HDU", "open", "getheader", "getdata", "write_table", "write_images",
+           "FITSFormatError"]
 
 BLOCK = 2880
 CARD = 80
 
 _BITPIX = {8: ">u1", 16: ">i2", 32: ">i4", 64: ">i8", -32: ">f4", -64: ">f8"}
-_TFORM = {"L": ("?", 1), "B": (">u1", 1), "I": (">i2", 2), "J": (">i4", 4),
+_TFORM = {"L": ("u1", 1), "B": (">u1", 1), "I": (">i2", 2), "J": (">i4", 4),
           "K": (">i8", 8), "E": (">f4", 4), "D": (">f8", 8), "C": (">c8", 8),
           "M": (">c16", 16)}
 
@@ -184,7 +185,7 @@ class HDU:
         row_bytes = int(hdr["NAXIS1"])
         n_rows = int(hdr["NAXIS2"])
         names, formats, offsets = [], [], []
-        scaled = {}
+        scaled, logical = {}, set()
         off = 0
         for i in range(1, int(hdr["TFIELDS"]) + 1):
             form = str(hdr["TFORM%d" % i]).strip()
@@ -208,7 +209,9 @@ class HDU:
                 names.append(name)
                 formats.append(dt)
                 offsets.append(off)
-                if ("TZERO%d" % i) in hdr or ("TSCAL%d" % i) in hdr:
+                if code == "L":
+                    logical.add(name)
+                elif ("TZERO%d" % i) in hdr or ("TSCAL%d" % i) in hdr:
                     scaled[name] = (hdr.get("TSCAL%d" % i, 1), hdr.get("TZERO%d" % i, 0))
             off += width
         if off != row_bytes:
@@ -219,7 +222,9 @@ class HDU:
         out_fields = []
         for name, f in zip(names, formats):
             base = f.base
-            if base.kind in "iufc":
+            if name in logical:
+                out_fields.append((name, np.dtype(bool), f.shape))
+            elif base.kind in "iufc":
                 nb = base.newbyteorder("=")
                 if name in scaled:
                     nb = np.dtype(np.float64)
@@ -232,6 +237,8 @@ class HDU:
             if name in scaled:
                 sc, z = scaled[name]
                 col = col.astype(np.float64) * sc + z
+            elif name in logical:
+                col = col == ord("T")       # logicals are the bytes 'T' / 'F'
             out[name] = col
         return out
 
@@ -261,3 +268,110 @@ def getheader(path, ext=0):
 
 def getdata(path, ext=0):
     return open(path)[ext].data
+
+
+# ---------------------------------------------------------------------------
+# Writing (binary tables and images), for catalog state files
+
+_WRITE_CODES = {"?": "L", "u1": "B", "i2": "I", "i4": "J", "i8": "K", "f4": "E", "f8": "D",
+                "c8": "C", "c16": "M"}
+
+
+def _card(key, value, comment=""):
+    if isinstance(value, bool):
+        v = "%20s" % ("T" if value else "F")
+    elif isinstance(value, (int, np.integer)):
+        v = "%20d" % int(value)
+    elif isinstance(value, (float, np.floating)):
+        v = "%20s" % repr(float(value)).upper()
+    else:
+        v = "'%-8s'" % str(value).replace("'", "''")
+    text = "%-8s= %s" % (key[:8], v)
+    if comment:
+        text += " / " + comment
+    if len(text) > CARD:
+        raise FITSFormatError("card for %s does not fit in 80 characters" % key)
+    return text.ljust(CARD)
+
+
+def _header_bytes(cards):
+    body = "".join(cards) + "END".ljust(CARD)
+    body += " " * ((-len(body)) % BLOCK)
+    return body.encode("ascii")
+
+
+def _pad(data):
+    return data + b"\x00" * ((-len(data)) % BLOCK)
+
+
+def table_hdu_bytes(table, header=None):
+    """A BINTABLE extension (header + data bytes) of a structured array."""
+    table = np.asarray(table)
+    cols, forms = [], []
+    for name in table.dtype.names:
+        f = table.dtype[name]
+        base, shape = f.base, f.shape
+        n = int(np.prod(shape)) if shape else 1
+        if base.kind == "S":
+            forms.append("%dA" % base.itemsize)
+            cols.append((name, base, shape))
+            continue
+        key = base.str[1:] if base.kind != "b" else "?"
+        if key not in _WRITE_CODES:
+            raise FITSFormatError("column %s of dtype %s cannot be written" % (name, base))
+        forms.append(("%d" % n if n != 1 else "") + _WRITE_CODES[key])
+        cols.append((name, base.newbyteorder(">") if base.kind != "b" else base, shape))
+    big = np.zeros(len(table), dtype=[(n, b if b.kind != "b" else np.dtype("u1"), s) if s
+                                      else (n, b if b.kind != "b" else np.dtype("u1"))
+                                      for n, b, s in cols])
+    for name in table.dtype.names:
+        if table.dtype[name].base.kind == "b":
+            big[name] = np.where(table[name], ord("T"), ord("F"))   # logicals as 'T' / 'F'
+        else:
+            big[name] = table[name]
+    cards = [_card("XTENSION", "BINTABLE", "binary table extension"),
+             _card("BITPIX", 8), _card("NAXIS", 2), _card("NAXIS1", big.dtype.itemsize),
+             _card("NAXIS2", len(big)), _card("PCOUNT", 0), _card("GCOUNT", 1),
+             _card("TFIELDS", len(cols))]
+    for i, (name, form) in enumerate(zip(table.dtype.names, forms), start=1):
+        cards += [_card("TTYPE%d" % i, name), _card("TFORM%d" % i, form)]
+    for k, v in (header or {}).items():
+        cards.append(_card(k, v))
+    return _header_bytes(cards) + _pad(big.tobytes())
+
+
+def write_table(path, table, header=None):
+    """Write a structured array as a FITS file: an empty primary HDU and
+    one BINTABLE extension (extra header cards from ``header``)."""
+    primary = _header_bytes([_card("SIMPLE", True), _card("BITPIX", 8), _card("NAXIS", 0),
+                             _card("EXTEND", True)])
+    with builtins.open(path, "wb") as fh:
+        fh.write(primary + table_hdu_bytes(table, header))
+
+
+def image_hdu_bytes(data, header=None, primary=False):
+    """An image HDU (header + data bytes) of a 2-d float or integer array."""
+    a = np.asarray(data)
+    codes = {np.dtype(np.float64): -64, np.dtype(np.float32): -32, np.dtype(np.int32): 32,
+             np.dtype(np.int16): 16, np.dtype(np.uint8): 8, np.dtype(np.int64): 64}
+    bitpix = codes.get(a.dtype.newbyteorder("="))
+    if bitpix is None:
+        raise FITSFormatError("image dtype %s cannot be written" % a.dtype)
+    cards = ([_card("SIMPLE", True)] if primary else [_card("XTENSION", "IMAGE")])
+    cards += [_card("BITPIX", bitpix), _card("NAXIS", a.ndim)]
+    cards += [_card("NAXIS%d" % (i + 1), n) for i, n in enumerate(a.shape[::-1])]
+    cards += ([_card("EXTEND", True)] if primary else [_card("PCOUNT", 0), _card("GCOUNT", 1)])
+    for k, v in (header or {}).items():
+        cards.append(_card(k, v))
+    return _header_bytes(cards) + _pad(a.astype(a.dtype.newbyteorder(">")).tobytes())
+
+
+def write_images(path, images, headers=None, table=None, table_header=None):
+    """Write 2-d arrays as the primary HDU and IMAGE extensions (numpy C
+    order: the last axis is NAXIS1), optionally followed by a BINTABLE."""
+    headers = headers or [None] * len(images)
+    with builtins.open(path, "wb") as fh:
+        for i, (im, hdr) in enumerate(zip(images, headers)):
+            fh.write(image_hdu_bytes(im, hdr, primary=(i == 0)))
+        if table is not None:
+            fh.write(table_hdu_bytes(table, table_header))
