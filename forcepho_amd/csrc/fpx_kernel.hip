@@ -118,6 +118,8 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask) {
 //                                       (T, dT/dq, dT/dpa, centre)
 //   cwl      MAXSOURCES x 4 floats     CW of every source for the exposure
 //   rad2     kRad2 floats              the patch's radius-bin variances
+//   order    kPsfOrder ints            the exposure's PSF entries with equal
+//                                       centres made adjacent (see psf_order)
 //   rows     W x kRowBytes             per-wave accumulators: chi2 (double)
 //                                       then MAXSOURCES*NPARAMS gradient sums
 //   index    index_cap + 1 ints        cull flags, then the survivors'
@@ -127,7 +129,8 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask) {
 constexpr int kRowBytes = 8 + MAXSOURCES * NPARAMS * 4;
 
 constexpr int kRad2 = 16;  // >= MAXRADII
-constexpr int kExpoFloats = MAXSOURCES * (kGeoFloats + 4) + kRad2;
+constexpr int kPsfOrder = FPX_MAX_PSF_PER_SOURCE;
+constexpr int kExpoFloats = MAXSOURCES * (kGeoFloats + 4) + kRad2 + kPsfOrder;
 
 __host__ __device__ constexpr size_t lds_bytes_for(int waves, int ppl, int gauss_cap,
                                                    int index_cap) {
@@ -142,6 +145,7 @@ struct Lds {
   float *geo;
   float *cwl;
   float *rad2;
+  int *order;
   unsigned char *rows;
   int *index;
 };
@@ -153,7 +157,8 @@ __device__ __forceinline__ Lds carve(unsigned char *smem, int waves, int ppl, in
   l.geo = l.part + waves * 64 * ppl;
   l.cwl = l.geo + MAXSOURCES * kGeoFloats;
   l.rad2 = l.cwl + MAXSOURCES * 4;
-  l.rows = reinterpret_cast<unsigned char *>(l.rad2 + kRad2);
+  l.order = reinterpret_cast<int *>(l.rad2 + kRad2);
+  l.rows = reinterpret_cast<unsigned char *>(l.order + kPsfOrder);
   l.index = reinterpret_cast<int *>(l.rows + waves * kRowBytes);
   return l;
 }
@@ -181,26 +186,72 @@ __device__ __forceinline__ float bin_covar(const Lds &l, const FpxPSFSourceGauss
   return l.rad2[min(max(psf.sersic_radius_bin, 0), MAXRADII - 1)];
 }
 
+// Order the exposure's n_psf PSF entries so that entries with equal centres
+// (xcen, ycen) are adjacent: by the first entry of their centre, then by
+// index.  Every source of an exposure uses the same entries (the source-0 PSF
+// rule), and all radius bins of a PSF component share its centre, so the
+// records of one source then come in runs of one centre, whose pixel offsets
+// the passes compute once (model_pass, grad_pass).  Uses the record area as
+// scratch (3 n_psf words: it holds at least min(max_gauss, 16) records of kRec
+// floats, and n_psf <= max_gauss, n_psf <= kPsfOrder); called before the
+// tile's records are built.
+template <int W>
+__device__ void psf_order(const FpxPatch *__restrict__ P, int pstart, int n_psf, const Lds &l) {
+  float *cx = l.recs;
+  float *cy = cx + kPsfOrder;
+  int *lead = reinterpret_cast<int *>(cy + kPsfOrder);
+  __syncthreads();  // previous users of the index and the order are done
+  for (int j = threadIdx.x; j < n_psf; j += W * 64) {
+    cx[j] = P->psfgauss[pstart + j].xcen;
+    cy[j] = P->psfgauss[pstart + j].ycen;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < n_psf; j += W * 64) {
+    int g = j;
+    for (int i = 0; i < j; ++i) {
+      if (cx[i] == cx[j] && cy[i] == cy[j]) {
+        g = i;
+        break;
+      }
+    }
+    lead[j] = g;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < n_psf; j += W * 64) {
+    const int gj = lead[j];
+    int pos = 0;
+    for (int i = 0; i < n_psf; ++i) {
+      const int gi = lead[i];
+      pos += (gi < gj || (gi == gj && i < j)) ? 1 : 0;
+    }
+    l.order[pos] = j;
+  }
+  __syncthreads();
+}
+
 template <int W>
 __device__ int cull_tile(const FpxPatch *__restrict__ P, int pstart, int n_src, int n_psf,
                          const Box &box, const Lds &l, int wave, int lane) {
   const int n = n_src * n_psf;
   int *index = l.index;
   __syncthreads();  // previous users of the index are done; geometry is in LDS
-  for (int lg = threadIdx.x; lg < n; lg += W * 64) {
-    const int s = lg / n_psf;
-    const FpxPSFSourceGaussian psf = P->psfgauss[pstart + lg - s * n_psf];
-    index[lg] = gauss_reaches(load_geometry(l.geo + s * kGeoFloats), psf, bin_covar(l, psf),
-                              box.x0, box.x1, box.y0, box.y1) ? 1 : 0;
+  // position q = s * n_psf + j holds component k = order[j] of source s
+  for (int q = threadIdx.x; q < n; q += W * 64) {
+    const int s = q / n_psf;
+    const int k = l.order[q - s * n_psf];
+    const FpxPSFSourceGaussian psf = P->psfgauss[pstart + k];
+    index[q] = gauss_reaches(load_geometry(l.geo + s * kGeoFloats), psf, bin_covar(l, psf),
+                             box.x0, box.x1, box.y0, box.y1) ? s * n_psf + k : -1;
   }
   __syncthreads();
   if (wave == 0) {
     int count = 0;
     for (int b0 = 0; b0 < n; b0 += 64) {
-      const int lg = b0 + lane;
-      const bool keep = lg < n && index[lg] != 0;
+      const int q = b0 + lane;
+      const int lg = q < n ? index[q] : -1;
+      const bool keep = lg >= 0;
       const unsigned long long m = __builtin_amdgcn_ballot_w64(keep);
-      // every lane has read its flag above; writes land at or below b0 + lane
+      // every lane has read its entry above; writes land at or below b0 + lane
       if (keep) index[count + lanes_below(m)] = lg;
       count += __builtin_popcountll(m);
     }
@@ -226,18 +277,14 @@ __device__ void build_records(const FpxPatch *__restrict__ P,
   __syncthreads();
 }
 
-// The pixel-side quantities of one Gaussian at a lane's PPL pixels: offsets,
-// their products, log2 of the Gaussian and the cutoff flag.  Returns a bit
-// per pixel slot i (a 64-pixel block of the tile) that is set when some lane
-// of the wavefront has pixel i inside the cutoff.
+// The pixel offsets from a Gaussian centre at a lane's PPL pixels and their
+// products.  Records come in runs of one centre (psf_order), so the passes
+// compute these only when the centre changes.
 template <int PPL>
-__device__ __forceinline__ unsigned gauss_at_pixels(const float4 r0, const float cyy,
-                                                const float (&xp)[PPL], const float (&yp)[PPL],
-                                                float (&dx)[PPL], float (&dy)[PPL],
-                                                float (&dxx)[PPL], float (&dyy)[PPL],
-                                                float (&dxy)[PPL], float (&lg)[PPL],
-                                                bool (&on)[PPL]) {
-  unsigned blocks = 0;
+__device__ __forceinline__ void offsets_at_pixels(const float4 r0, const float (&xp)[PPL],
+                                                  const float (&yp)[PPL], float (&dx)[PPL],
+                                                  float (&dy)[PPL], float (&dxx)[PPL],
+                                                  float (&dyy)[PPL], float (&dxy)[PPL]) {
 #pragma unroll
   for (int i = 0; i < PPL; ++i) {
     dx[i] = xp[i] - r0.x;
@@ -245,12 +292,36 @@ __device__ __forceinline__ unsigned gauss_at_pixels(const float4 r0, const float
     dxx[i] = dx[i] * dx[i];
     dyy[i] = dy[i] * dy[i];
     dxy[i] = dx[i] * dy[i];
+  }
+}
+
+// log2 of one Gaussian at a lane's PPL pixels and the cutoff flag.  Returns a
+// bit per pixel slot i (a 64-pixel block of the tile) that is set when some
+// lane of the wavefront has pixel i inside the cutoff.
+template <int PPL>
+__device__ __forceinline__ unsigned gauss_log2(const float4 r0, const float cyy,
+                                               const float (&dxx)[PPL], const float (&dyy)[PPL],
+                                               const float (&dxy)[PPL], float (&lg)[PPL],
+                                               bool (&on)[PPL]) {
+  unsigned blocks = 0;
+#pragma unroll
+  for (int i = 0; i < PPL; ++i) {
     lg[i] = fmaf(r0.z, dxx[i], fmaf(r0.w, dxy[i], cyy * dyy[i]));
     on[i] = !(lg[i] < kLog2Min);
     blocks |= wave_any(on[i]) ? (1u << i) : 0u;
   }
   return blocks;
 }
+
+// Bits of a record's centre as a wave-uniform pair, to detect a change of
+// centre with a scalar compare (all lanes read the same record).
+__device__ __forceinline__ long long centre_key(const float4 r0) {
+  const unsigned x = __builtin_amdgcn_readfirstlane(__builtin_bit_cast(unsigned, r0.x));
+  const unsigned y = __builtin_amdgcn_readfirstlane(__builtin_bit_cast(unsigned, r0.y));
+  return (long long)(((unsigned long long)x << 32) | y);
+}
+// no record has this centre (two NaN halves)
+constexpr long long kNoCentre = (long long)0x7fc000017fc00001ull;
 
 // A wavefront's share of a chunk's n records: runs of kRun consecutive
 // records dealt round-robin over the W wavefronts (run r to wavefront r mod W),
@@ -264,14 +335,21 @@ __device__ __forceinline__ void model_pass(const float *__restrict__ recs, int w
                                            int n, const float (&xp)[PPL],
                                            const float (&yp)[PPL], float (&m)[PPL]) {
   const float4 *R4 = reinterpret_cast<const float4 *>(recs);
+  float dx[PPL], dy[PPL], dxx[PPL], dyy[PPL], dxy[PPL];
+  long long centre = kNoCentre;
   for (int r0 = wave * kRun; r0 < n; r0 += W * kRun)
   for (int g = r0; g < min(r0 + kRun, n); ++g) {
     const float4 r0 = R4[g * (kRec / 4) + 0];  // xc yc cxx cxy
     const float4 r1 = R4[g * (kRec / 4) + 1];  // cyy h0 sxx sxy
     const float4 r2 = R4[g * (kRec / 4) + 2];  // syy amp src dA/dflux
-    float dx[PPL], dy[PPL], dxx[PPL], dyy[PPL], dxy[PPL], lg[PPL];
+    const long long key = centre_key(r0);
+    if (key != centre) {
+      offsets_at_pixels<PPL>(r0, xp, yp, dx, dy, dxx, dyy, dxy);
+      centre = key;
+    }
+    float lg[PPL];
     bool on[PPL];
-    const unsigned blocks = gauss_at_pixels<PPL>(r0, r1.x, xp, yp, dx, dy, dxx, dyy, dxy, lg, on);
+    const unsigned blocks = gauss_log2<PPL>(r0, r1.x, dxx, dyy, dxy, lg, on);
     if (blocks == 0) continue;
 #pragma unroll
     for (int i = 0; i < PPL; ++i) {
@@ -341,6 +419,8 @@ __device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int wa
   float aF = 0.f, aX = 0.f, aY = 0.f, aQ = 0.f, aP = 0.f, aN = 0.f, aR = 0.f;
   bool touched = false;
   int cur = -1;
+  float dx[PPL], dy[PPL], dxx[PPL], dyy[PPL], dxy[PPL];
+  long long centre = kNoCentre;
   for (int r0 = wave * kRun; r0 < n; r0 += W * kRun)
   for (int g = r0; g < min(r0 + kRun, n); ++g) {
     const float4 *rg = R4 + g * (kRec / 4);
@@ -354,9 +434,14 @@ __device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int wa
       touched = false;
       cur = src;
     }
-    float dx[PPL], dy[PPL], dxx[PPL], dyy[PPL], dxy[PPL], lg[PPL];
+    const long long key = centre_key(r0);
+    if (key != centre) {
+      offsets_at_pixels<PPL>(r0, xp, yp, dx, dy, dxx, dyy, dxy);
+      centre = key;
+    }
+    float lg[PPL];
     bool on[PPL];
-    const unsigned blocks = gauss_at_pixels<PPL>(r0, r1.x, xp, yp, dx, dy, dxx, dyy, dxy, lg, on);
+    const unsigned blocks = gauss_log2<PPL>(r0, r1.x, dxx, dyy, dxy, lg, on);
     if (blocks == 0) continue;
     touched = true;
     float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, a4 = 0.f, a5 = 0.f;
@@ -460,6 +545,7 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
     box.y1 = uniform_f(wave_max(y1));
   }
 
+  psf_order<W>(P, pstart, n_psf, l);
   const int n_surv = cull_tile<W>(P, pstart, n_src, n_psf, box, l, wave, lane);
   const int n_chunks = (n_surv + gauss_cap - 1) / gauss_cap;
   auto chunk_size = [&](int c) { return min(gauss_cap, n_surv - c * gauss_cap); };
@@ -873,6 +959,9 @@ int launch_lds(DeviceState *st, int cfg, int max_gauss, size_t *lds, int *cap, i
   }
   *cap = pick_gauss_cap(kc, max_gauss, st->lds_max, static_lds);
   if (*cap < 1) return set_err(FPX_E_ARG, "max_gauss too large for the LDS of one workgroup");
+  // the record area doubles as psf_order's scratch of 3 n_psf words
+  if ((long)*cap * fpx::kRec < 3L * std::min(max_gauss, fpx::kPsfOrder))
+    return set_err(FPX_E_ARG, "max_gauss too large for the LDS of one workgroup");
   *index_cap = max_gauss;
   *lds = fpx::lds_bytes_for(kc.waves, kc.ppl, *cap, *index_cap);
   return FPX_OK;
