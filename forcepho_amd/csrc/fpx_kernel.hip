@@ -118,8 +118,6 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask) {
 //                                       (T, dT/dq, dT/dpa, centre)
 //   cwl      MAXSOURCES x 4 floats     CW of every source for the exposure
 //   rad2     kRad2 floats              the patch's radius-bin variances
-//   order    kPsfOrder ints            the exposure's PSF entries with equal
-//                                       centres made adjacent (see psf_order)
 //   rows     W x kRowBytes             per-wave accumulators: chi2 (double)
 //                                       then MAXSOURCES*NPARAMS gradient sums
 //   index    index_cap + 1 ints        cull flags, then the survivors'
@@ -129,8 +127,7 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask) {
 constexpr int kRowBytes = 8 + MAXSOURCES * NPARAMS * 4;
 
 constexpr int kRad2 = 16;  // >= MAXRADII
-constexpr int kPsfOrder = FPX_MAX_PSF_PER_SOURCE;
-constexpr int kExpoFloats = MAXSOURCES * (kGeoFloats + 4) + kRad2 + kPsfOrder;
+constexpr int kExpoFloats = MAXSOURCES * (kGeoFloats + 4) + kRad2;
 
 __host__ __device__ constexpr size_t lds_bytes_for(int waves, int ppl, int gauss_cap,
                                                    int index_cap) {
@@ -145,7 +142,6 @@ struct Lds {
   float *geo;
   float *cwl;
   float *rad2;
-  int *order;
   unsigned char *rows;
   int *index;
 };
@@ -157,8 +153,7 @@ __device__ __forceinline__ Lds carve(unsigned char *smem, int waves, int ppl, in
   l.geo = l.part + waves * 64 * ppl;
   l.cwl = l.geo + MAXSOURCES * kGeoFloats;
   l.rad2 = l.cwl + MAXSOURCES * 4;
-  l.order = reinterpret_cast<int *>(l.rad2 + kRad2);
-  l.rows = reinterpret_cast<unsigned char *>(l.order + kPsfOrder);
+  l.rows = reinterpret_cast<unsigned char *>(l.rad2 + kRad2);
   l.index = reinterpret_cast<int *>(l.rows + waves * kRowBytes);
   return l;
 }
@@ -186,72 +181,26 @@ __device__ __forceinline__ float bin_covar(const Lds &l, const FpxPSFSourceGauss
   return l.rad2[min(max(psf.sersic_radius_bin, 0), MAXRADII - 1)];
 }
 
-// Order the exposure's n_psf PSF entries so that entries with equal centres
-// (xcen, ycen) are adjacent: by the first entry of their centre, then by
-// index.  Every source of an exposure uses the same entries (the source-0 PSF
-// rule), and all radius bins of a PSF component share its centre, so the
-// records of one source then come in runs of one centre, whose pixel offsets
-// the passes compute once (model_pass, grad_pass).  Uses the record area as
-// scratch (3 n_psf words: it holds at least min(max_gauss, 16) records of kRec
-// floats, and n_psf <= max_gauss, n_psf <= kPsfOrder); called before the
-// tile's records are built.
-template <int W>
-__device__ void psf_order(const FpxPatch *__restrict__ P, int pstart, int n_psf, const Lds &l) {
-  float *cx = l.recs;
-  float *cy = cx + kPsfOrder;
-  int *lead = reinterpret_cast<int *>(cy + kPsfOrder);
-  __syncthreads();  // previous users of the index and the order are done
-  for (int j = threadIdx.x; j < n_psf; j += W * 64) {
-    cx[j] = P->psfgauss[pstart + j].xcen;
-    cy[j] = P->psfgauss[pstart + j].ycen;
-  }
-  __syncthreads();
-  for (int j = threadIdx.x; j < n_psf; j += W * 64) {
-    int g = j;
-    for (int i = 0; i < j; ++i) {
-      if (cx[i] == cx[j] && cy[i] == cy[j]) {
-        g = i;
-        break;
-      }
-    }
-    lead[j] = g;
-  }
-  __syncthreads();
-  for (int j = threadIdx.x; j < n_psf; j += W * 64) {
-    const int gj = lead[j];
-    int pos = 0;
-    for (int i = 0; i < n_psf; ++i) {
-      const int gi = lead[i];
-      pos += (gi < gj || (gi == gj && i < j)) ? 1 : 0;
-    }
-    l.order[pos] = j;
-  }
-  __syncthreads();
-}
-
 template <int W>
 __device__ int cull_tile(const FpxPatch *__restrict__ P, int pstart, int n_src, int n_psf,
                          const Box &box, const Lds &l, int wave, int lane) {
   const int n = n_src * n_psf;
   int *index = l.index;
   __syncthreads();  // previous users of the index are done; geometry is in LDS
-  // position q = s * n_psf + j holds component k = order[j] of source s
-  for (int q = threadIdx.x; q < n; q += W * 64) {
-    const int s = q / n_psf;
-    const int k = l.order[q - s * n_psf];
-    const FpxPSFSourceGaussian psf = P->psfgauss[pstart + k];
-    index[q] = gauss_reaches(load_geometry(l.geo + s * kGeoFloats), psf, bin_covar(l, psf),
-                             box.x0, box.x1, box.y0, box.y1) ? s * n_psf + k : -1;
+  for (int lg = threadIdx.x; lg < n; lg += W * 64) {
+    const int s = lg / n_psf;
+    const FpxPSFSourceGaussian psf = P->psfgauss[pstart + lg - s * n_psf];
+    index[lg] = gauss_reaches(load_geometry(l.geo + s * kGeoFloats), psf, bin_covar(l, psf),
+                              box.x0, box.x1, box.y0, box.y1) ? 1 : 0;
   }
   __syncthreads();
   if (wave == 0) {
     int count = 0;
     for (int b0 = 0; b0 < n; b0 += 64) {
-      const int q = b0 + lane;
-      const int lg = q < n ? index[q] : -1;
-      const bool keep = lg >= 0;
+      const int lg = b0 + lane;
+      const bool keep = lg < n && index[lg] != 0;
       const unsigned long long m = __builtin_amdgcn_ballot_w64(keep);
-      // every lane has read its entry above; writes land at or below b0 + lane
+      // every lane has read its flag above; writes land at or below b0 + lane
       if (keep) index[count + lanes_below(m)] = lg;
       count += __builtin_popcountll(m);
     }
@@ -278,8 +227,10 @@ __device__ void build_records(const FpxPatch *__restrict__ P,
 }
 
 // The pixel offsets from a Gaussian centre at a lane's PPL pixels and their
-// products.  Records come in runs of one centre (psf_order), so the passes
-// compute these only when the centre changes.
+// products.  The passes compute these only when the centre changes from one
+// record to the next: all radius bins of a PSF component share its centre,
+// and the patch packer orders each exposure's PSF entries by centre
+// (patches/patch.py), so a source's records come in runs of one centre.
 template <int PPL>
 __device__ __forceinline__ void offsets_at_pixels(const float4 r0, const float (&xp)[PPL],
                                                   const float (&yp)[PPL], float (&dx)[PPL],
@@ -545,7 +496,6 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
     box.y1 = uniform_f(wave_max(y1));
   }
 
-  psf_order<W>(P, pstart, n_psf, l);
   const int n_surv = cull_tile<W>(P, pstart, n_src, n_psf, box, l, wave, lane);
   const int n_chunks = (n_surv + gauss_cap - 1) / gauss_cap;
   auto chunk_size = [&](int c) { return min(gauss_cap, n_surv - c * gauss_cap); };
@@ -959,9 +909,6 @@ int launch_lds(DeviceState *st, int cfg, int max_gauss, size_t *lds, int *cap, i
   }
   *cap = pick_gauss_cap(kc, max_gauss, st->lds_max, static_lds);
   if (*cap < 1) return set_err(FPX_E_ARG, "max_gauss too large for the LDS of one workgroup");
-  // the record area doubles as psf_order's scratch of 3 n_psf words
-  if ((long)*cap * fpx::kRec < 3L * std::min(max_gauss, fpx::kPsfOrder))
-    return set_err(FPX_E_ARG, "max_gauss too large for the LDS of one workgroup");
   *index_cap = max_gauss;
   *lds = fpx::lds_bytes_for(kc.waves, kc.ppl, *cap, *index_cap);
   return FPX_OK;

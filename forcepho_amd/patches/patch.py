@@ -10,7 +10,8 @@ Gaussians ordered [exposure][source][n_psf_per_source]).
 
 import numpy as np
 
-__all__ = ["PatchBase", "Patch", "PSF_COLS", "PSF_DTYPE", "LinearWCS", "scale_at_sky"]
+__all__ = ["PatchBase", "Patch", "PSF_COLS", "PSF_DTYPE", "LinearWCS", "scale_at_sky",
+           "group_by_centre"]
 
 # PSF Gaussian record, forcepho/patches/storage.py:27-28
 PSF_COLS = ["amp", "xcen", "ycen", "Cxx", "Cyy", "Cxy"]
@@ -147,6 +148,21 @@ class PatchBase:
         return n
 
 
+def group_by_centre(block):
+    """PSF entries reordered so that entries with the same centre (xcen,
+    ycen) are adjacent: groups in order of their first entry, entries within
+    a group in their original order (a stable permutation).  A store lists
+    entries radius bin by radius bin; every bin of a PSF component shares
+    the component's centre."""
+    block = np.asarray(block)
+    if len(block) < 2:
+        return block.copy()
+    key = np.stack([block["xcen"].astype(np.float32), block["ycen"].astype(np.float32)], axis=1)
+    _, first, inverse = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    lead = first[np.asarray(inverse).reshape(-1)]
+    return block[np.argsort(lead, kind="stable")]
+
+
 class LinearWCS:
     """A local linear world<->pixel map of one exposure: pixel = crpix +
     CW (sky - crval_sky), CW = 3600 D diag(cos dec0, 1), D = (3600 CD)^-1.
@@ -247,7 +263,10 @@ class Patch(PatchBase):
     def _pack_psf(self, scene):
         """psfgauss ordered [exposure][source][n_psf_per_source(band)]
         (forcepho/patches/patch.py:359-398).  Every source of an exposure gets
-        that exposure's PSF block."""
+        that exposure's PSF block, its entries grouped by centre
+        (``group_by_centre``): the kernel's sum over entries does not depend on
+        their order, and it computes pixel offsets once per run of entries
+        with one centre."""
         n_psf = np.array([len(self.psfs[self.band_start[b]]) for b in range(self.n_bands)],
                          dtype=np.int32)
         self.n_psf_per_source = n_psf
@@ -258,7 +277,7 @@ class Patch(PatchBase):
         s = 0
         for e in range(self.n_exp):
             self.psfgauss_start[e] = s
-            block = np.asarray(self.psfs[e], dtype=PSF_DTYPE)
+            block = group_by_centre(np.asarray(self.psfs[e], dtype=PSF_DTYPE))
             assert len(block) == per_exp[e], "PSF size must be constant within a band"
             for _ in range(self.n_sources):
                 self.psfgauss[s:s + len(block)] = block

@@ -99,3 +99,24 @@ def test_n_evals_counts_real_pixels(packed):
     for e in range(patch.n_exp):
         n += patch.original_sizes[e] * patch.n_sources * patch.n_psf_per_source[patch.bands[e]]
     assert patch.n_evals() == n
+
+
+def test_group_by_centre_is_a_stable_grouping():
+    """PSF entries with one centre become adjacent, groups in order of first
+    appearance, entries within a group in their original order; the block is
+    a permutation of the input."""
+    from forcepho_amd.patches.patch import group_by_centre, PSF_DTYPE
+    rng = np.random.default_rng(4)
+    n_comp, n_bins = 5, 7
+    b = np.zeros(n_comp * n_bins, dtype=PSF_DTYPE)
+    cx, cy = rng.normal(size=n_comp), rng.normal(size=n_comp)
+    comp = np.tile(np.arange(n_comp), n_bins)            # bin-major, as a store lists them
+    b["xcen"], b["ycen"] = cx[comp], cy[comp]
+    b["sersic_bin"] = np.repeat(np.arange(n_bins), n_comp)
+    b["amp"] = np.arange(len(b))
+    g = group_by_centre(b)
+    assert sorted(g["amp"].tolist()) == list(range(len(b)))
+    assert (g["xcen"].reshape(n_comp, n_bins) == cx[:, None].astype(np.float32)).all()
+    assert (g["sersic_bin"].reshape(n_comp, n_bins) == np.arange(n_bins)).all()
+    # already grouped input is unchanged
+    assert np.array_equal(group_by_centre(g), g)

@@ -55,7 +55,9 @@ def test_fits_patch_packing():
     np.testing.assert_allclose(patch.crpix[0, 0], [31.0, 15.5], atol=1e-4)
     np.testing.assert_allclose(patch.D[0, 0], np.eye(2) / 0.03, rtol=1e-6)
     assert patch.n_psf_per_source[0] == 36 and patch.n_radii == 9
-    assert (patch.psfgauss["sersic_bin"] == np.repeat(np.arange(9), 4)).all()
+    # entries grouped by centre: each PSF component's 9 radius bins together
+    assert (patch.psfgauss["sersic_bin"] == np.tile(np.arange(9), 4)).all()
+    assert (np.diff(patch.psfgauss["xcen"].reshape(4, 9), axis=1) == 0).all()
     assert scene.sources[0].sersic == 2.2 and scene.sources[0].rhalf == 0.1
 
 
