@@ -21,6 +21,11 @@ import numpy as np
 from .kernel_limits import MAXBANDS, MAXRADII, MAXSOURCES, NPARAMS
 from ._lib import lib, check, SLOT_DTYPE
 
+# device output block of a proposer: pchi2 (double[MAXBANDS]) then the
+# Responses (float[MAXBANDS][MAXSOURCES][NPARAMS])
+_CHI2_BYTES = MAXBANDS * 8
+_OUT_BYTES = _CHI2_BYTES + MAXBANDS * MAXSOURCES * NPARAMS * 4
+
 __all__ = ["ProposerBase", "HIPProposer", "GPUProposer", "Proposer",
            "source_struct_dtype", "response_struct_dtype", "DEFAULT_CONFIG"]
 
@@ -84,8 +89,11 @@ class HIPProposer(ProposerBase):
         n_acc = lib().fpx_accum_doubles(MAXBANDS)
         self._accum = torch.zeros(n_acc, dtype=torch.float64, device=dev)
         self._ticket = torch.zeros(2, dtype=torch.int32, device=dev)
-        self._chi2 = torch.zeros(MAXBANDS, dtype=torch.float64, device=dev)
-        self._resp = torch.zeros(MAXBANDS * MAXSOURCES * NPARAMS, dtype=torch.float32, device=dev)
+        # outputs back to back (chi2 doubles, then the Responses) so that one
+        # copy brings them to the host
+        self._out = torch.zeros(_OUT_BYTES, dtype=torch.uint8, device=dev)
+        self._chi2 = self._out[:_CHI2_BYTES].view(torch.float64)
+        self._resp = self._out[_CHI2_BYTES:].view(torch.float32)
         self._prop = torch.zeros(MAXSOURCES * source_struct_dtype.itemsize, dtype=torch.uint8,
                                  device=dev)
         self._slot = torch.zeros(SLOT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
@@ -93,8 +101,10 @@ class HIPProposer(ProposerBase):
         self._queue = torch.zeros(2, dtype=torch.int32, device=dev)
         pin = dict(pin_memory=True)
         self._prop_h = torch.zeros(self._prop.numel(), dtype=torch.uint8, **pin)
-        self._chi2_h = torch.zeros(MAXBANDS, dtype=torch.float64, **pin)
-        self._resp_h = torch.zeros(self._resp.numel(), dtype=torch.float32, **pin)
+        self._prop_np = self._prop_h.numpy()
+        self._out_h = torch.zeros(_OUT_BYTES, dtype=torch.uint8, **pin)
+        self._chi2_np = self._out_h.numpy()[:_CHI2_BYTES].view(np.float64)
+        self._resp_np = self._out_h.numpy()[_CHI2_BYTES:].view(np.float32)
         self._slot_key = None
         self._h2d_done = None   # event of the last copy out of the pinned proposal buffer
 
@@ -176,14 +186,14 @@ class HIPProposer(ProposerBase):
             self._slot_key = key
         return n_tiles
 
-    def launch(self, proposal, patch, pixel_weight=None, outputs=True):
+    def launch(self, proposal, patch, pixel_weight=None, outputs=True, stream=None):
         """Queue one evaluation on the current stream (no synchronisation).
         ``proposal`` is a numpy source_struct_dtype array or a device uint8
         tensor already holding the records.  ``outputs=False`` launches only
         the tile kernel (fpx_evaluate_tiles): its sums stay in the
         accumulator until a launch with outputs (or ``finalize``)."""
-        n_tiles = self._load(proposal, patch, pixel_weight)
-        stream = self._stream()
+        stream = stream or self._stream()
+        n_tiles = self._load(proposal, patch, pixel_weight, stream)
         if not outputs:
             check(lib().fpx_evaluate_tiles(self._slot.data_ptr(), 1, None, n_tiles,
                                            patch.max_gauss(), self.config, int(self.persistent),
@@ -271,7 +281,7 @@ class HIPProposer(ProposerBase):
         patch.reorder_tiles(self.pix_per_tile, perm)
         return perm
 
-    def _load(self, proposal, patch, pixel_weight):
+    def _load(self, proposal, patch, pixel_weight, stream=None):
         """Stage the proposal and the slot for ``patch``; returns the tile
         count.  A numpy proposal is copied through pinned memory into the
         proposer's buffer; a device tensor is read in place by the launch (it
@@ -286,10 +296,10 @@ class HIPProposer(ProposerBase):
             raw = np.ascontiguousarray(proposal, dtype=source_struct_dtype).view(np.uint8)
             if self._h2d_done is not None:
                 self._h2d_done.synchronize()   # the pinned buffer may still be in flight
-            self._prop_h[:raw.size].copy_(torch.from_numpy(raw.reshape(-1)))
-            self._prop[:raw.size].copy_(self._prop_h[:raw.size], non_blocking=True)
+            self._prop_np[:raw.size] = raw.reshape(-1)
+            self._prop.copy_(self._prop_h, non_blocking=True)
             self._h2d_done = torch.cuda.Event()
-            self._h2d_done.record(self._stream())
+            self._h2d_done.record(stream or self._stream())
             return self._slot_for(patch, pixel_weight)
         if proposal.device != self.device or not proposal.is_contiguous():
             raise ValueError("device proposals must be contiguous tensors on the proposer's device")
@@ -305,14 +315,12 @@ class HIPProposer(ProposerBase):
         if verbose:
             print(f"fpx launch: config {self.config}, {n_bands} bands, {n_sources} sources",
                   file=sys.stderr, flush=True)
-        self.launch(proposal, patch)
         stream = self._stream()
-        self._chi2_h[:n_bands].copy_(self._chi2[:n_bands], non_blocking=True)
-        nr = n_bands * MAXSOURCES * NPARAMS
-        self._resp_h[:nr].copy_(self._resp[:nr], non_blocking=True)
+        self.launch(proposal, patch, stream=stream)
+        self._out_h.copy_(self._out, non_blocking=True)
         stream.synchronize()
-        chi_out = self._chi2_h[:n_bands].numpy().astype(self.chi_dtype, copy=True)
-        derivs = self._resp_h[:nr].numpy().reshape(n_bands, MAXSOURCES, NPARAMS)
+        chi_out = self._chi2_np[:n_bands].astype(self.chi_dtype, copy=True)
+        derivs = self._resp_np[:n_bands * MAXSOURCES * NPARAMS].reshape(n_bands, MAXSOURCES, NPARAMS)
         chi2_derivs = derivs[:, :n_sources].astype(np.float64)
         if patch.return_residual:
             residuals = patch.retrieve_array("residual")
@@ -333,11 +341,12 @@ class HIPProposer(ProposerBase):
         if pixel_weight.dtype.itemsize != 4 or pixel_weight.numel() < patch.npix:
             raise ValueError("pixel_weight must be float32 over all packed pixels")
         n_bands, n_sources = int(patch.n_bands), int(patch.n_sources)
-        self.launch(proposal, patch, pixel_weight=pixel_weight)
+        stream = self._stream()
+        self.launch(proposal, patch, pixel_weight=pixel_weight, stream=stream)
         nr = n_bands * MAXSOURCES * NPARAMS
-        self._resp_h[:nr].copy_(self._resp[:nr], non_blocking=True)
-        self._stream().synchronize()
-        d = self._resp_h[:nr].numpy().reshape(n_bands, MAXSOURCES, NPARAMS)
+        self._out_h.copy_(self._out, non_blocking=True)
+        stream.synchronize()
+        d = self._resp_np[:nr].reshape(n_bands, MAXSOURCES, NPARAMS)
         return d[:, :n_sources].astype(np.float64)
 
 
