@@ -200,9 +200,11 @@ def single_plain_measurement(args):
 
 def autograd_measurement(args, patch, scene):
     """The multi-band patch behind torch.autograd (PatchModel): chi2 value +
-    gradient calls per second, and one leapfrog trajectory of
-    --leapfrog-steps steps driven by the autograd gradient (hmc.leapfrog, the
-    inner integrator of forcepho's HMC), in unit-scaled parameters."""
+    gradient calls per second (the fused chi2, one chi2-mode launch, and the
+    chi2 built by torch from the differentiable residual image), and one
+    leapfrog trajectory of --leapfrog-steps steps driven by the fused
+    gradient (hmc.leapfrog, the inner integrator of forcepho's HMC), in
+    unit-scaled parameters."""
     import torch
     from forcepho_amd.autograd import PatchModel
     from forcepho_amd.hmc import leapfrog
@@ -216,29 +218,34 @@ def autograd_measurement(args, patch, scene):
     q0 = theta / units
     q0.reshape(-1, nb + 6)[:, nb + 2] *= 0.99
 
-    def U_and_grad(q):
+    def U_and_grad(q, chi2=model.chi2):
         t = torch.tensor(q * units, dtype=torch.float64, device=model.proposer.device,
                          requires_grad=True)
-        u = 0.5 * model.chi2(t)
+        u = 0.5 * chi2(t)
         u.backward()
         return u.item(), t.grad.cpu().numpy() * units
 
-    for _ in range(3):
-        U_and_grad(q0)
-    n = 20
-    t0 = time.perf_counter()
-    for _ in range(n):
-        U_and_grad(q0)
-    call_s = (time.perf_counter() - t0) / n
+    def per_call(chi2):
+        for _ in range(3):
+            U_and_grad(q0, chi2)
+        n = 20
+        t0 = time.perf_counter()
+        for _ in range(n):
+            U_and_grad(q0, chi2)
+        return (time.perf_counter() - t0) / n
+
+    call_s = per_call(model.chi2)
+    resid_s = per_call(model.chi2_from_residual)
     p0 = np.random.default_rng(args.seed).normal(size=len(q0))
     leapfrog(q0, p0, U_and_grad, eps=0.01, n_steps=2)
     t0 = time.perf_counter()
     _, _, _, calls = leapfrog(q0, p0, U_and_grad, eps=0.01, n_steps=args.leapfrog_steps)
     traj_s = time.perf_counter() - t0
     n_evals = patch.n_evals()
-    return {"workload": "torch.autograd chi2 of the multi-band patch (forward: residual image, "
-                        "backward: hand-written gradient pass)",
+    return {"workload": "torch.autograd chi2 of the multi-band patch (fused: one chi2-mode "
+                        "launch gives value and gradient)",
             "value_and_grad_per_s": 1.0 / call_s, "ms_per_value_and_grad": call_s * 1e3,
+            "residual_path_ms_per_value_and_grad": resid_s * 1e3,
             "evals_per_s": n_evals / call_s,
             "leapfrog_steps": args.leapfrog_steps, "gradient_calls": calls,
             "trajectory_s": traj_s, "trajectories_per_s": 1.0 / traj_s}

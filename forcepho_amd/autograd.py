@@ -9,10 +9,14 @@ it recomputes every ImageGaussian and its moments from the proposal and
 returns dL/dtheta, never storing per-pixel intermediates.  The theano Ops
 of forcepho/model.py:515-551 played the same role for pymc3.
 
-``PatchModel.chi2(theta)`` builds the reference's chi2 convention
-sum (r ierr)^2 - (d ierr)^2 (compute_gaussians_kernel.cc:286-293) on top,
-so ``-0.5 * chi2`` is forcepho's log-likelihood and its autograd gradient
-matches FastPosterior.lnprob_grad in the untransformed space.
+``PatchModel.chi2(theta)`` is the reference's chi2 convention
+sum (r ierr)^2 - (d ierr)^2 (compute_gaussians_kernel.cc:286-293), so
+``-0.5 * chi2`` is forcepho's log-likelihood and its autograd gradient
+matches FastPosterior.lnprob_grad in the untransformed space.  It takes the
+value and the gradient from one chi2-mode launch, as the reference's
+FastPosterior.evaluate does (model.py:302-340), and with device-built records replays
+one recorded graph per call; ``chi2_from_residual`` builds the same chi2 in
+torch on top of the residual image.
 
 When the scene's amplitude model can be evaluated on the device (the
 analytic LogNormalMixtureAmplitudes, or forcepho's spline tables shared by
@@ -26,7 +30,7 @@ import numpy as np
 
 from .model import stack_grad
 from .device_scene import DeviceRecords, device_amplitude_models_ok, stack_grad_device
-from .kernel_limits import MAXSOURCES
+from .kernel_limits import MAXBANDS, MAXSOURCES
 
 __all__ = ["PatchModel", "PatchResidual"]
 
@@ -45,9 +49,11 @@ class PatchModel:
             residual buffer)
     scene : Scene whose active sources map onto the patch's sources
     proposer : HIPProposer (created if None)
+    use_graph : with device-built records, ``chi2`` replays one recorded
+                graph (record build, chi2 launch, gradient stacking) per call
     """
 
-    def __init__(self, patch, scene, proposer=None):
+    def __init__(self, patch, scene, proposer=None, use_graph=True):
         torch = _torch()
         from .proposal import HIPProposer
         self.patch, self.scene = patch, scene
@@ -69,6 +75,9 @@ class PatchModel:
             # fixed buffers that the launches read in place (their device
             # slots are then built once)
             self.backward_records = torch.zeros_like(buf)
+        self.use_graph = bool(use_graph) and self.records is not None
+        self._graph = None
+        self._graph_key = None
 
     def device_proposal(self, theta, copy=True):
         """Source records of theta (any device, float64 [ndim]) as a uint8
@@ -77,6 +86,58 @@ class PatchModel:
         t = theta.detach().to(self.records.records.device, dtype=self.records.lnr.dtype)
         self.records.write(t.reshape(1, self.n_sources, self.n_bands + 6))
         return self.records.records.clone() if copy else self.records.records
+
+    def _chi2_and_grad(self, theta):
+        """Queue the record build, the chi2-mode launch and the gradient
+        stacking of theta (float64 device tensor [ndim]); returns the chi2
+        (0-d) and dchi2/dtheta [ndim] as float64 tensors of this call."""
+        p = self.proposer
+        p.launch(self.device_proposal(theta, copy=False), self.patch)
+        resp = p._resp.view(1, MAXBANDS, MAXSOURCES, 7)
+        grad = stack_grad_device(resp, self.n_sources, self.n_bands)[0].reshape(-1)
+        return p._chi2[:self.n_bands].sum(), grad
+
+    def _graph_state(self):
+        """What the recorded graph depends on: the device patch struct, the
+        tile table and the kernel configuration."""
+        p = self.proposer
+        tiles = self.patch.device_tiles(p.pix_per_tile)[0]
+        return (self.patch.device_patch.data_ptr(), tiles.data_ptr(), p.config, p.persistent)
+
+    def _capture(self, theta, key):
+        torch = _torch()
+        dev = self.proposer.device
+        self._graph = None
+        self._graph_theta = theta.detach().to(dev, torch.float64).clone()
+        side = torch.cuda.Stream(device=dev)
+        side.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(side):
+            self._chi2_and_grad(self._graph_theta)   # builds the slot outside the graph
+        torch.cuda.current_stream(dev).wait_stream(side)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            self._graph_out = self._chi2_and_grad(self._graph_theta)
+        # buffers the graph reads by address stay referenced with it
+        self._graph_refs = (self.proposer._slot, self.patch.device_tiles(self.proposer.pix_per_tile)[0])
+        self._graph, self._graph_key = graph, key
+
+    def chi2_and_grad(self, theta):
+        """chi2 and dchi2/dtheta of theta (float64 tensor [ndim]) as new
+        float64 device tensors, from one chi2-mode launch; no host
+        synchronisation."""
+        torch = _torch()
+        if not self.use_graph:
+            theta = theta.detach().to(self.proposer.device, torch.float64)
+            value, grad = self._chi2_and_grad(theta)
+            return value, grad
+        key = self._graph_state()
+        if self._graph is None or key != self._graph_key:
+            self._capture(theta, key)
+        else:
+            self._graph_theta.copy_(theta.detach())
+        self._graph.replay()
+        value, grad = self._graph_out
+        return value.clone(), grad.clone()
 
     def proposal(self, theta):
         self.scene.set_all_source_params(np.asarray(theta, dtype=np.float64))
@@ -102,7 +163,15 @@ class PatchModel:
         return PatchResidual.apply(theta, self)
 
     def chi2(self, theta):
-        """Differentiable sum over pixels of (r ierr)^2 - (d ierr)^2 (float64)."""
+        """Differentiable sum over pixels of (r ierr)^2 - (d ierr)^2 (float64).
+        One launch of the chi2 kernel computes the value and its gradient
+        together (as ``evaluate_proposal`` does); backward scales the saved
+        gradient.  Equal to ``chi2_from_residual``."""
+        return PatchChi2.apply(theta, self)
+
+    def chi2_from_residual(self, theta):
+        """The same chi2 built by torch from the differentiable residual
+        image (forward: model pass; backward: the kernel's gradient pass)."""
         r = self.residual(theta).double()
         return ((r * self.ierr) ** 2 - (self.data * self.ierr) ** 2).sum()
 
@@ -118,6 +187,46 @@ class PatchResidual:
         if cls._fn is None:
             cls._fn = _make_function()
         return cls._fn.apply(theta, model)
+
+
+class PatchChi2:
+    """Namespace of the chi2 autograd.Function: value and gradient from one
+    kernel launch."""
+
+    _fn = None
+
+    @classmethod
+    def apply(cls, theta, model):
+        if cls._fn is None:
+            cls._fn = _make_chi2_function()
+        return cls._fn.apply(theta, model)
+
+
+def _make_chi2_function():
+    torch = _torch()
+
+    class _Chi2(torch.autograd.Function):
+
+        @staticmethod
+        def forward(ctx, theta, model):
+            p = model.proposer
+            nb, ns = model.n_bands, model.n_sources
+            if model.records is not None:
+                value, g = model.chi2_and_grad(theta)
+            else:
+                chi2, d = p.evaluate_proposal(model.proposal(theta.detach().cpu().numpy()),
+                                              patch=model.patch)[:2]
+                g = torch.from_numpy(stack_grad(d, ns, nb)).to(p.device)
+                value = torch.tensor(float(np.sum(chi2)), dtype=torch.float64, device=p.device)
+            ctx.save_for_backward(g.to(dtype=theta.dtype, device=theta.device))
+            return value.to(dtype=torch.float64, device=theta.device)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            (g,) = ctx.saved_tensors
+            return grad_out.to(g.dtype) * g, None
+
+    return _Chi2
 
 
 def _make_function():

@@ -1,6 +1,7 @@
-"""torch.autograd over the patch model: backward (the kernel's gradient pass
-with dL/dresidual as pixel weights) against the chi2-mode gradient, the
-float64 oracle and central differences; a 32-step leapfrog driven by it."""
+"""torch.autograd over the patch model: the residual image's backward (the
+kernel's gradient pass with dL/dresidual as pixel weights) and the fused chi2
+(value and gradient of one chi2-mode launch) against each other, the float64
+oracle and central differences; a 32-step leapfrog driven by them."""
 
 import numpy as np
 import pytest
@@ -37,13 +38,19 @@ def test_autograd_matches_kernel_gradient_and_fp64():
     model, theta = _model(side=32)
     th = _offset(theta, 8)
     t = torch.tensor(th, dtype=torch.float64, requires_grad=True)
-    chi2 = model.chi2(t)
+    chi2 = model.chi2_from_residual(t)
     chi2.backward()
     g_auto = t.grad.numpy()
+    t2 = torch.tensor(th, dtype=torch.float64, requires_grad=True)
+    fused = model.chi2(t2)
+    (3.0 * fused).backward()
+    g_fused = t2.grad.numpy() / 3.0
     # the chi2-mode kernel gradient of the same proposal
     _, c_band, d = model.forward_residual(model.proposal(th))
     g_kernel = stack_grad(d, model.n_sources, model.n_bands)
     np.testing.assert_allclose(chi2.item(), c_band.sum(), rtol=1e-6)
+    np.testing.assert_allclose(fused.item(), c_band.sum(), rtol=1e-12)
+    np.testing.assert_allclose(g_fused, g_kernel, rtol=1e-12, atol=0)
     # float64 reference of both
     c64, g64, mass, smass = U.fp64_reference(model.patch, model.proposal(th))
     g64s = stack_grad(g64, model.n_sources, model.n_bands)
@@ -52,6 +59,7 @@ def test_autograd_matches_kernel_gradient_and_fp64():
     tol = U.GRAD_RTOL * np.abs(g64s) + U.GRAD_FLOOR * ms + U.MODEL_EPS * ss
     assert np.all(np.abs(g_auto - g64s) <= tol)
     assert np.all(np.abs(g_kernel - g64s) <= tol)
+    assert np.all(np.abs(g_fused - g64s) <= tol)
 
 
 def test_autograd_central_differences_full_size():
@@ -71,6 +79,9 @@ def test_autograd_central_differences_full_size():
     chi2 = ((r.double() * model.ierr) ** 2 - (model.data * model.ierr) ** 2).sum()
     chi2.backward()
     g = t.grad.numpy()
+    t2 = torch.tensor(th, dtype=torch.float64, requires_grad=True)
+    model.chi2(t2).backward()
+    g_fused = t2.grad.numpy()
     rr = r.detach().double() * model.ierr
     mm = (model.data - r.detach().double()) * model.ierr
     noise = 4.0 * 2.0 ** -24 * float(torch.sqrt(((2 * rr.abs() * mm.abs()) ** 2).sum()))
@@ -88,6 +99,7 @@ def test_autograd_central_differences_full_size():
             fd = (model.chi2(torch.tensor(tp)).item() - model.chi2(torch.tensor(tm)).item()) / (2 * h * units[i])
         tol = 2e-2 * abs(g[i]) + 2 * noise / (2 * h * units[i])
         assert abs(fd - g[i]) <= tol, (i, fd, g[i], tol)
+        assert abs(fd - g_fused[i]) <= tol, (i, fd, g_fused[i], tol)
 
 
 def test_leapfrog_trajectory_conserves_energy():
@@ -152,3 +164,45 @@ def test_device_records_match_host_proposal():
     np.testing.assert_allclose(grads[0][0], grads[1][0], rtol=1e-9)
     g = grads[1][1]
     np.testing.assert_allclose(grads[0][1], g, rtol=1e-4, atol=1e-5 * np.abs(g).max())
+
+
+def test_chi2_graph_replays_match_eager():
+    """The recorded chi2 graph (record build, chi2 launch, gradient
+    stacking) replayed for several parameter vectors, with eager launches of
+    the same proposer between the replays, gives the eager chi2 and gradient
+    of each; a change of kernel configuration records the graph again."""
+    import torch
+    model, theta = _model(side=48, n_bands=4, n_sources=6)
+    nb = 4
+    rng = np.random.default_rng(5)
+    thetas = []
+    for k in range(4):
+        t = _offset(theta, nb).reshape(-1, nb + 6)
+        t[:, :nb] *= 1 + 0.05 * rng.normal(size=(len(t), nb))
+        t[:, nb + 2] = np.clip(t[:, nb + 2] * (1 + 0.05 * rng.normal(size=len(t))), 0.2, 0.99)
+        thetas.append(t.reshape(-1))
+
+    def run(th):
+        v, g = model.chi2_and_grad(torch.tensor(th, dtype=torch.float64, device="cuda"))
+        return v.item(), g.cpu().numpy()
+
+    def same(a, b):
+        np.testing.assert_allclose(a[0], b[0], rtol=1e-12)
+        np.testing.assert_allclose(a[1], b[1], rtol=1e-6, atol=1e-6 * np.abs(b[1]).max())
+
+    model.use_graph = False
+    eager = [run(th) for th in thetas]
+    model.use_graph = True
+    for _ in range(2):
+        for th, ref in zip(thetas, eager):
+            got = run(th)
+            model.forward_residual(model.proposal(thetas[0]))     # eager work between replays
+            same(got, ref)
+    graph = model._graph
+    cfg = model.proposer.config
+    model.proposer.set_config(2 if cfg != 2 else 1)
+    got = run(thetas[1])
+    assert model._graph is not graph
+    model.use_graph = False
+    same(got, run(thetas[1]))
+    model.proposer.set_config(cfg)
