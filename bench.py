@@ -201,10 +201,10 @@ def single_plain_measurement(args):
 def autograd_measurement(args, patch, scene):
     """The multi-band patch behind torch.autograd (PatchModel): chi2 value +
     gradient calls per second (the fused chi2, one chi2-mode launch, and the
-    chi2 built by torch from the differentiable residual image), and one
-    leapfrog trajectory of --leapfrog-steps steps driven by the fused
-    gradient (hmc.leapfrog, the inner integrator of forcepho's HMC), in
-    unit-scaled parameters."""
+    chi2 built by torch from the differentiable residual image: a model-only
+    launch forward, the gradient pass backward), and a leapfrog trajectory
+    of --leapfrog-steps steps driven by each (hmc.leapfrog, the inner
+    integrator of forcepho's HMC), in unit-scaled parameters."""
     import torch
     from forcepho_amd.autograd import PatchModel
     from forcepho_amd.hmc import leapfrog
@@ -237,15 +237,22 @@ def autograd_measurement(args, patch, scene):
     call_s = per_call(model.chi2)
     resid_s = per_call(model.chi2_from_residual)
     p0 = np.random.default_rng(args.seed).normal(size=len(q0))
-    leapfrog(q0, p0, U_and_grad, eps=0.01, n_steps=2)
-    t0 = time.perf_counter()
-    _, _, _, calls = leapfrog(q0, p0, U_and_grad, eps=0.01, n_steps=args.leapfrog_steps)
-    traj_s = time.perf_counter() - t0
+
+    def trajectory(chi2):
+        f = lambda q: U_and_grad(q, chi2)
+        leapfrog(q0, p0, f, eps=0.01, n_steps=2)
+        t0 = time.perf_counter()
+        _, _, _, calls = leapfrog(q0, p0, f, eps=0.01, n_steps=args.leapfrog_steps)
+        return time.perf_counter() - t0, calls
+
+    traj_s, calls = trajectory(model.chi2)
+    traj_resid_s, _ = trajectory(model.chi2_from_residual)
     n_evals = patch.n_evals()
     return {"workload": "torch.autograd chi2 of the multi-band patch (fused: one chi2-mode "
                         "launch gives value and gradient)",
             "value_and_grad_per_s": 1.0 / call_s, "ms_per_value_and_grad": call_s * 1e3,
             "residual_path_ms_per_value_and_grad": resid_s * 1e3,
+            "residual_path_trajectory_s": traj_resid_s,
             "evals_per_s": n_evals / call_s,
             "leapfrog_steps": args.leapfrog_steps, "gradient_calls": calls,
             "trajectory_s": traj_s, "trajectories_per_s": 1.0 / traj_s}

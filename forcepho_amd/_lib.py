@@ -12,7 +12,7 @@ import numpy as np
 
 from .build import LIB_PATH
 
-__all__ = ["lib", "check", "FpxError", "TILE_DTYPE", "SLOT_DTYPE",
+__all__ = ["lib", "check", "FpxError", "TILE_DTYPE", "SLOT_DTYPE", "SLOT_MODEL_ONLY",
            "STRUCT_PATCH", "STRUCT_SOURCE", "STRUCT_RESPONSE", "STRUCT_PSFGAUSS",
            "STRUCT_TILE", "STRUCT_SLOT", "EXPORTED_SYMBOLS"]
 
@@ -26,6 +26,8 @@ SLOT_DTYPE = np.dtype([("patch", np.uintp), ("proposal", np.uintp), ("tiles", np
                        ("pdchi2_dp", np.uintp), ("accum", np.uintp), ("ticket", np.uintp),
                        ("n_bands", np.int32), ("n_tiles", np.int32),
                        ("flags", np.int32), ("reserved", np.int32)], align=True)
+
+SLOT_MODEL_ONLY = 1    # FpxSlot.flags: model pass only (FPX_SLOT_MODEL_ONLY)
 
 _c_int, _c_size, _p = ctypes.c_int32, ctypes.c_size_t, ctypes.c_void_p
 

@@ -49,8 +49,10 @@ class PatchModel:
             residual buffer)
     scene : Scene whose active sources map onto the patch's sources
     proposer : HIPProposer (created if None)
-    use_graph : with device-built records, ``chi2`` replays one recorded
-                graph (record build, chi2 launch, gradient stacking) per call
+    use_graph : with device-built records, each device call (chi2 and its
+                gradient, the residual image, the backward pass) replays a
+                graph recorded on its first use: record build, kernel launch
+                and gradient stacking in one submission
     """
 
     def __init__(self, patch, scene, proposer=None, use_graph=True):
@@ -67,17 +69,16 @@ class PatchModel:
         self.ierr = torch.from_numpy(np.asarray(patch.ierr, np.float64)).to(dev)
         self.n_sources, self.n_bands = int(patch.n_sources), int(patch.n_bands)
         self.ndim = self.n_sources * (self.n_bands + 6)
+        self._data_chi2 = ((self.data * self.ierr) ** 2).sum()
         self.records = None
         if device_amplitude_models_ok(scene):
+            # the launches read this buffer in place (their device slots are
+            # then built once)
             buf = torch.zeros(MAXSOURCES * 264, dtype=torch.uint8, device=dev)
             self.records = DeviceRecords([scene], [self.n_bands], buf, self.n_sources,
                                          self.n_bands)
-            # fixed buffers that the launches read in place (their device
-            # slots are then built once)
-            self.backward_records = torch.zeros_like(buf)
         self.use_graph = bool(use_graph) and self.records is not None
-        self._graph = None
-        self._graph_key = None
+        self._graphs = {}
 
     def device_proposal(self, theta, copy=True):
         """Source records of theta (any device, float64 [ndim]) as a uint8
@@ -87,57 +88,86 @@ class PatchModel:
         self.records.write(t.reshape(1, self.n_sources, self.n_bands + 6))
         return self.records.records.clone() if copy else self.records.records
 
-    def _chi2_and_grad(self, theta):
-        """Queue the record build, the chi2-mode launch and the gradient
-        stacking of theta (float64 device tensor [ndim]); returns the chi2
-        (0-d) and dchi2/dtheta [ndim] as float64 tensors of this call."""
-        p = self.proposer
-        p.launch(self.device_proposal(theta, copy=False), self.patch)
-        resp = p._resp.view(1, MAXBANDS, MAXSOURCES, 7)
-        grad = stack_grad_device(resp, self.n_sources, self.n_bands)[0].reshape(-1)
-        return p._chi2[:self.n_bands].sum(), grad
+    # -- device calls (records built on the device) ---------------------------
+
+    def _grad(self, sign=1.0):
+        resp = self.proposer._resp.view(1, MAXBANDS, MAXSOURCES, 7)
+        g = stack_grad_device(resp, self.n_sources, self.n_bands)[0].reshape(-1)
+        return g if sign == 1.0 else g.neg()
+
+    def _chi2_body(self, theta):
+        self.proposer.launch(self.device_proposal(theta, copy=False), self.patch)
+        return self.proposer._chi2[:self.n_bands].sum(), self._grad()
+
+    def _residual_body(self, theta):
+        self.proposer.launch(self.device_proposal(theta, copy=False), self.patch,
+                             model_only=True)
+        return (self.patch.device_ptrs["residual"].view(_torch().float32)[:self.patch.npix],)
+
+    def _vjp_body(self, theta, weight):
+        # residual = data - model: d(sum_pix w r)/dtheta = -sum_pix w dmodel/dtheta
+        self.proposer.launch(self.device_proposal(theta, copy=False), self.patch,
+                             pixel_weight=weight)
+        return (self._grad(-1.0),)
 
     def _graph_state(self):
-        """What the recorded graph depends on: the device patch struct, the
+        """What a recorded graph depends on: the device patch struct, the
         tile table and the kernel configuration."""
         p = self.proposer
         tiles = self.patch.device_tiles(p.pix_per_tile)[0]
         return (self.patch.device_patch.data_ptr(), tiles.data_ptr(), p.config, p.persistent)
 
-    def _capture(self, theta, key):
+    def _call(self, body, *inputs):
+        """Run ``body`` on device copies of ``inputs``: directly, or by
+        replaying its recorded graph.  Returns the body's output tensors
+        (a graph's are overwritten by its next replay)."""
         torch = _torch()
         dev = self.proposer.device
-        self._graph = None
-        self._graph_theta = theta.detach().to(dev, torch.float64).clone()
-        side = torch.cuda.Stream(device=dev)
-        side.wait_stream(torch.cuda.current_stream(dev))
-        with torch.cuda.stream(side):
-            self._chi2_and_grad(self._graph_theta)   # builds the slot outside the graph
-        torch.cuda.current_stream(dev).wait_stream(side)
-        graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(graph):
-            self._graph_out = self._chi2_and_grad(self._graph_theta)
-        # buffers the graph reads by address stay referenced with it
-        self._graph_refs = (self.proposer._slot, self.patch.device_tiles(self.proposer.pix_per_tile)[0])
-        self._graph, self._graph_key = graph, key
+        if not self.use_graph:
+            return body(*(x.detach().to(dev) for x in inputs))
+        key = self._graph_state()
+        rec = self._graphs.get(body.__name__)
+        if rec is not None and rec["key"] == key:
+            for s, x in zip(rec["inputs"], inputs):
+                s.copy_(x.detach())
+        else:
+            static = [x.detach().to(dev).clone() for x in inputs]
+            side = torch.cuda.Stream(device=dev)
+            side.wait_stream(torch.cuda.current_stream(dev))
+            with torch.cuda.stream(side):
+                body(*static)               # builds the device slot outside the graph
+            torch.cuda.current_stream(dev).wait_stream(side)
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph):
+                out = body(*static)
+            # the buffers the graph reads by address stay referenced with it
+            refs = (self.proposer._slot, self.patch.device_tiles(self.proposer.pix_per_tile)[0])
+            rec = dict(key=key, graph=graph, inputs=static, out=out, refs=refs)
+            self._graphs[body.__name__] = rec
+        rec["graph"].replay()
+        return rec["out"]
 
     def chi2_and_grad(self, theta):
         """chi2 and dchi2/dtheta of theta (float64 tensor [ndim]) as new
         float64 device tensors, from one chi2-mode launch; no host
         synchronisation."""
-        torch = _torch()
-        if not self.use_graph:
-            theta = theta.detach().to(self.proposer.device, torch.float64)
-            value, grad = self._chi2_and_grad(theta)
-            return value, grad
-        key = self._graph_state()
-        if self._graph is None or key != self._graph_key:
-            self._capture(theta, key)
-        else:
-            self._graph_theta.copy_(theta.detach())
-        self._graph.replay()
-        value, grad = self._graph_out
+        value, grad = self._call(self._chi2_body, theta.to(_torch().float64))
         return value.clone(), grad.clone()
+
+    def residual_device(self, theta):
+        """data - model(theta) over the packed pixels (float32 device tensor
+        [npix], a copy) from one model-only launch."""
+        return self._call(self._residual_body, theta.to(_torch().float64))[0].clone()
+
+    def vjp_theta(self, theta, weight):
+        """sum_pix weight[pix] * d residual[pix] / d theta (float64 device
+        tensor [ndim]) from one gradient-pass launch; weight: float32
+        [npix] device tensor."""
+        torch = _torch()
+        w = weight.to(torch.float32).contiguous()
+        return self._call(self._vjp_body, theta.to(torch.float64), w)[0].clone()
+
+    # -- host records (any amplitude model) -----------------------------------
 
     def proposal(self, theta):
         self.scene.set_all_source_params(np.asarray(theta, dtype=np.float64))
@@ -146,10 +176,11 @@ class PatchModel:
     def forward_residual(self, proposal, outputs=True):
         """Launch the chi2 kernel for ``proposal``; returns (residual float32
         device tensor [npix] (a copy), chi2 per band, dchi2 [n_bands, n_src, 7]),
-        or the residual alone (no host synchronisation) without ``outputs``."""
+        or, without ``outputs``, the residual alone from a model-only launch
+        (no host synchronisation)."""
         torch = _torch()
         p = self.proposer
-        p.launch(proposal, self.patch)
+        p.launch(proposal, self.patch, model_only=not outputs)
         resid = self.patch.device_ptrs["residual"].view(torch.float32)[:self.patch.npix].clone()
         if not outputs:
             return resid
@@ -157,6 +188,8 @@ class PatchModel:
         chi2 = p._chi2[:nb].cpu().numpy()
         resp = p._resp[:nb * 210].view(nb, 30, 7)[:, :self.n_sources].cpu().numpy()
         return resid, chi2, resp.astype(np.float64)
+
+    # -- differentiable entry points ------------------------------------------
 
     def residual(self, theta):
         """Differentiable residual image; theta: float64 tensor [ndim]."""
@@ -173,7 +206,7 @@ class PatchModel:
         """The same chi2 built by torch from the differentiable residual
         image (forward: model pass; backward: the kernel's gradient pass)."""
         r = self.residual(theta).double()
-        return ((r * self.ierr) ** 2 - (self.data * self.ierr) ** 2).sum()
+        return ((r * self.ierr) ** 2).sum() - self._data_chi2
 
 
 class PatchResidual:
@@ -236,29 +269,25 @@ def _make_function():
 
         @staticmethod
         def forward(ctx, theta, model):
-            if model.records is not None:
-                records = model.device_proposal(theta, copy=False)
-                resid = model.forward_residual(records, outputs=False)
-                proposal = records.clone()          # kept for backward
-            else:
-                proposal = model.proposal(theta.detach().cpu().numpy())
-                resid = model.forward_residual(proposal, outputs=False)
             ctx.model = model
-            ctx.proposal = proposal
             ctx.theta_meta = (theta.dtype, theta.device)
-            return resid
+            if model.records is not None:
+                ctx.save_for_backward(theta)
+                return model.residual_device(theta)
+            proposal = model.proposal(theta.detach().cpu().numpy())
+            ctx.proposal = proposal
+            return model.forward_residual(proposal, outputs=False)
 
         @staticmethod
         def backward(ctx, grad_resid):
             model = ctx.model
             dtype, device = ctx.theta_meta
+            if model.records is not None:
+                (theta,) = ctx.saved_tensors
+                g = model.vjp_theta(theta, grad_resid)
+                return g.to(dtype=dtype, device=device), None
             # residual = data - model, so dL/dtheta = -sum_pix g * dmodel/dtheta
             pw = (-grad_resid).to(torch.float32).contiguous()
-            if model.records is not None:
-                model.backward_records.copy_(ctx.proposal)
-                d = model.proposer.vjp_device(model.backward_records, pw, model.patch)
-                g = stack_grad_device(d[None], model.n_sources, model.n_bands)[0]
-                return g.reshape(-1).to(dtype=dtype, device=device), None
             d = model.proposer.vjp(ctx.proposal, pw, model.patch)
             g = stack_grad(d, model.n_sources, model.n_bands)
             return torch.from_numpy(g).to(dtype=dtype, device=device), None

@@ -536,6 +536,10 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
     for (int i = 0; i < PPL; ++i) w2[i] = valid[i] ? slot.pixel_weight[pix[i]] : 0.0f;
   }
 
+  // a model-only slot stops at its residual and chi2 (the flag is the same
+  // for the whole workgroup, and process_tile's barrier follows)
+  if (!vjp && (__builtin_amdgcn_readfirstlane(slot.flags) & FPX_SLOT_MODEL_ONLY)) return;
+
   bool wave_has_weight = false;
 #pragma unroll
   for (int i = 0; i < PPL; ++i) wave_has_weight |= (w2[i] != 0.0f);

@@ -19,7 +19,7 @@ import sys
 import numpy as np
 
 from .kernel_limits import MAXBANDS, MAXRADII, MAXSOURCES, NPARAMS
-from ._lib import lib, check, SLOT_DTYPE
+from ._lib import lib, check, SLOT_DTYPE, SLOT_MODEL_ONLY
 
 # device output block of a proposer: pchi2 (double[MAXBANDS]) then the
 # Responses (float[MAXBANDS][MAXSOURCES][NPARAMS])
@@ -155,14 +155,14 @@ class HIPProposer(ProposerBase):
         import torch
         return torch.cuda.current_stream(self.device)
 
-    def _slot_for(self, patch, pixel_weight=None, prop_ptr=None):
-        """(Re)build the device FpxSlot when the patch struct, tiles, weights
-        or proposal buffer change."""
+    def _slot_for(self, patch, pixel_weight=None, prop_ptr=None, flags=0):
+        """(Re)build the device FpxSlot when the patch struct, tiles, weights,
+        proposal buffer or flags change."""
         tiles, n_tiles = patch.device_tiles(self.pix_per_tile)
         pw = 0 if pixel_weight is None else pixel_weight.data_ptr()
         prop_ptr = self._prop.data_ptr() if prop_ptr is None else prop_ptr
         key = (patch.device_patch.data_ptr(), tiles.data_ptr(), n_tiles, int(patch.n_bands), pw,
-               prop_ptr)
+               prop_ptr, flags)
         if key != self._slot_key and key in self._slots:
             self._slot, self._slot_key = self._slots[key], key
         elif key != self._slot_key:
@@ -182,18 +182,25 @@ class HIPProposer(ProposerBase):
             slot["ticket"] = self._ticket.data_ptr()
             slot["n_bands"] = int(patch.n_bands)
             slot["n_tiles"] = n_tiles
+            slot["flags"] = flags
             self._slot.copy_(torch.from_numpy(slot.view(np.uint8).reshape(-1).copy()))
             self._slot_key = key
         return n_tiles
 
-    def launch(self, proposal, patch, pixel_weight=None, outputs=True, stream=None):
+    def launch(self, proposal, patch, pixel_weight=None, outputs=True, stream=None,
+               model_only=False):
         """Queue one evaluation on the current stream (no synchronisation).
         ``proposal`` is a numpy source_struct_dtype array or a device uint8
         tensor already holding the records.  ``outputs=False`` launches only
         the tile kernel (fpx_evaluate_tiles): its sums stay in the
-        accumulator until a launch with outputs (or ``finalize``)."""
+        accumulator until a launch with outputs (or ``finalize``).
+        ``model_only`` (chi2 mode) runs the model pass alone: the residual
+        and chi2 are written, the gradient is zero."""
+        if model_only and pixel_weight is not None:
+            raise ValueError("model_only is a chi2-mode option (no pixel_weight)")
         stream = stream or self._stream()
-        n_tiles = self._load(proposal, patch, pixel_weight, stream)
+        n_tiles = self._load(proposal, patch, pixel_weight, stream,
+                             SLOT_MODEL_ONLY if model_only else 0)
         if not outputs:
             check(lib().fpx_evaluate_tiles(self._slot.data_ptr(), 1, None, n_tiles,
                                            patch.max_gauss(), self.config, int(self.persistent),
@@ -281,7 +288,7 @@ class HIPProposer(ProposerBase):
         patch.reorder_tiles(self.pix_per_tile, perm)
         return perm
 
-    def _load(self, proposal, patch, pixel_weight, stream=None):
+    def _load(self, proposal, patch, pixel_weight, stream=None, flags=0):
         """Stage the proposal and the slot for ``patch``; returns the tile
         count.  A numpy proposal is copied through pinned memory into the
         proposer's buffer; a device tensor is read in place by the launch (it
@@ -300,12 +307,12 @@ class HIPProposer(ProposerBase):
             self._prop.copy_(self._prop_h, non_blocking=True)
             self._h2d_done = torch.cuda.Event()
             self._h2d_done.record(stream or self._stream())
-            return self._slot_for(patch, pixel_weight)
+            return self._slot_for(patch, pixel_weight, flags=flags)
         if proposal.device != self.device or not proposal.is_contiguous():
             raise ValueError("device proposals must be contiguous tensors on the proposer's device")
         if proposal.numel() * proposal.element_size() < n_src * source_struct_dtype.itemsize:
             raise ValueError("device proposal holds fewer records than the patch has sources")
-        return self._slot_for(patch, pixel_weight, prop_ptr=proposal.data_ptr())
+        return self._slot_for(patch, pixel_weight, prop_ptr=proposal.data_ptr(), flags=flags)
 
     def evaluate_proposal(self, proposal, patch=None, verbose=False, unpack=True):
         """Evaluate chi2 and dchi2/dparam of ``proposal`` on ``patch``."""

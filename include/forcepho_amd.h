@@ -118,7 +118,11 @@ typedef struct FpxTile {
  *   pixel_weight : NULL for chi2 mode.  Non-NULL selects vector-Jacobian mode:
  *            no model pass, pchi2 is written as 0 and pdchi2_dp receives
  *            sum_pix pixel_weight[pix] * d model[pix] / d param.
- *   flags  : reserved, 0 */
+ *   flags  : FPX_SLOT_MODEL_ONLY (chi2 mode): the model pass alone; the
+ *            residual and pchi2 are written, pdchi2_dp is zero.  Other bits
+ *            reserved, 0.  No counterpart in the reference, whose kernel
+ *            always runs both passes; it serves forward passes that need
+ *            the residual image and not the gradient. */
 typedef struct FpxSlot {
     const FpxPatch *patch;
     const FpxSource *proposal;
@@ -133,6 +137,8 @@ typedef struct FpxSlot {
     int32_t flags;
     int32_t reserved;
 } FpxSlot;
+
+#define FPX_SLOT_MODEL_ONLY 1
 
 /* Error codes. */
 #define FPX_OK 0

@@ -166,11 +166,12 @@ def test_device_records_match_host_proposal():
     np.testing.assert_allclose(grads[0][1], g, rtol=1e-4, atol=1e-5 * np.abs(g).max())
 
 
-def test_chi2_graph_replays_match_eager():
-    """The recorded chi2 graph (record build, chi2 launch, gradient
-    stacking) replayed for several parameter vectors, with eager launches of
-    the same proposer between the replays, gives the eager chi2 and gradient
-    of each; a change of kernel configuration records the graph again."""
+def test_graph_replays_match_eager():
+    """The recorded graphs (record build, launch, gradient stacking) of the
+    chi2, the residual image and the backward pass, replayed for several
+    parameter vectors with eager launches of the same proposer between the
+    replays, give the eager results of each; a change of kernel
+    configuration records the graphs again."""
     import torch
     model, theta = _model(side=48, n_bands=4, n_sources=6)
     nb = 4
@@ -181,14 +182,20 @@ def test_chi2_graph_replays_match_eager():
         t[:, :nb] *= 1 + 0.05 * rng.normal(size=(len(t), nb))
         t[:, nb + 2] = np.clip(t[:, nb + 2] * (1 + 0.05 * rng.normal(size=len(t))), 0.2, 0.99)
         thetas.append(t.reshape(-1))
+    weight = torch.tensor(rng.normal(size=model.patch.npix), dtype=torch.float32, device="cuda")
 
     def run(th):
-        v, g = model.chi2_and_grad(torch.tensor(th, dtype=torch.float64, device="cuda"))
-        return v.item(), g.cpu().numpy()
+        t = torch.tensor(th, dtype=torch.float64, device="cuda")
+        v, g = model.chi2_and_grad(t)
+        r = model.residual_device(t)
+        w = model.vjp_theta(t, weight * (1.0 + float(th[0])))
+        return v.item(), g.cpu().numpy(), r.cpu().numpy(), w.cpu().numpy()
 
     def same(a, b):
         np.testing.assert_allclose(a[0], b[0], rtol=1e-12)
-        np.testing.assert_allclose(a[1], b[1], rtol=1e-6, atol=1e-6 * np.abs(b[1]).max())
+        for x, y in ((a[1], b[1]), (a[3], b[3])):
+            np.testing.assert_allclose(x, y, rtol=1e-6, atol=1e-6 * np.abs(y).max())
+        np.testing.assert_array_equal(a[2], b[2])
 
     model.use_graph = False
     eager = [run(th) for th in thetas]
@@ -198,11 +205,28 @@ def test_chi2_graph_replays_match_eager():
             got = run(th)
             model.forward_residual(model.proposal(thetas[0]))     # eager work between replays
             same(got, ref)
-    graph = model._graph
+    assert set(model._graphs) == {"_chi2_body", "_residual_body", "_vjp_body"}
+    graph = model._graphs["_chi2_body"]["graph"]
     cfg = model.proposer.config
     model.proposer.set_config(2 if cfg != 2 else 1)
     got = run(thetas[1])
-    assert model._graph is not graph
+    assert model._graphs["_chi2_body"]["graph"] is not graph
     model.use_graph = False
     same(got, run(thetas[1]))
     model.proposer.set_config(cfg)
+
+
+def test_model_only_launch():
+    """A model-only launch (FPX_SLOT_MODEL_ONLY) writes the residual image
+    of the full chi2 launch bit for bit and its chi2, with a zero gradient."""
+    model, theta = _model(side=48, n_bands=4, n_sources=6)
+    p, patch = model.proposer, model.patch
+    prop = model.proposal(_offset(theta, 4))
+    resid_full, chi2_full, resp_full = model.forward_residual(prop)
+    assert np.abs(resp_full).max() > 0
+    resid_model = model.forward_residual(prop, outputs=False)
+    np.testing.assert_array_equal(resid_model.cpu().numpy(), resid_full.cpu().numpy())
+    np.testing.assert_allclose(p._chi2[:4].cpu().numpy(), chi2_full, rtol=1e-12)
+    assert not p._resp[:4 * 30 * 7].cpu().numpy().any()
+    with pytest.raises(ValueError):
+        p.launch(prop, patch, pixel_weight=resid_model, model_only=True)
