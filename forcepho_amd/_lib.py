@@ -14,9 +14,11 @@ from .build import LIB_PATH
 
 __all__ = ["lib", "check", "FpxError", "TILE_DTYPE", "SLOT_DTYPE", "SLOT_MODEL_ONLY",
            "STRUCT_PATCH", "STRUCT_SOURCE", "STRUCT_RESPONSE", "STRUCT_PSFGAUSS",
-           "STRUCT_TILE", "STRUCT_SLOT", "EXPORTED_SYMBOLS"]
+           "STRUCT_TILE", "STRUCT_SLOT", "STRUCT_AMPMODEL", "EXPORTED_SYMBOLS",
+           "AmplitudeModel", "AMP_LOGNORMAL", "AMP_SPLINE", "MAX_SPLINE_DEGREE"]
 
-STRUCT_PATCH, STRUCT_SOURCE, STRUCT_RESPONSE, STRUCT_PSFGAUSS, STRUCT_TILE, STRUCT_SLOT = range(6)
+(STRUCT_PATCH, STRUCT_SOURCE, STRUCT_RESPONSE, STRUCT_PSFGAUSS, STRUCT_TILE, STRUCT_SLOT,
+ STRUCT_AMPMODEL) = range(7)
 
 # FpxTile / FpxSlot as numpy dtypes (include/forcepho_amd.h)
 TILE_DTYPE = np.dtype([("exposure", np.int32), ("band", np.int32),
@@ -28,6 +30,22 @@ SLOT_DTYPE = np.dtype([("patch", np.uintp), ("proposal", np.uintp), ("tiles", np
                        ("flags", np.int32), ("reserved", np.int32)], align=True)
 
 SLOT_MODEL_ONLY = 1    # FpxSlot.flags: model pass only (FPX_SLOT_MODEL_ONLY)
+
+AMP_LOGNORMAL, AMP_SPLINE = 0, 1   # FpxAmplitudeModel.kind
+MAX_SPLINE_DEGREE = 5              # FPX_MAX_SPLINE_DEGREE
+
+
+class AmplitudeModel(ctypes.Structure):
+    """FpxAmplitudeModel (include/forcepho_amd.h); array fields hold device
+    addresses."""
+    _fields_ = [("kind", ctypes.c_int32), ("n_radii", ctypes.c_int32),
+                ("lnr", ctypes.c_void_p), ("lnr_count", ctypes.c_void_p),
+                ("kx", ctypes.c_int32), ("ky", ctypes.c_int32),
+                ("tx_stride", ctypes.c_int32), ("ty_stride", ctypes.c_int32),
+                ("cx", ctypes.c_int32), ("cy", ctypes.c_int32),
+                ("tx", ctypes.c_void_p), ("ty", ctypes.c_void_p),
+                ("tx_len", ctypes.c_void_p), ("ty_len", ctypes.c_void_p),
+                ("coef", ctypes.c_void_p)]
 
 _c_int, _c_size, _p = ctypes.c_int32, ctypes.c_size_t, ctypes.c_void_p
 
@@ -49,6 +67,8 @@ _SIGNATURES = {
     "fpx_finalize": (_c_int, [_p, _c_int, _p]),
     "fpx_count_pairs": (_c_int, [_p, _c_int, _p, _c_int, _c_int, _p, _p, _p]),
     "fpx_resident_groups": (_c_int, [_c_int, _c_int, _p]),
+    "fpx_build_sources": (_c_int, [_p, _c_int, _c_int, _c_int, _p, _p, _p, _p, _p]),
+    "fpx_stack_gradients": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _p, _p, _p]),
     "fpx_probe_valu": (_c_int, [_c_int, _c_int, _p, _c_int, _p, _p]),
 }
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)

@@ -84,20 +84,25 @@ class PatchModel:
         """Source records of theta (any device, float64 [ndim]) as a uint8
         device tensor: a copy owned by the caller, or (copy=False) the
         builder's own buffer, overwritten by the next call."""
-        t = theta.detach().to(self.records.records.device, dtype=self.records.lnr.dtype)
-        self.records.write(t.reshape(1, self.n_sources, self.n_bands + 6))
+        torch = _torch()
+        t = theta.detach().to(self.records.records.device, dtype=torch.float64)
+        self.records.write(t.reshape(1, self.n_sources, self.n_bands + 6).contiguous())
         return self.records.records.clone() if copy else self.records.records
 
     # -- device calls (records built on the device) ---------------------------
 
-    def _grad(self, sign=1.0):
-        resp = self.proposer._resp.view(1, MAXBANDS, MAXSOURCES, 7)
-        g = stack_grad_device(resp, self.n_sources, self.n_bands)[0].reshape(-1)
-        return g if sign == 1.0 else g.neg()
+    def _stacked(self):
+        """dchi2/dtheta [ndim] and the chi2 summed over bands (0-d) of the
+        proposer's last launch (one fpx_stack_gradients launch)."""
+        p = self.proposer
+        g, c = stack_grad_device(p._resp.view(1, MAXBANDS, MAXSOURCES, 7), self.n_sources,
+                                 self.n_bands, chi2=p._chi2.view(1, MAXBANDS))
+        return g.reshape(-1), c[0]
 
     def _chi2_body(self, theta):
         self.proposer.launch(self.device_proposal(theta, copy=False), self.patch)
-        return self.proposer._chi2[:self.n_bands].sum(), self._grad()
+        g, c = self._stacked()
+        return c, g
 
     def _residual_body(self, theta):
         self.proposer.launch(self.device_proposal(theta, copy=False), self.patch,
@@ -108,7 +113,7 @@ class PatchModel:
         # residual = data - model: d(sum_pix w r)/dtheta = -sum_pix w dmodel/dtheta
         self.proposer.launch(self.device_proposal(theta, copy=False), self.patch,
                              pixel_weight=weight)
-        return (self._grad(-1.0),)
+        return (self._stacked()[0].neg(),)
 
     def _graph_state(self):
         """What a recorded graph depends on: the device patch struct, the

@@ -16,8 +16,8 @@ INCLUDE_DIR = os.path.join(REPO_DIR, "include")
 CSRC_DIR = os.path.join(PKG_DIR, "csrc")
 LIB_PATH = os.path.join(PKG_DIR, "libfpx.so")
 
-SOURCES = ["fpx_kernel.hip"]
-HEADERS = ["fpx_gauss.h"]
+SOURCES = ["fpx_kernel.hip", "fpx_records.hip"]
+HEADERS = ["fpx_gauss.h", "fpx_internal.h"]
 
 # Packed FP32 (v_pk_*_f32) is disabled for all device code: the compiler
 # otherwise forms those instructions from scalar float code by itself.

@@ -47,6 +47,7 @@
 
 #include "forcepho_amd.h"
 #include "fpx_gauss.h"
+#include "fpx_internal.h"
 
 static_assert(sizeof(FpxPatch) == 152, "Patch layout (patch.cc:131-208)");
 static_assert(offsetof(FpxPatch, n_sources) == 72, "Patch.n_sources offset");
@@ -970,6 +971,11 @@ int launch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
 
 }  // namespace
 
+namespace fpx {
+int report_error(int code, const char *msg) { return set_err(code, msg); }
+int report_hip_error(hipError_t e, const char *where) { return hip_err(e, where); }
+}  // namespace fpx
+
 extern "C" {
 
 int fpx_abi_version(void) { return FPX_ABI_VERSION; }
@@ -984,6 +990,7 @@ size_t fpx_struct_size(int which) {
     case FPX_STRUCT_PSFGAUSS: return sizeof(FpxPSFSourceGaussian);
     case FPX_STRUCT_TILE: return sizeof(FpxTile);
     case FPX_STRUCT_SLOT: return sizeof(FpxSlot);
+    case FPX_STRUCT_AMPMODEL: return sizeof(FpxAmplitudeModel);
     default: return 0;
   }
 }

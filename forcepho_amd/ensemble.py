@@ -123,8 +123,7 @@ class HMCEnsemble:
         jac = rng * sz * (1.0 - sz)
         self.records.write(theta)
         self.batch.launch()
-        g_theta = stack_grad_device(self.resp, self.S, self.NB)
-        chi2 = self.chi2.sum(dim=1)
+        g_theta, chi2 = stack_grad_device(self.resp, self.S, self.NB, chi2=self.chi2)
         lndet = (torch.log(jac) * self.fmask).sum(dim=(1, 2))
         self.U.copy_(0.5 * chi2 - lndet)
         self.g.copy_((0.5 * g_theta * jac - (1.0 - 2.0 * sz)) * self.fmask)

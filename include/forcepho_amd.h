@@ -140,6 +140,37 @@ typedef struct FpxSlot {
 
 #define FPX_SLOT_MODEL_ONLY 1
 
+/* A mixture-amplitude model that fpx_build_sources evaluates on the device:
+ * the mixture amplitudes of a source and their derivatives with respect to
+ * sersic and rh.  Array pointers are device pointers.
+ *   kind FPX_AMP_LOGNORMAL: a log-normal weight over ln radius,
+ *     sig = 0.6 + 0.25 n, mu = ln rh - 0.15 (n - 1),
+ *     A_i = exp(-z_i^2 / 2) / sum_j exp(-z_j^2 / 2), z_i = (lnr_i - mu) / sig;
+ *     lnr [n_prop][MAXRADII] the ln radii of each proposal, lnr_count [n_prop]
+ *     how many are used (the rest of A is zero).
+ *   kind FPX_AMP_SPLINE: forcepho's per-radius bicubic (any degree <= 5)
+ *     tensor-product B-splines of (sersic, rh) (forcepho/sources.py:623-681,
+ *     scipy's tck): radius i has knots tx[i][0..tx_len[i]) (row stride
+ *     tx_stride), ty likewise, coefficients coef[i][a * cy + b] (row stride
+ *     cx * cy); arguments are clamped to the knot range as FITPACK does.
+ * 88 bytes. */
+typedef struct FpxAmplitudeModel {
+    int32_t kind;
+    int32_t n_radii;          /* spline: radii of the model (<= MAXRADII) */
+    const double *lnr;
+    const int32_t *lnr_count;
+    int32_t kx, ky;
+    int32_t tx_stride, ty_stride;
+    int32_t cx, cy;
+    const double *tx, *ty;
+    const int32_t *tx_len, *ty_len;
+    const double *coef;
+} FpxAmplitudeModel;
+
+#define FPX_AMP_LOGNORMAL 0
+#define FPX_AMP_SPLINE 1
+#define FPX_MAX_SPLINE_DEGREE 5
+
 /* Error codes. */
 #define FPX_OK 0
 #define FPX_E_ARG -1      /* invalid argument or limit exceeded */
@@ -153,6 +184,7 @@ typedef struct FpxSlot {
 #define FPX_STRUCT_PSFGAUSS 3
 #define FPX_STRUCT_TILE 4
 #define FPX_STRUCT_SLOT 5
+#define FPX_STRUCT_AMPMODEL 6
 
 /* ------------------------------------------------------------------------ */
 /* Host-only helpers (no GPU needed).                                       */
@@ -266,6 +298,37 @@ int fpx_count_pairs(const FpxSlot *slots, int32_t n_slots,
  * over the compute units, so a non-persistent launch of at most that many
  * tiles normally runs tile i on compute unit i mod n_cu (fpx_device_info). */
 int fpx_resident_groups(int32_t cfg, int32_t max_gauss, int32_t *groups);
+
+/* Source records of n_prop proposals from their parameter vectors, on the
+ * device: Galaxy.proposal (forcepho/sources.py:607-621, the amplitudes from
+ * SplineMixtureAmplitudes, :657-681) for many proposals in one launch, so
+ * that an HMC step or an autograd pass builds its records without the host.
+ *   theta   : device double [n_prop][n_src_max][n_bands_max + 6], per source
+ *             the band fluxes then ra, dec, q, pa, sersic, rh
+ *   n_sources, n_bands : device int32 [n_prop], the sources and bands of
+ *             each proposal (fluxes of later bands are written as 0, records
+ *             of later sources as all zero); NULL = n_src_max / n_bands_max
+ *   model   : host pointer to the amplitude model (its arrays on the device)
+ *   records : device FpxSource [n_prop][MAXSOURCES]; the first n_src_max
+ *             records of each proposal are written
+ * The values are computed in double and rounded to float. */
+int fpx_build_sources(const double *theta, int32_t n_prop, int32_t n_src_max,
+                      int32_t n_bands_max, const int32_t *n_sources,
+                      const int32_t *n_bands, const FpxAmplitudeModel *model,
+                      FpxSource *records, void *stream);
+
+/* dchi2/dtheta in the parameter order of fpx_build_sources, stacked from
+ * per-band responses as forcepho's Posterior.stack_grad does
+ * (forcepho/model.py:370-388): the flux of band b from band b's response,
+ * ra, dec, q, pa, sersic, rh summed over the bands b < n_bands_max; and the
+ * chi2 summed over those bands.
+ *   resp     : device FpxResponse [n_prop][MAXBANDS]
+ *   chi2     : device double [n_prop][MAXBANDS]
+ *   grad     : device double [n_prop][n_src_max][n_bands_max + 6] or NULL
+ *   chi2_sum : device double [n_prop] or NULL */
+int fpx_stack_gradients(const FpxResponse *resp, const double *chi2, int32_t n_prop,
+                        int32_t n_src_max, int32_t n_bands_max, double *grad,
+                        double *chi2_sum, void *stream);
 
 /* Throughput probe of the VALU pipes used by the hot loop: kind 0 = FP32
  * FMA, kind 1 = v_exp_f32, 8 independent chains of `iters` steps per lane.

@@ -39,6 +39,8 @@ def test_struct_sizes_match_dtypes():
     assert L.fpx_struct_size(_lib.STRUCT_PSFGAUSS) == PSF_DTYPE.itemsize == 28
     assert L.fpx_struct_size(_lib.STRUCT_TILE) == _lib.TILE_DTYPE.itemsize == 16
     assert L.fpx_struct_size(_lib.STRUCT_SLOT) == _lib.SLOT_DTYPE.itemsize == 80
+    assert L.fpx_struct_size(_lib.STRUCT_AMPMODEL) == ctypes.sizeof(_lib.AmplitudeModel) == 88
+    assert _lib.AmplitudeModel.tx.offset == 48 and _lib.AmplitudeModel.coef.offset == 80
     assert L.fpx_struct_size(99) == 0
 
 

@@ -1,105 +1,56 @@
-"""Device evaluation of forcepho's spline mixture amplitudes
-(forcepho/sources.py:623-681) against scipy's own evaluation of the same
-splines: values and first derivatives of RectBivariateSpline and
-SmoothBivariateSpline forms of the reference's table, of splines of other
-degrees, inside and outside the knot range (FITPACK clamps the arguments),
-and the Source records DeviceRecords writes from them against
-Galaxy.proposal.  torch runs on the CPU here; the GPU tests run the same
-code on the device."""
+"""Host side of the device records (forcepho/sources.py:607-681 on the
+device): the spline tables DeviceSplineAmplitudes lays out for
+fpx_build_sources, the checks of DeviceRecords, and the argument checks of
+fpx_build_sources / fpx_stack_gradients, which return before any launch (no
+GPU needed).  The kernels' values are tested against scipy and
+Galaxy.proposal in test_gpu_records.py."""
+
+import ctypes
 
 import numpy as np
 import pytest
 
 from fpx_testutil import verification_amplitudes
+from forcepho_amd import _lib
 from forcepho_amd.sources import Galaxy, Scene, SplineMixtureAmplitudes
 from forcepho_amd.device_scene import DeviceSplineAmplitudes, DeviceRecords
 
 torch = pytest.importorskip("torch")
 
 
-def _scipy(model, n, r):
-    A = np.array([[s(x, y)[0, 0] for s in model.splines] for x, y in zip(n, r)])
-    dn = np.array([[s(x, y, dx=1)[0, 0] for s in model.splines] for x, y in zip(n, r)])
-    dr = np.array([[s(x, y, dy=1)[0, 0] for s in model.splines] for x, y in zip(n, r)])
-    return A, dn, dr
-
-
-def _points(rng, nrange, rrange, k=150):
-    n = np.concatenate([rng.uniform(*nrange, k), nrange, [2.2, 3.0]])
-    r = np.concatenate([rng.uniform(*rrange, k), rrange, [0.1, 0.09]])
-    return n, r
-
-
 @pytest.mark.parametrize("spline_type", ["rect", "smooth"])
-def test_reference_table_splines(spline_type):
+def test_spline_tables_layout(spline_type):
+    """Knots padded with their last value, lengths per radius, coefficient
+    c_i[a, b] at a * Cy + b, and the FpxAmplitudeModel pointing at them."""
     model = verification_amplitudes(spline_type)
     dev = DeviceSplineAmplitudes(model, torch.device("cpu"))
-    rng = np.random.default_rng(1)
-    # inside the table (1..5, 0.03..0.3) and beyond it on every side
-    n, r = _points(rng, (0.5, 5.5), (0.01, 0.4))
-    A, dn, dr = (t.numpy() for t in dev(torch.tensor(n), torch.tensor(r)))
-    A0, dn0, dr0 = _scipy(model, n, r)
-    np.testing.assert_allclose(A, A0, rtol=0, atol=1e-13)
-    np.testing.assert_allclose(dn, dn0, rtol=0, atol=1e-12 * np.abs(dn0).max())
-    np.testing.assert_allclose(dr, dr0, rtol=0, atol=1e-12 * np.abs(dr0).max())
-    # batch shape [n, S] -> [n, S, R]
-    A2, _, _ = dev(torch.tensor(n[:12]).reshape(3, 4), torch.tensor(r[:12]).reshape(3, 4))
-    assert tuple(A2.shape) == (3, 4, len(model.radii))
-    np.testing.assert_allclose(A2.reshape(12, -1).numpy(), A0[:12], atol=1e-13)
+    assert dev.R == len(model.splines) and (dev.kx, dev.ky) == (3, 3)
+    for i, s in enumerate(model.splines):
+        tx, ty, c = (np.asarray(v) for v in s.tck)
+        n = len(tx)
+        assert int(dev.tx_len[i]) == n and int(dev.ty_len[i]) == len(ty)
+        np.testing.assert_array_equal(dev.tx[i, :n].numpy(), tx)
+        assert (dev.tx[i, n:].numpy() == tx[-1]).all()
+        nx, ny = n - 4, len(ty) - 4
+        cc = dev.c[i].numpy().reshape(dev.Cx, dev.Cy)
+        np.testing.assert_array_equal(cc[:nx, :ny], c.reshape(nx, ny))
+        assert not cc[nx:].any() and not cc[:, ny:].any()
+    st = dev.struct()
+    assert st.kind == _lib.AMP_SPLINE and st.n_radii == dev.R
+    assert st.tx_stride == dev.tx.shape[1] and st.cy == dev.Cy
+    assert st.coef == dev.c.data_ptr() and st.ty_len == dev.ty_len.data_ptr()
 
 
-@pytest.mark.parametrize("kx,ky", [(2, 2), (2, 3), (3, 2), (5, 4)])
-def test_other_degrees_and_uneven_knots(kx, ky):
-    """Degrees other than cubic and per-radius knots of different lengths
-    (a smoothing spline per radius with its own knots)."""
-    from scipy.interpolate import RectBivariateSpline, SmoothBivariateSpline
-    rng = np.random.default_rng(kx * 10 + ky)
-    xg = np.sort(np.concatenate([[0.0, 4.0], rng.uniform(0, 4, 9)]))
-    yg = np.sort(np.concatenate([[0.0, 1.0], rng.uniform(0, 1, 7)]))
+def test_spline_tables_reject_high_degree():
+    from scipy.interpolate import RectBivariateSpline
+    x = np.linspace(0, 1, 12)
     model = SplineMixtureAmplitudes.__new__(SplineMixtureAmplitudes)
-    model.radii = np.arange(3.0)
-    Z = [np.sin(xg[:, None] * (i + 1)) * np.cos(3 * yg[None, :]) + i for i in range(2)]
-    model.splines = [RectBivariateSpline(xg, yg, z, kx=kx, ky=ky) for z in Z]
-    xs, ys = rng.uniform(0, 4, 80), rng.uniform(0, 1, 80)
-    zs = np.exp(-xs / 3) * ys ** 2 + 0.01 * rng.normal(size=80)
-    model.splines.append(SmoothBivariateSpline(xs, ys, zs, kx=kx, ky=ky, s=0.05))
-    assert len({len(s.tck[0]) for s in model.splines}) > 1
-    dev = DeviceSplineAmplitudes(model, torch.device("cpu"))
-    n, r = _points(rng, (-0.5, 4.5), (-0.1, 1.1))
-    A, dn, dr = (t.numpy() for t in dev(torch.tensor(n), torch.tensor(r)))
-    A0, dn0, dr0 = _scipy(model, n, r)
-    np.testing.assert_allclose(A, A0, rtol=0, atol=1e-12 * max(1, np.abs(A0).max()))
-    np.testing.assert_allclose(dn, dn0, rtol=0, atol=1e-11 * np.abs(dn0).max())
-    np.testing.assert_allclose(dr, dr0, rtol=0, atol=1e-11 * np.abs(dr0).max())
-
-
-def test_device_records_from_spline_scene():
-    """Records written on the device from theta equal Galaxy.proposal's."""
-    from forcepho_amd.kernel_limits import MAXSOURCES
-    from forcepho_amd.proposal import source_struct_dtype
-    model = verification_amplitudes("rect")
-    rng = np.random.default_rng(3)
-    nb, ns = 2, 3
-    srcs = []
-    for k in range(ns):
-        g = Galaxy(filters=["a", "b"], amplitude_model=model)
-        g.set_params(np.concatenate([rng.uniform(0.5, 2, nb), rng.normal(0, 1e-5, 2),
-                                     [rng.uniform(0.4, 1), rng.uniform(-1, 1),
-                                      rng.uniform(1.2, 4.8), rng.uniform(0.04, 0.28)]]))
-        srcs.append(g)
-    scene = Scene(srcs)
-    buf = torch.zeros(2 * MAXSOURCES * 264, dtype=torch.uint8)
-    recs = DeviceRecords([scene, scene], [nb, nb], buf, ns, nb)
-    theta = torch.tensor(np.stack([scene.get_all_source_params().reshape(ns, nb + 6)] * 2))
-    recs.write(theta)
-    got = buf.numpy().view(source_struct_dtype).reshape(2, MAXSOURCES)
-    host = scene.get_proposal()
-    for f in ("ra", "dec", "q", "pa", "nsersic", "rh", "fluxes"):
-        np.testing.assert_array_equal(got[0, :ns][f], host[f])
-        np.testing.assert_array_equal(got[1, :ns][f], host[f])
-    for f in ("mixture_amplitudes", "damplitude_drh", "damplitude_dnsersic"):
-        np.testing.assert_allclose(got[0, :ns][f], host[f], rtol=1e-6,
-                                   atol=1e-7 * np.abs(host[f]).max())
+    model.radii = np.arange(1.0)
+    model.splines = [RectBivariateSpline(x, x, np.outer(x, x), kx=5, ky=5)]
+    DeviceSplineAmplitudes(model, torch.device("cpu"))
+    model.splines[0].degrees = (6, 5)          # beyond FPX_MAX_SPLINE_DEGREE
+    with pytest.raises(ValueError):
+        DeviceSplineAmplitudes(model, torch.device("cpu"))
 
 
 def test_device_records_reject_mixed_models():
@@ -111,3 +62,63 @@ def test_device_records_reject_mixed_models():
     assert not device_amplitude_models_ok(Scene([a, b]))
     with pytest.raises(TypeError):
         DeviceRecords([Scene([a]), Scene([b])], [1, 1], torch.zeros(2 * 30 * 264, dtype=torch.uint8), 1, 1)
+
+
+def test_device_records_check_theta():
+    """write() takes only a contiguous float64 [n, S, K] tensor on the
+    records' device (checked before any launch)."""
+    scene = Scene([Galaxy(filters=["a", "b"]) for _ in range(3)])
+    recs = DeviceRecords([scene], [2], torch.zeros(30 * 264, dtype=torch.uint8), 3, 2)
+    assert recs.model.kind == _lib.AMP_LOGNORMAL
+    assert recs.n_sources.tolist() == [3] and recs.n_bands.tolist() == [2]
+    for bad in (torch.zeros(1, 3, 7), torch.zeros(1, 3, 8, dtype=torch.float32),
+                torch.zeros(1, 8, 3, dtype=torch.float64).transpose(1, 2)):
+        with pytest.raises(ValueError):
+            recs.write(bad)
+    with pytest.raises(ValueError):
+        DeviceRecords([scene], [2], torch.zeros(30 * 264, dtype=torch.uint8), 2, 2)
+    with pytest.raises(ValueError):
+        DeviceRecords([scene], [2], torch.zeros(10 * 264, dtype=torch.uint8), 3, 2)
+
+
+def _rc(fn, *args):
+    rc = fn(*args)
+    return rc, _lib.lib().fpx_last_error().decode()
+
+
+def test_build_sources_argument_checks():
+    L = _lib.lib()
+    good = _lib.AmplitudeModel(kind=_lib.AMP_LOGNORMAL, lnr=8, lnr_count=8)
+    ptr = ctypes.c_void_p(8)
+
+    def call(model, n=1, S=1, NB=1, theta=ptr, recs=ptr):
+        return _rc(L.fpx_build_sources, theta, n, S, NB, None, None,
+                   None if model is None else ctypes.byref(model), recs, None)
+
+    assert call(good, S=0)[0] == -1 and "n_src_max" in call(good, S=0)[1]
+    assert call(good, S=31)[0] == -1
+    assert call(good, NB=31)[0] == -1 and "n_bands_max" in call(good, NB=31)[1]
+    assert call(good, n=-1)[0] == -1
+    assert call(None)[0] == -1
+    assert call(_lib.AmplitudeModel(kind=7))[1] == "unknown amplitude model kind"
+    assert "without radii" in call(_lib.AmplitudeModel(kind=_lib.AMP_LOGNORMAL))[1]
+    spl = dict(kind=_lib.AMP_SPLINE, n_radii=9, kx=3, ky=3, tx_stride=11, ty_stride=11,
+               cx=7, cy=7, tx=8, ty=8, tx_len=8, ty_len=8, coef=8)
+    assert call(_lib.AmplitudeModel(**dict(spl, kx=6)))[0] == -1
+    assert call(_lib.AmplitudeModel(**dict(spl, n_radii=11)))[0] == -1
+    assert "NULL table" in call(_lib.AmplitudeModel(**dict(spl, coef=None)))[1]
+    assert "smaller than" in call(_lib.AmplitudeModel(**dict(spl, tx_stride=7)))[1]
+    assert call(good, n=0)[0] == 0          # nothing to write: no launch
+    assert call(good, theta=None)[0] == -1
+
+
+def test_stack_gradients_argument_checks():
+    L = _lib.lib()
+    ptr = ctypes.c_void_p(8)
+    f = L.fpx_stack_gradients
+    assert _rc(f, ptr, ptr, 1, 0, 1, ptr, ptr, None)[0] == -1
+    assert _rc(f, ptr, ptr, 1, 1, 31, ptr, ptr, None)[0] == -1
+    assert "without its input" in _rc(f, None, ptr, 1, 1, 1, ptr, None, None)[1]
+    assert "without its input" in _rc(f, ptr, None, 1, 1, 1, None, ptr, None)[1]
+    assert _rc(f, ptr, ptr, 1, 1, 1, None, None, None)[0] == 0     # no outputs
+    assert _rc(f, ptr, ptr, 0, 1, 1, ptr, ptr, None)[0] == 0       # no proposals
