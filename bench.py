@@ -417,6 +417,36 @@ def cpu_baseline(patch, proposal, n_evals, budget_s=12.0):
                       f"{el:.1f} s on one host core"}
 
 
+def cpu_baseline_parallel(patch, proposal, n_evals, budget_s=6.0):
+    """The C restatement of the reference CPU kernel (bit-identical to it on
+    the test fixtures) on every host thread of this process's CPU share at
+    once (OMP_NUM_THREADS, else the CPU count), each thread evaluating the
+    whole workload repeatedly; ctypes releases the GIL during the kernel
+    calls.  Aggregate evals/s over the wall time until every thread ends."""
+    import threading
+    import oracle
+    threads = max(1, int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1))
+    counts = [0] * threads
+    oracle.evaluate_cpu(patch, proposal)        # loads the library once
+    t0 = time.perf_counter()
+    stop = t0 + budget_s
+
+    def work(i):
+        while time.perf_counter() < stop:
+            oracle.evaluate_cpu(patch, proposal)
+            counts[i] += 1
+
+    pool = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    for t in pool:
+        t.start()
+    for t in pool:
+        t.join()
+    el = time.perf_counter() - t0
+    return {"value": n_evals * sum(counts) / el, "unit": "evals/s", "cores": threads,
+            "kind": "port", "sample": f"{sum(counts)} full evaluation(s) of the benchmark "
+                                      f"workload on {threads} threads in {el:.1f} s"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -516,6 +546,7 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(patch, proposal, n_evals)
+        line["cpu_baseline_parallel"] = cpu_baseline_parallel(patch, proposal, n_evals)
     print(json.dumps(line), flush=True)
 
 
