@@ -582,12 +582,18 @@ __device__ void finish_tile(const FpxSlot &slot, int band, int n_src, unsigned c
 }
 
 // Outputs of every slot from its accumulator, which is left zero: pchi2[b]
-// and the float Response of each band.  One workgroup per slot.
-__global__ __launch_bounds__(256) void finalize_kernel(const FpxSlot *__restrict__ slots,
-                                                       int n_slots) {
+// and the float Response of each band.  Grid (slot, chunk of kFinalizeThreads
+// accumulator values): every thread moves at most one value, so the latency
+// of the agent-scope load and store is paid once, not once per value.
+constexpr int kFinalizeThreads = 256;
+constexpr int kFinalizeChunks = (MAXBANDS * kAcc + kFinalizeThreads - 1) / kFinalizeThreads;
+
+__global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(
+    const FpxSlot *__restrict__ slots, int n_slots) {
   const FpxSlot slot = slots[blockIdx.x];
   const int nb = min(slot.n_bands, MAXBANDS);
-  for (int j = threadIdx.x; j < nb * kAcc; j += blockDim.x) {
+  const int j = blockIdx.y * kFinalizeThreads + threadIdx.x;
+  if (j < nb * kAcc) {
     const double v = __hip_atomic_load(slot.accum + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int b = j / kAcc;
     const int k = j - b * kAcc;
@@ -937,7 +943,8 @@ int launch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
   const void *finalize = reinterpret_cast<const void *>(&fpx::finalize_kernel);
   if (n_items == 0) {  // no pixels: chi2 and gradients are zero
     if (finalize_outputs)
-      FPX_HIP(hipLaunchKernel(finalize, dim3(n_slots), dim3(256), fargs, 0, (hipStream_t)stream));
+      FPX_HIP(hipLaunchKernel(finalize, dim3(n_slots, fpx::kFinalizeChunks),
+                              dim3(fpx::kFinalizeThreads), fargs, 0, (hipStream_t)stream));
     return FPX_OK;
   }
   const KernelCfg &kc = kConfigs[cfg];
@@ -965,7 +972,8 @@ int launch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
                   (void *)&persistent, (void *)&queue};
   FPX_HIP(hipLaunchKernel(kc.fn, dim3(grid), dim3(threads), args, lds, (hipStream_t)stream));
   if (finalize_outputs)
-    FPX_HIP(hipLaunchKernel(finalize, dim3(n_slots), dim3(256), fargs, 0, (hipStream_t)stream));
+    FPX_HIP(hipLaunchKernel(finalize, dim3(n_slots, fpx::kFinalizeChunks),
+                            dim3(fpx::kFinalizeThreads), fargs, 0, (hipStream_t)stream));
   return FPX_OK;
 }
 
@@ -1116,8 +1124,9 @@ int fpx_finalize(const FpxSlot *slots, int32_t n_slots, void *stream) {
   int rc = current_device_state(&st);
   if (rc) return rc;
   void *fargs[] = {(void *)&slots, (void *)&n_slots};
-  FPX_HIP(hipLaunchKernel(reinterpret_cast<const void *>(&fpx::finalize_kernel), dim3(n_slots),
-                          dim3(256), fargs, 0, (hipStream_t)stream));
+  FPX_HIP(hipLaunchKernel(reinterpret_cast<const void *>(&fpx::finalize_kernel),
+                          dim3(n_slots, fpx::kFinalizeChunks), dim3(fpx::kFinalizeThreads), fargs,
+                          0, (hipStream_t)stream));
   return FPX_OK;
 }
 
