@@ -119,3 +119,78 @@ def test_trajectory_energy_and_gradient_errors(ensemble):
           np.percentile(errs, [50, 90, 99, 100]))
     assert errs.max() <= 1.0
     assert np.median(errs) <= 0.1
+
+
+def test_uneven_ensemble_gradient_error_distribution():
+    """The benchmark's HMC ensemble workload in small: its uneven patch mix
+    (the five patches with the fewest pairs among the 64 that the benchmark
+    draws), two chains each, one graph-captured 32-step trajectory of every
+    chain; then the batched kernel's gradient at every chain's position
+    against the float64 oracle evaluated on the same float32 records.  Prints
+    the distribution of |error| / tolerance and of the error relative to the
+    largest gradient entry of the band, next to the same for the reference's
+    CPU kernel on the same records where it is built; every entry within
+    tolerance."""
+    import torch
+    from forcepho_amd.patches import SimplePatch
+    from forcepho_amd.ensemble import HMCEnsemble
+    from forcepho_amd.kernel_limits import MAXSOURCES
+    from forcepho_amd.sources import source_struct_dtype
+    rng = np.random.default_rng(2)          # the benchmark's ensemble draws (seed 0 + 2)
+    specs = [scenes.spec_uneven(rng) for _ in range(64)]
+    sizes = []
+    for k, spec in enumerate(specs):
+        p = SimplePatch()
+        p.build_patch(spec.stamps, spec.scene, sky_zero=spec.sky_zero)
+        sizes.append((p.n_evals(), k))
+    pick = [k for _, k in sorted(sizes)[:5]]
+    pats, scs, bnds, thetas = [], [], [], []
+    for k in pick:
+        patch, scene, theta = scenes.build_patch(specs[k], SimplePatch, oracle.render_model)
+        patch.send_to_device()
+        pats.append(patch)
+        scs.append(scene)
+        bnds.append(_bounds(theta, int(patch.n_bands)))
+        thetas.append(theta)
+    ens = HMCEnsemble(pats, scs, 2, bnds)
+    start = []
+    for pi, c in ens.batch.slot_map:
+        t = thetas[pi].copy()
+        nb = int(pats[pi].n_bands)
+        t.reshape(-1, nb + 6)[:, nb + 2] *= 0.98 - 0.001 * c
+        start.append(t)
+    ens.set_theta(start)
+    gen = torch.Generator(device=ens.device)
+    gen.manual_seed(4)
+    ens.eps.fill_(0.005)
+    print(f"patches {pick}: pairs {[int(s) for s, _ in sorted(sizes)[:5]]}", flush=True)
+    ens.hmc_step(32, generator=gen)
+    ens.potential()                          # the batched launch at the final positions
+    torch.cuda.synchronize()
+    recs = ens.batch.proposals.cpu().numpy().view(source_struct_dtype).reshape(-1, MAXSOURCES)
+    resp = ens.resp.cpu().numpy()
+    ref_mod = oracle.reference_module()
+    errs, rel, ref_errs, ref_rel = [], [], [], []
+    for s, (pi, _) in enumerate(ens.batch.slot_map):
+        ns, nb = int(pats[pi].n_sources), int(pats[pi].n_bands)
+        d = resp[s, :nb, :ns].astype(np.float64)
+        prop = recs[s, :ns].copy()
+        _, g64, mass, smass = U.fp64_reference(pats[pi], prop)
+        scale = np.abs(g64).max(axis=(1, 2), keepdims=True)
+        errs.append(U.grad_errors(d, g64, mass, smass).reshape(-1))
+        rel.append((np.abs(d - g64) / scale).reshape(-1))
+        if ref_mod is not None:
+            dr = oracle.evaluate_reference(pats[pi], prop, module=ref_mod)[1].astype(np.float64)
+            ref_errs.append(U.grad_errors(dr, g64, mass, smass).reshape(-1))
+            ref_rel.append((np.abs(dr - g64) / scale).reshape(-1))
+    errs, rel = np.concatenate(errs), np.concatenate(rel)
+    q = [50, 90, 99, 100]
+    print("gradient |error| / tolerance quantiles", q, np.percentile(errs, q), flush=True)
+    print("gradient |error| / max|g| of the band quantiles", q, np.percentile(rel, q), flush=True)
+    if ref_mod is not None:
+        ref_errs, ref_rel = np.concatenate(ref_errs), np.concatenate(ref_rel)
+        print("reference CPU kernel: |error| / tolerance", np.percentile(ref_errs, q),
+              "|error| / max|g|", np.percentile(ref_rel, q), flush=True)
+        assert ref_errs.max() <= 1.0
+    assert errs.max() <= 1.0
+    assert np.median(errs) <= 0.1
