@@ -62,6 +62,8 @@ _SIGNATURES = {
     "fpx_init": (_c_int, [_c_int]),
     "fpx_device_info": (_c_int, [_p, _p, _p]),
     "fpx_evaluate_proposal": (_c_int, [_p, _c_int, _c_int, _c_int, _p]),
+    "fpx_evaluate_sync": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _p, _p, _p, _c_size,
+                                   _p, _p, _c_size, _p]),
     "fpx_evaluate_batch": (_c_int, [_p, _c_int, _p, _c_int, _c_int, _c_int, _c_int, _p, _p]),
     "fpx_evaluate_tiles": (_c_int, [_p, _c_int, _p, _c_int, _c_int, _c_int, _c_int, _p, _p]),
     "fpx_finalize": (_c_int, [_p, _c_int, _p]),

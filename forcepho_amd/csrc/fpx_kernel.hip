@@ -1104,6 +1104,25 @@ int fpx_evaluate_proposal(const FpxSlot *slot, int32_t n_tiles, int32_t max_gaus
   return launch(slot, 1, nullptr, n_tiles, max_gauss, cfg, 0, nullptr, stream);
 }
 
+int fpx_evaluate_sync(const FpxSlot *slot, int32_t n_tiles, int32_t max_gauss, int32_t cfg,
+                      int32_t persistent, uint32_t *queue, void *proposal_dev,
+                      const void *proposal_host, size_t proposal_bytes, const void *out_dev,
+                      void *out_host, size_t out_bytes, void *stream) {
+  if (!proposal_dev || !proposal_host || !out_dev || !out_host)
+    return set_err(FPX_E_ARG, "NULL proposal or output buffer");
+  if (proposal_bytes > sizeof(FpxSource) * MAXSOURCES)
+    return set_err(FPX_E_ARG, "proposal larger than MAXSOURCES records");
+  if (out_bytes == 0) return set_err(FPX_E_ARG, "no output bytes to copy back");
+  const hipStream_t s = (hipStream_t)stream;
+  if (proposal_bytes > 0)
+    FPX_HIP(hipMemcpyAsync(proposal_dev, proposal_host, proposal_bytes, hipMemcpyHostToDevice, s));
+  const int rc = launch(slot, 1, nullptr, n_tiles, max_gauss, cfg, persistent ? 1 : 0, queue, stream);
+  if (rc) return rc;
+  FPX_HIP(hipMemcpyAsync(out_host, out_dev, out_bytes, hipMemcpyDeviceToHost, s));
+  FPX_HIP(hipStreamSynchronize(s));
+  return FPX_OK;
+}
+
 int fpx_evaluate_batch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
                        int32_t n_items, int32_t max_gauss, int32_t cfg, int32_t persistent,
                        uint32_t *queue, void *stream) {

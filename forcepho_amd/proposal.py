@@ -323,9 +323,30 @@ class HIPProposer(ProposerBase):
             print(f"fpx launch: config {self.config}, {n_bands} bands, {n_sources} sources",
                   file=sys.stderr, flush=True)
         stream = self._stream()
-        self.launch(proposal, patch, stream=stream)
-        self._out_h.copy_(self._out, non_blocking=True)
-        stream.synchronize()
+        if isinstance(proposal, np.ndarray):
+            # host proposal: copy in, launch, copy out and wait in one native
+            # call (fpx_evaluate_sync) through the pinned staging buffers
+            if len(proposal) != n_sources:
+                raise ValueError(f"proposal has {len(proposal)} sources, patch has {n_sources}")
+            if n_sources > MAXSOURCES or n_bands > MAXBANDS:
+                raise ValueError("patch exceeds MAXSOURCES/MAXBANDS")
+            raw = np.ascontiguousarray(proposal, dtype=source_struct_dtype).view(np.uint8)
+            if self._h2d_done is not None:
+                self._h2d_done.synchronize()   # an earlier launch's copy may be in flight
+                self._h2d_done = None
+            self._prop_np[:raw.size] = raw.reshape(-1)
+            n_tiles = self._slot_for(patch, None)
+            out_bytes = _CHI2_BYTES + n_bands * MAXSOURCES * NPARAMS * 4
+            check(lib().fpx_evaluate_sync(self._slot.data_ptr(), n_tiles, patch.max_gauss(),
+                                          self.config, int(self.persistent),
+                                          self._queue.data_ptr(), self._prop.data_ptr(),
+                                          self._prop_h.data_ptr(), raw.size,
+                                          self._out.data_ptr(), self._out_h.data_ptr(),
+                                          out_bytes, stream.cuda_stream), "fpx_evaluate_sync")
+        else:
+            self.launch(proposal, patch, stream=stream)
+            self._out_h.copy_(self._out, non_blocking=True)
+            stream.synchronize()
         chi_out = self._chi2_np[:n_bands].astype(self.chi_dtype, copy=True)
         derivs = self._resp_np[:n_bands * MAXSOURCES * NPARAMS].reshape(n_bands, MAXSOURCES, NPARAMS)
         chi2_derivs = derivs[:, :n_sources].astype(np.float64)

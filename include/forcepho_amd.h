@@ -251,6 +251,20 @@ int fpx_device_info(int32_t *n_cu, int32_t *lds_per_group, int32_t *clock_khz);
 int fpx_evaluate_proposal(const FpxSlot *slot, int32_t n_tiles,
                           int32_t max_gauss, int32_t cfg, void *stream);
 
+/* One evaluation with host buffers, synchronous: proposal_bytes of the
+ * proposal are copied from proposal_host to proposal_dev (the slot's
+ * proposal buffer), the two kernels of fpx_evaluate_proposal (or the
+ * persistent form, with queue) run, and out_bytes from out_dev (the slot's
+ * pchi2 / pdchi2_dp region) are copied to out_host before the call returns.
+ * The PyCUDA call EvaluateProposal(patch, cuda.In(proposal),
+ * cuda.Out(chi2), cuda.Out(derivs)) of forcepho/proposal.py:133-136 as one
+ * native call.  Host buffers may be pinned (hipHostMalloc) or pageable. */
+int fpx_evaluate_sync(const FpxSlot *slot, int32_t n_tiles, int32_t max_gauss,
+                      int32_t cfg, int32_t persistent, uint32_t *queue,
+                      void *proposal_dev, const void *proposal_host,
+                      size_t proposal_bytes, const void *out_dev, void *out_host,
+                      size_t out_bytes, void *stream);
+
 /* Many evaluations in one launch (uneven patches, many chains).
  *   slots        : device FpxSlot[n_slots]
  *   item_offsets : device int32[n_slots + 1], prefix sums of the slots'

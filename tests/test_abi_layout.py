@@ -136,3 +136,20 @@ def test_launch_argument_checks_need_no_device():
     assert L.fpx_evaluate_batch(ctypes.c_void_p(8), 1, None, 4, 10, 99, 0, None, None) < 0
     assert L.fpx_evaluate_batch(ctypes.c_void_p(8), 1, None, 4, 0, 0, 0, None, None) < 0
     assert L.fpx_evaluate_batch(ctypes.c_void_p(8), 1, None, 4, 10, 0, 1, None, None) < 0
+
+
+def test_evaluate_sync_argument_checks():
+    """fpx_evaluate_sync refuses missing buffers and oversized copies before
+    any device work (no GPU needed)."""
+    L = _lib.lib()
+    p = ctypes.c_void_p(8)
+
+    def call(prop_dev=p, prop_host=p, nbytes=264, out_dev=p, out_host=p, out_bytes=8):
+        rc = L.fpx_evaluate_sync(p, 1, 1, 0, 0, None, prop_dev, prop_host, nbytes, out_dev,
+                                 out_host, out_bytes, None)
+        return rc, L.fpx_last_error().decode()
+
+    assert call(prop_dev=None) == (-1, "NULL proposal or output buffer")
+    assert call(out_host=None)[0] == -1
+    assert call(nbytes=264 * 31) == (-1, "proposal larger than MAXSOURCES records")
+    assert call(out_bytes=0) == (-1, "no output bytes to copy back")
