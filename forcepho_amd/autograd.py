@@ -117,10 +117,12 @@ class PatchModel:
 
     def _graph_state(self):
         """What a recorded graph depends on: the device patch struct, the
+        residual buffer (``swap_on_device`` exchanges it with the data), the
         tile table and the kernel configuration."""
         p = self.proposer
         tiles = self.patch.device_tiles(p.pix_per_tile)[0]
-        return (self.patch.device_patch.data_ptr(), tiles.data_ptr(), p.config, p.persistent)
+        return (self.patch.device_patch.data_ptr(), self.patch.device_ptrs["residual"].data_ptr(),
+                tiles.data_ptr(), p.config, p.persistent)
 
     def _call(self, body, *inputs):
         """Run ``body`` on device copies of ``inputs``: directly, or by

@@ -230,3 +230,22 @@ def test_model_only_launch():
     assert not p._resp[:4 * 30 * 7].cpu().numpy().any()
     with pytest.raises(ValueError):
         p.launch(prop, patch, pixel_weight=resid_model, model_only=True)
+
+
+def test_graphs_follow_swap_on_device():
+    """After swap_on_device (the device data becomes the last residual) the
+    recorded residual graph is recorded again: its output equals the eager
+    launch bit for bit and is the residual of the swapped data, 2 r - d."""
+    import torch
+    model, theta = _model(side=32, n_bands=2, n_sources=3)
+    t = torch.tensor(_offset(theta, 2), dtype=torch.float64, device="cuda")
+    r0 = model.residual_device(t).cpu().numpy().astype(np.float64)
+    model.patch.swap_on_device()
+    r1 = model.residual_device(t).cpu().numpy()
+    model.use_graph = False
+    r1e = model.residual_device(t).cpu().numpy()
+    np.testing.assert_array_equal(r1, r1e)
+    d = model.patch.data.astype(np.float64)
+    valid = model.patch.ierr > 0
+    scale = np.abs(d).max()
+    assert np.abs(r1[valid] - (2 * r0[valid] - d[valid])).max() <= 1e-5 * scale
