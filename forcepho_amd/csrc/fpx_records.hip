@@ -2,8 +2,8 @@
 // device.
 //
 // build_sources_kernel is Galaxy.proposal (forcepho/sources.py:607-621) for
-// many proposals at once: one thread per (proposal, source) writes the
-// FpxSource record, with the mixture amplitudes and their sersic / rh
+// many proposals at once: a group of 16 lanes per (proposal, source) writes
+// the FpxSource record, one lane per radius, with the mixture amplitudes and their sersic / rh
 // derivatives evaluated in double:
 //   * the log-normal weight model of the synthetic scenes, in closed form;
 //   * forcepho's SplineMixtureAmplitudes (sources.py:657-681), per-radius
@@ -118,73 +118,75 @@ __device__ void spline_amplitude(const FpxAmplitudeModel &m, int i, double n, do
   }
 }
 
+// A record is written by a group of kGroup lanes: lane r evaluates radius r
+// (r < MAXRADII) and the group's sums and minimum over radii are butterfly
+// shuffles, so the latency of the double exp / log / divides is paid once per
+// record instead of once per radius.
+constexpr int kGroup = 16;
+static_assert(kGroup >= MAXRADII && 64 % kGroup == 0, "a record's radii fit one lane group");
+
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int m = kGroup / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, kGroup);
+  return v;
+}
+
+__device__ __forceinline__ double group_min(double v) {
+#pragma unroll
+  for (int m = kGroup / 2; m > 0; m >>= 1) v = fmin(v, __shfl_xor(v, m, kGroup));
+  return v;
+}
+
 __global__ __launch_bounds__(64) void build_sources_kernel(
     const double *__restrict__ theta, int n_prop, int S, int NB,
     const int32_t *__restrict__ n_sources, const int32_t *__restrict__ n_bands,
     const FpxAmplitudeModel model, FpxSource *__restrict__ records) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int idx = gid / kGroup;  // (proposal, source); the same for the whole group
+  const int r = gid - idx * kGroup;
   if (idx >= n_prop * S) return;
   const int k = idx / S;
   const int s = idx - k * S;
   const int K = NB + 6;
   const double *th = theta + (size_t)idx * K;
   FpxSource *rec = records + (size_t)k * MAXSOURCES + s;
+  float *dst = reinterpret_cast<float *>(rec);
+  constexpr int kFloats = (int)(sizeof(FpxSource) / sizeof(float));
   const int ns = n_sources ? n_sources[k] : S;
   const int nb = n_bands ? n_bands[k] : NB;
   if (s >= ns) {  // past the proposal's sources: an all-zero record
-    float *dst = reinterpret_cast<float *>(rec);
-    for (int j = 0; j < (int)(sizeof(FpxSource) / sizeof(float)); ++j) dst[j] = 0.0f;
+    for (int j = r; j < kFloats; j += kGroup) dst[j] = 0.0f;
     return;
   }
-  rec->ra = (float)th[NB];
-  rec->dec = (float)th[NB + 1];
-  rec->q = (float)th[NB + 2];
-  rec->pa = (float)th[NB + 3];
-  rec->nsersic = (float)th[NB + 4];
-  rec->rh = (float)th[NB + 5];
-  for (int b = 0; b < MAXBANDS; ++b) rec->fluxes[b] = (b < nb && b < NB) ? (float)th[b] : 0.0f;
+  // ra dec q pa nsersic rh, then the band fluxes
+  for (int j = r; j < 6 + MAXBANDS; j += kGroup) {
+    const int b = j - 6;
+    dst[j] = j < 6 ? (float)th[NB + j] : ((b < nb && b < NB) ? (float)th[b] : 0.0f);
+  }
   const double n = th[NB + 4], rh = th[NB + 5];
+  double A = 0.0, dA_dn = 0.0, dA_drh = 0.0;
   if (model.kind == FPX_AMP_SPLINE) {
-    for (int i = 0; i < MAXRADII; ++i) {
-      double A = 0.0, dA_dn = 0.0, dA_drh = 0.0;
-      if (i < model.n_radii) spline_amplitude(model, i, n, rh, A, dA_dn, dA_drh);
-      rec->mixture_amplitudes[i] = (float)A;
-      rec->damplitude_drh[i] = (float)dA_drh;
-      rec->damplitude_dnsersic[i] = (float)dA_dn;
-    }
-    return;
+    if (r < model.n_radii) spline_amplitude(model, r, n, rh, A, dA_dn, dA_drh);
+  } else {
+    const int R = min(model.lnr_count[k], MAXRADII);
+    const bool valid = r < R;
+    const double sig = 0.6 + 0.25 * n;
+    const double mu = log(rh) - 0.15 * (n - 1.0);
+    // lanes past the radii read radius MAXRADII - 1 (in bounds) and discard it
+    const double lnr = model.lnr[(size_t)k * MAXRADII + min(r, MAXRADII - 1)];
+    const double z = valid ? (lnr - mu) / sig : 0.0;
+    const double z2min = group_min(valid ? z * z : __builtin_inf());
+    const double w = valid ? exp(-0.5 * (z * z - z2min)) : 0.0;
+    A = w / group_sum(w);
+    const double l_rh = z / (sig * rh);
+    const double l_n = z / sig * (-0.15) + z * z * 0.25 / sig;
+    dA_drh = A * (l_rh - group_sum(A * l_rh));
+    dA_dn = A * (l_n - group_sum(A * l_n));
   }
-  const int R = min(model.lnr_count[k], MAXRADII);
-  const double *lnr = model.lnr + (size_t)k * MAXRADII;
-  const double sig = 0.6 + 0.25 * n;
-  const double mu = log(rh) - 0.15 * (n - 1.0);
-  double z[MAXRADII];
-  double z2min = __builtin_inf();
-#pragma unroll
-  for (int i = 0; i < MAXRADII; ++i) {
-    z[i] = i < R ? (lnr[i] - mu) / sig : 0.0;
-    if (i < R) z2min = fmin(z2min, z[i] * z[i]);
-  }
-  double w[MAXRADII], wsum = 0.0;
-#pragma unroll
-  for (int i = 0; i < MAXRADII; ++i) {
-    w[i] = i < R ? exp(-0.5 * (z[i] * z[i] - z2min)) : 0.0;
-    wsum += w[i];
-  }
-  double A[MAXRADII], l_rh[MAXRADII], l_n[MAXRADII], s_rh = 0.0, s_n = 0.0;
-#pragma unroll
-  for (int i = 0; i < MAXRADII; ++i) {
-    A[i] = w[i] / wsum;
-    l_rh[i] = z[i] / (sig * rh);
-    l_n[i] = z[i] / sig * (-0.15) + z[i] * z[i] * 0.25 / sig;
-    s_rh += A[i] * l_rh[i];
-    s_n += A[i] * l_n[i];
-  }
-#pragma unroll
-  for (int i = 0; i < MAXRADII; ++i) {
-    rec->mixture_amplitudes[i] = (float)A[i];
-    rec->damplitude_drh[i] = (float)(A[i] * (l_rh[i] - s_rh));
-    rec->damplitude_dnsersic[i] = (float)(A[i] * (l_n[i] - s_n));
+  if (r < MAXRADII) {
+    rec->mixture_amplitudes[r] = (float)A;
+    rec->damplitude_drh[r] = (float)dA_drh;
+    rec->damplitude_dnsersic[r] = (float)dA_dn;
   }
 }
 
@@ -255,7 +257,7 @@ int fpx_build_sources(const double *theta, int32_t n_prop, int32_t n_src_max,
   }
   if (n_prop == 0) return FPX_OK;
   if (!theta || !records) return report_error(FPX_E_ARG, "NULL theta or records");
-  const long threads = (long)n_prop * n_src_max;
+  const long threads = (long)n_prop * n_src_max * fpx::kGroup;
   if (threads > (1L << 30)) return report_error(FPX_E_ARG, "too many records for one launch");
   const int grid = (int)((threads + 63) / 64);
   hipLaunchKernelGGL(fpx::build_sources_kernel, dim3(grid), dim3(64), 0, (hipStream_t)stream,
