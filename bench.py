@@ -450,8 +450,9 @@ def cpu_baseline_parallel(patch, proposal, n_evals, budget_s=6.0):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); "
-              "multi-GPU runs are not part of this build", file=sys.stderr)
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the path shards "
+              "by patch and chain with no exchange, and multi-GPU runs are not supported",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     from forcepho_amd._lib import lib
