@@ -309,6 +309,7 @@ def ensemble_measurement(args):
     capture_s = time.perf_counter() - t1
     traj_s = time_region(ens.graph.replay, 1)
     n_evals = ens.batch.n_evals() * (args.leapfrog_steps + 1)
+    small = small_ensemble_measurement(args)
     return {"workload": f"{ens.n_slots} chains ({args.ensemble_chains} per patch) on "
                         f"{len(patches)} uneven patches, graph-captured {args.leapfrog_steps}-step "
                         "leapfrog, state on the device",
@@ -316,7 +317,54 @@ def ensemble_measurement(args):
             "trajectories_per_s": ens.n_slots / traj_s,
             "evals_per_s": n_evals / traj_s,
             "kernel_config": "{}{}".format(ens.batch.config, "p" if ens.batch.persistent else ""),
-            "host_build_s": build_s, "warmup_and_capture_s": capture_s}
+            "host_build_s": build_s, "warmup_and_capture_s": capture_s,
+            "small_regions": small}
+
+
+def small_ensemble_measurement(args, n_patches=16, chains=4, side=64, n_bands=3, n_sources=4):
+    """A dispatcher-sized ensemble (16 small regions of 64x64 px in 3 bands
+    with 4 sources, 4 chains each): time per leapfrog step of the recorded
+    trajectory against the batched chi2 launch alone, i.e. what the record
+    build, transform, potential and updates add per step."""
+    import torch
+    from forcepho_amd import scenes
+    from forcepho_amd.patches import SimplePatch
+    from forcepho_amd.ensemble import HMCEnsemble
+    rng = np.random.default_rng(args.seed + 5)
+    patches, scns, thetas = [], [], []
+    for _ in range(n_patches):
+        spec = scenes.draw_patch_spec(rng, n_bands, [1] * n_bands, [side] * n_bands, n_sources,
+                                      [4] * n_bands)
+        patch, scene, theta = scenes.build_patch(spec, SimplePatch, scenes.render_on_device)
+        patch.return_residual = False
+        patch.send_to_device()
+        patches.append(patch)
+        scns.append(scene)
+        thetas.append(theta)
+    ens = HMCEnsemble(patches, scns, chains, [ensemble_bounds(t, n_bands) for t in thetas])
+    start = []
+    for pi, c in ens.batch.slot_map:
+        t = thetas[pi].copy()
+        t.reshape(-1, n_bands + 6)[:, n_bands + 2] *= 0.98 - 0.001 * c
+        start.append(t)
+    ens.set_theta(start)
+    ens.potential()
+    ens.batch.autotune(repeats=3)
+    gen = torch.Generator(device=ens.device)
+    gen.manual_seed(args.seed)
+    ens.eps.fill_(0.005)
+    ens.p.copy_(torch.randn(ens.p.shape, generator=gen, dtype=torch.float64,
+                            device=ens.device) * ens.fmask)
+    ens.capture(args.leapfrog_steps)
+    traj_s = time_region(ens.graph.replay, 10)
+    launch_s = time_region(ens.batch.launch, 50)
+    steps = args.leapfrog_steps + 1
+    return {"workload": f"{n_patches} regions of {side}x{side} px, {n_bands} bands, "
+                        f"{n_sources} sources, {chains} chains each, graph-captured "
+                        f"{args.leapfrog_steps}-step leapfrog",
+            "chains": ens.n_slots, "trajectory_s": traj_s,
+            "trajectories_per_s": ens.n_slots / traj_s,
+            "us_per_step": traj_s / steps * 1e6, "batch_launch_us": launch_s * 1e6}
 
 
 def time_region(fn, steps):

@@ -69,7 +69,9 @@ _SIGNATURES = {
     "fpx_finalize": (_c_int, [_p, _c_int, _p]),
     "fpx_count_pairs": (_c_int, [_p, _c_int, _p, _c_int, _c_int, _p, _p, _p]),
     "fpx_resident_groups": (_c_int, [_c_int, _c_int, _p]),
-    "fpx_build_sources": (_c_int, [_p, _c_int, _c_int, _c_int, _p, _p, _p, _p, _p]),
+    "fpx_build_sources": (_c_int, [_p, _p, _p, _c_int, _c_int, _c_int, _p, _p, _p, _p, _p]),
+    "fpx_leapfrog_update": (_c_int, [_p, _p, _p, _p, _p, _p, _c_int, _c_int, _c_int,
+                                     ctypes.c_double, _c_int, _p, _p, _p, _p, _p]),
     "fpx_stack_gradients": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _p, _p, _p]),
     "fpx_probe_valu": (_c_int, [_c_int, _c_int, _p, _c_int, _p, _p]),
 }

@@ -137,8 +137,13 @@ __device__ __forceinline__ double group_min(double v) {
   return v;
 }
 
+// 1 / (1 + exp(-x)): the logistic function of forcepho's BoundedTransform
+// (forcepho/model.py:596-637).
+__device__ __forceinline__ double sigmoid(double x) { return 1.0 / (1.0 + exp(-x)); }
+
 __global__ __launch_bounds__(64) void build_sources_kernel(
-    const double *__restrict__ theta, int n_prop, int S, int NB,
+    const double *__restrict__ theta, const double *__restrict__ lower,
+    const double *__restrict__ upper, int n_prop, int S, int NB,
     const int32_t *__restrict__ n_sources, const int32_t *__restrict__ n_bands,
     const FpxAmplitudeModel model, FpxSource *__restrict__ records) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -149,6 +154,13 @@ __global__ __launch_bounds__(64) void build_sources_kernel(
   const int s = idx - k * S;
   const int K = NB + 6;
   const double *th = theta + (size_t)idx * K;
+  const double *lo = lower ? lower + (size_t)idx * K : nullptr;
+  const double *hi = upper ? upper + (size_t)idx * K : nullptr;
+  // parameter j of the source: theta itself, or lower + sigmoid(z) (upper -
+  // lower) when theta holds the unconstrained z
+  auto param = [&](int j) -> double {
+    return lo ? lo[j] + sigmoid(th[j]) * (hi[j] - lo[j]) : th[j];
+  };
   FpxSource *rec = records + (size_t)k * MAXSOURCES + s;
   float *dst = reinterpret_cast<float *>(rec);
   constexpr int kFloats = (int)(sizeof(FpxSource) / sizeof(float));
@@ -161,9 +173,9 @@ __global__ __launch_bounds__(64) void build_sources_kernel(
   // ra dec q pa nsersic rh, then the band fluxes
   for (int j = r; j < 6 + MAXBANDS; j += kGroup) {
     const int b = j - 6;
-    dst[j] = j < 6 ? (float)th[NB + j] : ((b < nb && b < NB) ? (float)th[b] : 0.0f);
+    dst[j] = j < 6 ? (float)param(NB + j) : ((b < nb && b < NB) ? (float)param(b) : 0.0f);
   }
-  const double n = th[NB + 4], rh = th[NB + 5];
+  const double n = param(NB + 4), rh = param(NB + 5);
   double A = 0.0, dA_dn = 0.0, dA_drh = 0.0;
   if (model.kind == FPX_AMP_SPLINE) {
     if (r < model.n_radii) spline_amplitude(model, r, n, rh, A, dA_dn, dA_drh);
@@ -219,6 +231,66 @@ __global__ __launch_bounds__(64) void stack_kernel(const float *__restrict__ res
   }
 }
 
+// U(z), dU/dz and a leapfrog kick / drift of every chain (one workgroup per
+// chain).  With theta = lower + sigmoid(z) (upper - lower) (forcepho's
+// BoundedTransform, model.py:596-637):
+//   U    = chi2 / 2 - sum mask ln(dtheta/dz)
+//   dU/dz = mask (dchi2/dtheta dtheta/dz / 2 - (1 - 2 sigmoid(z)))
+// the negative log posterior of FastPosterior.evaluate (model.py:302-340,
+// -lnp with the Jacobian term) in z, dchi2/dtheta stacked from the per-band
+// responses as stack_kernel does.  Then, if p is given, p -= kick eps dU/dz
+// and, with drift, z += eps p mask.
+constexpr int kStepThreads = 256;
+
+__global__ __launch_bounds__(kStepThreads) void leapfrog_kernel(
+    const double *__restrict__ lower, const double *__restrict__ upper,
+    const double *__restrict__ mask, const double *__restrict__ eps,
+    const float *__restrict__ resp, const double *__restrict__ chi2, int S, int NB, double kick,
+    int drift, double *__restrict__ z, double *__restrict__ p, double *__restrict__ U,
+    double *__restrict__ g) {
+  __shared__ double part[kStepThreads / 64];
+  const int k = blockIdx.x;
+  const int K = NB + 6;
+  const int n = S * K;
+  const size_t base = (size_t)k * n;
+  const float *r = resp + (size_t)k * MAXBANDS * MAXSOURCES * NPARAMS;
+  const double e = eps[k];
+  double lndet = 0.0;
+  for (int i = threadIdx.x; i < n; i += kStepThreads) {
+    const int s = i / K;
+    const int j = i - s * K;
+    const double zi = z[base + i];
+    const double m = mask[base + i];
+    const double sz = sigmoid(zi);
+    const double jac = (upper[base + i] - lower[base + i]) * sz * (1.0 - sz);
+    double gt;
+    if (j < NB) {
+      gt = (double)r[(j * MAXSOURCES + s) * NPARAMS];
+    } else {
+      gt = 0.0;
+      for (int b = 0; b < NB; ++b) gt += (double)r[(b * MAXSOURCES + s) * NPARAMS + (j - NB + 1)];
+    }
+    const double gi = (0.5 * gt * jac - (1.0 - 2.0 * sz)) * m;
+    g[base + i] = gi;
+    if (m != 0.0) lndet += log(jac) * m;
+    if (p) {
+      const double pi = p[base + i] - kick * e * gi;
+      p[base + i] = pi;
+      if (drift) z[base + i] = zi + e * pi * m;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) lndet += __shfl_xor(lndet, off, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = lndet;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0.0, c = 0.0;
+    for (int w = 0; w < kStepThreads / 64; ++w) tot += part[w];
+    for (int b = 0; b < NB; ++b) c += chi2[(size_t)k * MAXBANDS + b];
+    U[k] = 0.5 * c - tot;
+  }
+}
+
 int check_launch(const char *where) {
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? FPX_OK : report_hip_error(e, where);
@@ -229,8 +301,9 @@ int check_launch(const char *where) {
 
 extern "C" {
 
-int fpx_build_sources(const double *theta, int32_t n_prop, int32_t n_src_max,
-                      int32_t n_bands_max, const int32_t *n_sources, const int32_t *n_bands,
+int fpx_build_sources(const double *theta, const double *lower, const double *upper,
+                      int32_t n_prop, int32_t n_src_max, int32_t n_bands_max,
+                      const int32_t *n_sources, const int32_t *n_bands,
                       const FpxAmplitudeModel *model, FpxSource *records, void *stream) {
   using fpx::report_error;
   if (n_prop < 0) return report_error(FPX_E_ARG, "n_prop < 0");
@@ -239,6 +312,8 @@ int fpx_build_sources(const double *theta, int32_t n_prop, int32_t n_src_max,
   if (n_bands_max < 1 || n_bands_max > MAXBANDS)
     return report_error(FPX_E_ARG, "n_bands_max out of range [1, MAXBANDS]");
   if (!model) return report_error(FPX_E_ARG, "no amplitude model");
+  if ((lower == nullptr) != (upper == nullptr))
+    return report_error(FPX_E_ARG, "lower and upper bounds come together");
   const FpxAmplitudeModel m = *model;
   if (m.kind == FPX_AMP_LOGNORMAL) {
     if (!m.lnr || !m.lnr_count) return report_error(FPX_E_ARG, "log-normal model without radii");
@@ -261,7 +336,8 @@ int fpx_build_sources(const double *theta, int32_t n_prop, int32_t n_src_max,
   if (threads > (1L << 30)) return report_error(FPX_E_ARG, "too many records for one launch");
   const int grid = (int)((threads + 63) / 64);
   hipLaunchKernelGGL(fpx::build_sources_kernel, dim3(grid), dim3(64), 0, (hipStream_t)stream,
-                     theta, n_prop, n_src_max, n_bands_max, n_sources, n_bands, m, records);
+                     theta, lower, upper, n_prop, n_src_max, n_bands_max, n_sources, n_bands, m,
+                     records);
   return fpx::check_launch("build_sources_kernel");
 }
 
@@ -285,6 +361,28 @@ int fpx_stack_gradients(const FpxResponse *resp, const double *chi2, int32_t n_p
                      reinterpret_cast<const float *>(resp), chi2, n_prop, n_src_max, n_bands_max,
                      grad, chi2_sum);
   return fpx::check_launch("stack_kernel");
+}
+
+int fpx_leapfrog_update(const double *lower, const double *upper, const double *mask,
+                        const double *eps, const FpxResponse *resp, const double *chi2,
+                        int32_t n_prop, int32_t n_src_max, int32_t n_bands_max, double kick,
+                        int32_t drift, double *z, double *p, double *U, double *g,
+                        void *stream) {
+  using fpx::report_error;
+  if (n_prop < 0) return report_error(FPX_E_ARG, "n_prop < 0");
+  if (n_src_max < 1 || n_src_max > MAXSOURCES)
+    return report_error(FPX_E_ARG, "n_src_max out of range [1, MAXSOURCES]");
+  if (n_bands_max < 1 || n_bands_max > MAXBANDS)
+    return report_error(FPX_E_ARG, "n_bands_max out of range [1, MAXBANDS]");
+  if (!lower || !upper || !mask || !eps || !resp || !chi2 || !z || !U || !g)
+    return report_error(FPX_E_ARG, "NULL array");
+  if (drift && !p) return report_error(FPX_E_ARG, "drift needs the momenta");
+  if (n_prop == 0) return FPX_OK;
+  hipLaunchKernelGGL(fpx::leapfrog_kernel, dim3(n_prop), dim3(fpx::kStepThreads), 0,
+                     (hipStream_t)stream, lower, upper, mask, eps,
+                     reinterpret_cast<const float *>(resp), chi2, n_src_max, n_bands_max, kick,
+                     drift ? 1 : 0, z, p, U, g);
+  return fpx::check_launch("leapfrog_kernel");
 }
 
 }  // extern "C"

@@ -189,16 +189,27 @@ class DeviceRecords:
             n, MAXSOURCES, REC_FLOATS)
         self.props.zero_()
 
-    def write(self, theta):
+    def write(self, theta, lower=None, upper=None):
         """Write the records of parameters theta [n, S, K] (float64,
         contiguous, on the records' device): one fpx_build_sources launch on
-        the current stream."""
+        the current stream.  With ``lower`` and ``upper`` (same shape), theta
+        is the unconstrained z of a bounded transform and the parameters are
+        lower + sigmoid(z) (upper - lower)."""
         torch = _torch()
-        if (tuple(theta.shape) != (self.n, self.S, self.K) or theta.dtype != torch.float64
-                or theta.device != self.records.device or not theta.is_contiguous()):
-            raise ValueError(f"theta must be a contiguous float64 [{self.n}, {self.S}, {self.K}] "
-                             f"tensor on {self.records.device}")
-        check(lib().fpx_build_sources(theta.data_ptr(), self.n, self.S, self.NB,
+        shape = (self.n, self.S, self.K)
+        for name, t in (("theta", theta), ("lower", lower), ("upper", upper)):
+            if t is None and name != "theta":
+                continue
+            if (tuple(t.shape) != shape or t.dtype != torch.float64
+                    or t.device != self.records.device or not t.is_contiguous()):
+                raise ValueError(f"{name} must be a contiguous float64 {list(shape)} tensor on "
+                                 f"{self.records.device}")
+        if (lower is None) != (upper is None):
+            raise ValueError("lower and upper come together")
+        check(lib().fpx_build_sources(theta.data_ptr(),
+                                      None if lower is None else lower.data_ptr(),
+                                      None if upper is None else upper.data_ptr(),
+                                      self.n, self.S, self.NB,
                                       self.n_sources.data_ptr(), self.n_bands.data_ptr(),
                                       ctypes.byref(self.model), self.records.data_ptr(),
                                       _stream(self.records.device)), "fpx_build_sources")

@@ -6,12 +6,13 @@ Every chain's position lives on the device; one leapfrog step is
       -> Source records (fluxes, shapes, mixture amplitudes and their
          derivatives)                                      (Galaxy.proposal,
                                                             forcepho/sources.py:607-621)
+         both in one fpx_build_sources launch,
       -> one batched chi2 + gradient launch for all chains  (fpx_evaluate_batch)
       -> dU/dz = 1/2 stack_grad(dchi2) dq/dz - dlndetjac/dz (FastPosterior.evaluate,
                                                             forcepho/model.py:302-387)
-      -> momentum / position updates,
+         and the momentum / position updates in one fpx_leapfrog_update launch,
 all as device operations with no host round trip, so a whole trajectory can
-be recorded once into a graph and replayed.  The mixture amplitudes and
+be recorded once into a graph (four kernels per step) and replayed.  The mixture amplitudes and
 their derivatives are evaluated on the device for every (chain, source) by
 device_scene.DeviceRecords: forcepho's spline tables (one
 SplineMixtureAmplitudes shared by all scenes) or the analytic
@@ -21,9 +22,10 @@ outside the graph.
 
 import numpy as np
 
+from ._lib import lib, check
 from .kernel_limits import MAXBANDS, MAXSOURCES, NPARAMS
 from .batch import BatchEvaluator
-from .device_scene import DeviceRecords, stack_grad_device
+from .device_scene import DeviceRecords
 
 __all__ = ["HMCEnsemble"]
 
@@ -114,29 +116,31 @@ class HMCEnsemble:
         return out
 
     # ------------------------------------------------------------------
+    def _evaluate_and_update(self, kick=0.0, drift=False, move=False):
+        """Records of the current z (bounded transform on the device), one
+        batched chi2 launch, then fpx_leapfrog_update: U(z) and dU/dz into
+        self.U / self.g and, with ``move``, p -= kick eps g and (with
+        ``drift``) z += eps p.  Three launches plus the batch's two."""
+        self.records.write(self.z, lower=self.lo, upper=self.hi)
+        self.batch.launch()
+        check(lib().fpx_leapfrog_update(
+            self.lo.data_ptr(), self.hi.data_ptr(), self.fmask.data_ptr(), self.eps.data_ptr(),
+            self.resp.data_ptr(), self.chi2.data_ptr(), self.n_slots, self.S, self.NB,
+            float(kick), int(drift), self.z.data_ptr(), self.p.data_ptr() if move else None,
+            self.U.data_ptr(), self.g.data_ptr(),
+            _torch().cuda.current_stream(self.device).cuda_stream), "fpx_leapfrog_update")
+
     def _potential_and_grad(self):
         """U(z) and dU/dz for the current z into self.U / self.g (device)."""
-        torch = _torch()
-        sz = torch_sigmoid(self.z)
-        rng = self.hi - self.lo
-        theta = self.lo + sz * rng
-        jac = rng * sz * (1.0 - sz)
-        self.records.write(theta)
-        self.batch.launch()
-        g_theta, chi2 = stack_grad_device(self.resp, self.S, self.NB, chi2=self.chi2)
-        lndet = (torch.log(jac) * self.fmask).sum(dim=(1, 2))
-        self.U.copy_(0.5 * chi2 - lndet)
-        self.g.copy_((0.5 * g_theta * jac - (1.0 - 2.0 * sz)) * self.fmask)
+        self._evaluate_and_update()
 
     def _trajectory(self, n_steps):
-        self._potential_and_grad()
-        self.p.sub_(0.5 * self.eps * self.g)
+        # potential at the start, half kick, drift; then per step the
+        # potential and a full kick + drift, the last one a half kick
+        self._evaluate_and_update(0.5, drift=True, move=True)
         for i in range(n_steps):
-            self.z.add_(self.eps * self.p * self.fmask)
-            self._potential_and_grad()
-            if i < n_steps - 1:
-                self.p.sub_(self.eps * self.g)
-        self.p.sub_(0.5 * self.eps * self.g)
+            last = i == n_steps - 1
+            self._evaluate_and_update(0.5 if last else 1.0, drift=not last, move=True)
 
     def capture(self, n_steps):
         """Record one n_steps leapfrog trajectory of every chain in a graph."""

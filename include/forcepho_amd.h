@@ -319,6 +319,10 @@ int fpx_resident_groups(int32_t cfg, int32_t max_gauss, int32_t *groups);
  * that an HMC step or an autograd pass builds its records without the host.
  *   theta   : device double [n_prop][n_src_max][n_bands_max + 6], per source
  *             the band fluxes then ra, dec, q, pa, sersic, rh
+ *   lower, upper : NULL, or device double arrays of theta's shape: theta
+ *             then holds the unconstrained z of forcepho's BoundedTransform
+ *             (forcepho/model.py:596-637) and the parameters are
+ *             lower + (upper - lower) / (1 + exp(-z))
  *   n_sources, n_bands : device int32 [n_prop], the sources and bands of
  *             each proposal (fluxes of later bands are written as 0, records
  *             of later sources as all zero); NULL = n_src_max / n_bands_max
@@ -326,10 +330,11 @@ int fpx_resident_groups(int32_t cfg, int32_t max_gauss, int32_t *groups);
  *   records : device FpxSource [n_prop][MAXSOURCES]; the first n_src_max
  *             records of each proposal are written
  * The values are computed in double and rounded to float. */
-int fpx_build_sources(const double *theta, int32_t n_prop, int32_t n_src_max,
-                      int32_t n_bands_max, const int32_t *n_sources,
-                      const int32_t *n_bands, const FpxAmplitudeModel *model,
-                      FpxSource *records, void *stream);
+int fpx_build_sources(const double *theta, const double *lower, const double *upper,
+                      int32_t n_prop, int32_t n_src_max, int32_t n_bands_max,
+                      const int32_t *n_sources, const int32_t *n_bands,
+                      const FpxAmplitudeModel *model, FpxSource *records,
+                      void *stream);
 
 /* dchi2/dtheta in the parameter order of fpx_build_sources, stacked from
  * per-band responses as forcepho's Posterior.stack_grad does
@@ -343,6 +348,26 @@ int fpx_build_sources(const double *theta, int32_t n_prop, int32_t n_src_max,
 int fpx_stack_gradients(const FpxResponse *resp, const double *chi2, int32_t n_prop,
                         int32_t n_src_max, int32_t n_bands_max, double *grad,
                         double *chi2_sum, void *stream);
+
+/* Potential, gradient and leapfrog update of n_prop HMC chains in the
+ * unconstrained z of forcepho's BoundedTransform (forcepho/model.py:596-637),
+ * from the per-band responses and chi2 of the chains' last evaluation (no
+ * counterpart in the reference, whose sampler does this on the host for one
+ * chain).  Per chain k, with theta = lower + sigmoid(z) (upper - lower) and
+ * J = dtheta/dz = (upper - lower) sigmoid(z) (1 - sigmoid(z)):
+ *   U[k] = chi2_k / 2 - sum mask ln J   (chi2 summed over bands b < n_bands_max)
+ *   g    = mask (dchi2/dtheta J / 2 - (1 - 2 sigmoid(z)))
+ * (the negative log posterior of FastPosterior.evaluate, model.py:302-340,
+ * with a flat prior; dchi2/dtheta stacked as fpx_stack_gradients does); then,
+ * if p is not NULL, p -= kick eps[k] g, and with drift z += eps[k] p mask.
+ *   lower, upper, mask, z, p, g : device double [n_prop][n_src_max][n_bands_max + 6]
+ *   eps, U   : device double [n_prop]
+ *   resp     : device FpxResponse [n_prop][MAXBANDS]; chi2: device double [n_prop][MAXBANDS] */
+int fpx_leapfrog_update(const double *lower, const double *upper, const double *mask,
+                        const double *eps, const FpxResponse *resp, const double *chi2,
+                        int32_t n_prop, int32_t n_src_max, int32_t n_bands_max,
+                        double kick, int32_t drift, double *z, double *p, double *U,
+                        double *g, void *stream);
 
 /* Throughput probe of the VALU pipes used by the hot loop: kind 0 = FP32
  * FMA, kind 1 = v_exp_f32, 8 independent chains of `iters` steps per lane.

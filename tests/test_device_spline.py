@@ -91,8 +91,8 @@ def test_build_sources_argument_checks():
     good = _lib.AmplitudeModel(kind=_lib.AMP_LOGNORMAL, lnr=8, lnr_count=8)
     ptr = ctypes.c_void_p(8)
 
-    def call(model, n=1, S=1, NB=1, theta=ptr, recs=ptr):
-        return _rc(L.fpx_build_sources, theta, n, S, NB, None, None,
+    def call(model, n=1, S=1, NB=1, theta=ptr, recs=ptr, lower=None, upper=None):
+        return _rc(L.fpx_build_sources, theta, lower, upper, n, S, NB, None, None,
                    None if model is None else ctypes.byref(model), recs, None)
 
     assert call(good, S=0)[0] == -1 and "n_src_max" in call(good, S=0)[1]
@@ -108,6 +108,7 @@ def test_build_sources_argument_checks():
     assert call(_lib.AmplitudeModel(**dict(spl, n_radii=11)))[0] == -1
     assert "NULL table" in call(_lib.AmplitudeModel(**dict(spl, coef=None)))[1]
     assert "smaller than" in call(_lib.AmplitudeModel(**dict(spl, tx_stride=7)))[1]
+    assert "come together" in call(good, lower=ptr)[1]
     assert call(good, n=0)[0] == 0          # nothing to write: no launch
     assert call(good, theta=None)[0] == -1
 
@@ -122,3 +123,18 @@ def test_stack_gradients_argument_checks():
     assert "without its input" in _rc(f, ptr, None, 1, 1, 1, None, ptr, None)[1]
     assert _rc(f, ptr, ptr, 1, 1, 1, None, None, None)[0] == 0     # no outputs
     assert _rc(f, ptr, ptr, 0, 1, 1, ptr, ptr, None)[0] == 0       # no proposals
+
+
+def test_leapfrog_update_argument_checks():
+    L = _lib.lib()
+    p = ctypes.c_void_p(8)
+
+    def call(n=1, S=1, NB=1, drift=0, mom=p, z=p):
+        return _rc(L.fpx_leapfrog_update, p, p, p, p, p, p, n, S, NB, 1.0, drift, z, mom, p, p,
+                   None)
+
+    assert call(S=0)[0] == -1 and call(S=31)[0] == -1 and call(NB=31)[0] == -1
+    assert call(n=-1)[0] == -1
+    assert call(z=None) == (-1, "NULL array")
+    assert call(drift=1, mom=None) == (-1, "drift needs the momenta")
+    assert call(n=0)[0] == 0

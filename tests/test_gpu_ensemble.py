@@ -194,3 +194,64 @@ def test_uneven_ensemble_gradient_error_distribution():
         assert ref_errs.max() <= 1.0
     assert errs.max() <= 1.0
     assert np.median(errs) <= 0.1
+
+
+def _torch_trajectory(ens, n_steps):
+    """The leapfrog of HMCEnsemble written out in torch operations (theta,
+    Jacobian, potential, gradient and updates as tensor expressions; records
+    from theta, the batched launch and the stacking as library calls)."""
+    import torch
+    from forcepho_amd.device_scene import stack_grad_device
+
+    def potential():
+        sz = torch.sigmoid(ens.z)
+        rng = ens.hi - ens.lo
+        jac = rng * sz * (1.0 - sz)
+        ens.records.write((ens.lo + sz * rng).contiguous())
+        ens.batch.launch()
+        g_theta, chi2 = stack_grad_device(ens.resp, ens.S, ens.NB, chi2=ens.chi2)
+        U = 0.5 * chi2 - (torch.log(jac) * ens.fmask).sum(dim=(1, 2))
+        return U, (0.5 * g_theta * jac - (1.0 - 2.0 * sz)) * ens.fmask
+
+    U, g = potential()
+    ens.p.sub_(0.5 * ens.eps * g)
+    for i in range(n_steps):
+        ens.z.add_(ens.eps * ens.p * ens.fmask)
+        U, g = potential()
+        if i < n_steps - 1:
+            ens.p.sub_(ens.eps * g)
+    ens.p.sub_(0.5 * ens.eps * g)
+    return U, g
+
+
+def test_fused_leapfrog_matches_torch_formulation(ensemble):
+    """fpx_build_sources from z with bounds and fpx_leapfrog_update give the
+    potential, gradient and 8-step trajectory of the same maps written as
+    torch operations."""
+    import torch
+    ens = ensemble[0]
+    gen = torch.Generator(device=ens.device)
+    gen.manual_seed(21)
+    z0 = ens.z.clone()
+    p0 = torch.randn(ens.p.shape, generator=gen, dtype=torch.float64, device=ens.device) * ens.fmask
+    ens.eps.fill_(0.004)
+    U_f, g_f = ens.potential()
+    ens.p.copy_(p0)
+    ens.trajectory(8, use_graph=False)
+    z_f, p_f = ens.z.clone(), ens.p.clone()
+    ens.z.copy_(z0)
+    ens.p.copy_(p0)
+    U_t, g_t = _torch_trajectory(ens, 0)[0], None
+    ens.z.copy_(z0)
+    ens.p.copy_(p0)
+    _torch_trajectory(ens, 8)
+    z_t, p_t = ens.z.clone(), ens.p.clone()
+    ens.z.copy_(z0)
+    ens.p.copy_(p0)
+    U0 = ens.potential()[0]
+    np.testing.assert_allclose(U_f.cpu().numpy(), U0.cpu().numpy(), rtol=1e-12)   # atomic order
+    np.testing.assert_allclose(U_f.cpu().numpy(), U_t.cpu().numpy(), rtol=1e-9)
+    for a, b in ((z_f, z_t), (p_f, p_t)):
+        a, b = a.cpu().numpy(), b.cpu().numpy()
+        np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-6 * np.abs(b).max())
+    assert np.abs(z_f.cpu().numpy() - z0.cpu().numpy()).max() > 0

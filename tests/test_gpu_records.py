@@ -173,3 +173,41 @@ def test_stack_gradients_match_host():
     np.testing.assert_allclose(c.cpu().numpy(), chi2[:, :NB].sum(axis=1), rtol=1e-14)
     g2 = stack_grad_device(torch.tensor(resp, device="cuda"), S, NB).cpu().numpy()
     np.testing.assert_array_equal(g2, g)
+
+
+def test_bounded_records_match_transformed_theta():
+    """Records from the unconstrained z with lower / upper bounds equal the
+    records of theta = lower + sigmoid(z) (upper - lower) computed by torch
+    (shape and flux fields to one float32 rounding, amplitudes to a few)."""
+    import torch
+    from forcepho_amd.device_scene import DeviceRecords
+    from forcepho_amd.sources import LogNormalMixtureAmplitudes
+    rng = np.random.default_rng(8)
+    counts = [(3, 2), (2, 2)]
+    scenes = _scenes(LogNormalMixtureAmplitudes, rng, counts)
+    S, NB = 3, 2
+    theta = np.zeros((2, S, NB + 6))
+    for k, ((ns, nb), scene) in enumerate(zip(counts, scenes)):
+        theta[k, :ns] = scene.get_all_source_params().reshape(ns, nb + 6)
+    lo, hi = theta * 0.8 - 1e-3, theta * 1.2 + 1e-3
+    lo[..., NB + 2], hi[..., NB + 2] = 0.2, 1.0
+    z = rng.normal(0, 0.5, theta.shape)
+    dev = dict(dtype=torch.float64, device="cuda")
+    zt, lt, ht = (torch.tensor(a, **dev) for a in (z, lo, hi))
+    out = []
+    for bounded in (True, False):
+        buf = torch.zeros(2 * MAXSOURCES * 264, dtype=torch.uint8, device="cuda")
+        recs = DeviceRecords(scenes, [nb for _, nb in counts], buf, S, NB)
+        if bounded:
+            recs.write(zt, lower=lt, upper=ht)
+        else:
+            recs.write((lt + torch.sigmoid(zt) * (ht - lt)).contiguous())
+        out.append(buf.cpu().numpy().view(source_struct_dtype).reshape(2, MAXSOURCES))
+    a, b = out
+    for k, (ns, nb) in enumerate(counts):
+        for f in ("ra", "dec", "q", "pa", "nsersic", "rh", "fluxes"):
+            x, y = a[k, :ns][f].astype(np.float64), b[k, :ns][f].astype(np.float64)
+            assert np.all(np.abs(x - y) <= F32_REL * np.abs(y)), f
+        for f in ("mixture_amplitudes", "damplitude_drh", "damplitude_dnsersic"):
+            x, y = a[k, :ns][f].astype(np.float64), b[k, :ns][f].astype(np.float64)
+            assert np.all(np.abs(x - y) <= 4 * F32_REL * np.abs(y) + 1e-7 * np.abs(y).max()), f
