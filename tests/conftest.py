@@ -1,8 +1,6 @@
 import os
 import sys
 
-import pytest
-
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 TESTS = os.path.dirname(os.path.abspath(__file__))
 for p in (REPO, TESTS):

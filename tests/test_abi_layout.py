@@ -6,7 +6,6 @@ import os
 import re
 
 import numpy as np
-import pytest
 
 from forcepho_amd import _lib, kernel_limits
 from forcepho_amd.patches import PatchBase, PSF_DTYPE

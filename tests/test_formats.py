@@ -14,7 +14,7 @@ import os
 import numpy as np
 import pytest
 
-from fpx_testutil import VERIF_DIR, VERIF_IMAGE, TEST_IMAGE, TEST_IMAGE_PSF
+from fpx_testutil import VERIF_DIR, VERIF_IMAGE, TEST_IMAGE_PSF
 from forcepho_amd.formats import fits
 from forcepho_amd.formats.hdf5 import File, HDF5FormatError
 from forcepho_amd.formats.wcs import TanWCS, WCSError

@@ -8,7 +8,6 @@ import pytest
 
 import fpx_testutil as U
 import oracle
-import oracle.fp64 as fp64
 from forcepho_amd import scenes
 from forcepho_amd.model import stack_grad
 
@@ -138,7 +137,6 @@ def test_device_records_match_host_proposal():
     """The device-built Source records (DeviceRecords) equal Galaxy.proposal's
     float32 fields, and autograd through them matches the host-record path."""
     import torch
-    from forcepho_amd.kernel_limits import MAXSOURCES
     from forcepho_amd.sources import source_struct_dtype
     model, theta = _model(side=32, n_bands=3, n_sources=4)
     assert model.records is not None

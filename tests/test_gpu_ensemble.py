@@ -7,9 +7,7 @@ import pytest
 
 import fpx_testutil as U
 import oracle
-import oracle.fp64 as fp64
 from forcepho_amd import scenes
-from forcepho_amd.model import stack_grad
 
 pytestmark = pytest.mark.gpu
 

@@ -6,7 +6,6 @@ import pytest
 
 import fpx_testutil as U
 import oracle
-import oracle.fp64 as fp64
 
 pytestmark = pytest.mark.gpu
 

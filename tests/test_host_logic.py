@@ -2,7 +2,6 @@
 amplitude derivatives, stack_grad and the Posterior plumbing (driven by a
 CPU stand-in proposer built on the checker, test-only)."""
 
-import copy
 
 import numpy as np
 import pytest

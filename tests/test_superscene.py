@@ -11,7 +11,6 @@ source the target iterations and leaves nothing checked out.
 """
 
 import json
-import os
 
 import numpy as np
 import pytest
