@@ -378,18 +378,6 @@ class HIPProposer(ProposerBase):
         return d[:, :n_sources].astype(np.float64)
 
 
-    def vjp_device(self, proposal, pixel_weight, patch=None):
-        """As ``vjp`` but asynchronous: returns a float32 device tensor
-        [MAXBANDS, MAXSOURCES, NPARAMS] (a copy; bands beyond the patch's are
-        zero) without synchronising."""
-        if patch is None:
-            patch = self.patch
-        if pixel_weight.dtype.itemsize != 4 or pixel_weight.numel() < patch.npix:
-            raise ValueError("pixel_weight must be float32 over all packed pixels")
-        self.launch(proposal, patch, pixel_weight=pixel_weight)
-        return self._resp.view(MAXBANDS, MAXSOURCES, NPARAMS).clone()
-
-
 # forcepho's names for the device proposer (forcepho/proposal.py:55,155)
 GPUProposer = HIPProposer
 Proposer = HIPProposer
