@@ -91,6 +91,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise FpxError(f"{LIB_PATH} is not built; run forcepho_amd.build.build_library() "
                            "(or __graft_entry__.build()) first")
+        # torch (the host layer's device memory and streams) ships its own HIP
+        # runtime under the same soname as the system one libfpx.so links: load
+        # torch first so that the library binds to that runtime instead of
+        # bringing in a second one (torch then finds no device)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         handle = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_LOCAL)
         for name, (res, args) in _SIGNATURES.items():
             fn = getattr(handle, name)

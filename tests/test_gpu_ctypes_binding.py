@@ -1,0 +1,141 @@
+"""The C ABI bound the way a forcepho maintainer would bind it: ctypes over
+libfpx.so and the HIP runtime, device memory from hipMalloc, no torch.  The
+patch arrays and the Patch struct are uploaded as forcepho's GPUPatchMixin
+uploads them (forcepho/patches/device_patch.py:257-341), the tile table
+comes from fpx_plan_tiles, and each evaluation is one fpx_evaluate_sync call
+(the reference's one PyCUDA call, forcepho/proposal.py:133-136).  Results
+are checked against the golden fixture made by the reference's CPU kernel
+and against this package's own HIPProposer on the same patch."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import fpx_testutil as U
+from forcepho_amd import _lib
+from forcepho_amd.kernel_limits import MAXBANDS, MAXSOURCES, NPARAMS
+from forcepho_amd.proposal import response_struct_dtype, source_struct_dtype
+
+pytestmark = pytest.mark.gpu
+
+H2D, D2H = 1, 2   # hipMemcpyHostToDevice, hipMemcpyDeviceToHost
+
+
+class RawDevice:
+    """hipMalloc / hipMemcpy / hipFree through ctypes (the runtime libfpx.so
+    itself is linked against)."""
+
+    def __init__(self):
+        _lib.lib()                                    # loads libfpx.so and its HIP runtime
+        self.hip = ctypes.CDLL("libamdhip64.so.7")
+        self.ptrs = []
+
+    def malloc(self, nbytes):
+        p = ctypes.c_void_p()
+        assert self.hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(max(int(nbytes), 8))) == 0
+        self.ptrs.append(p.value)
+        return p.value
+
+    def h2d(self, dst, arr):
+        arr = np.ascontiguousarray(arr)
+        assert self.hip.hipMemcpy(ctypes.c_void_p(dst), arr.ctypes.data_as(ctypes.c_void_p),
+                                  ctypes.c_size_t(arr.nbytes), H2D) == 0
+
+    def d2h(self, arr, src):
+        assert self.hip.hipMemcpy(arr.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(src),
+                                  ctypes.c_size_t(arr.nbytes), D2H) == 0
+
+    def free(self):
+        for p in self.ptrs:
+            self.hip.hipFree(ctypes.c_void_p(p))
+        self.ptrs = []
+
+
+def _upload_patch(dev, patch):
+    """Device copies of the patch's pointer fields and the Patch struct
+    pointing at them; returns (struct address, {name: address})."""
+    pdt = patch.patch_struct_dtype
+    ptrs = {}
+    for name in pdt.names:
+        if pdt[name] != patch.ptr_dtype or name == "residual":
+            continue
+        arr = np.ascontiguousarray(getattr(patch, name))
+        ptrs[name] = dev.malloc(arr.nbytes)
+        dev.h2d(ptrs[name], arr)
+    ptrs["residual"] = dev.malloc(4 * patch.npix)
+    ps = np.zeros(1, dtype=pdt)
+    for name in pdt.names:
+        ps[name] = ptrs[name] if pdt[name] == patch.ptr_dtype else getattr(patch, name)
+    addr = dev.malloc(ps.nbytes)
+    dev.h2d(addr, ps)
+    return addr, ptrs
+
+
+def _evaluate_raw(dev, patch, proposal, cfg=5):
+    L = _lib.lib()
+    ppt = ctypes.c_int32()
+    assert L.fpx_config_info(cfg, None, None, ctypes.byref(ppt)) == 0
+    d_patch, ptrs = _upload_patch(dev, patch)
+    idx = [np.ascontiguousarray(patch.exposure_start, np.int32),
+           np.ascontiguousarray(patch.exposure_N, np.int32),
+           np.ascontiguousarray(patch.band_start, np.int16),
+           np.ascontiguousarray(patch.band_N, np.int16)]
+    args = [a.ctypes.data for a in idx] + [int(patch.n_bands), ppt.value]
+    n_tiles = L.fpx_plan_tiles(*args, None, 0)
+    assert n_tiles > 0
+    tiles = np.zeros(n_tiles, dtype=_lib.TILE_DTYPE)
+    assert L.fpx_plan_tiles(*args, tiles.ctypes.data, n_tiles) == n_tiles
+    d_tiles = dev.malloc(tiles.nbytes)
+    dev.h2d(d_tiles, tiles)
+    n_acc = L.fpx_accum_doubles(MAXBANDS)
+    d_accum = dev.malloc(8 * n_acc)
+    dev.h2d(d_accum, np.zeros(n_acc))
+    out_bytes = 8 * MAXBANDS + response_struct_dtype.itemsize * MAXBANDS
+    d_out = dev.malloc(out_bytes)
+    d_prop = dev.malloc(source_struct_dtype.itemsize * MAXSOURCES)
+    nb, ns = int(patch.n_bands), int(patch.n_sources)
+    slot = np.zeros(1, dtype=_lib.SLOT_DTYPE)
+    slot["patch"], slot["proposal"], slot["tiles"] = d_patch, d_prop, d_tiles
+    slot["pchi2"], slot["pdchi2_dp"] = d_out, d_out + 8 * MAXBANDS
+    slot["accum"], slot["n_bands"], slot["n_tiles"] = d_accum, nb, n_tiles
+    d_slot = dev.malloc(slot.nbytes)
+    dev.h2d(d_slot, slot)
+    prop = np.ascontiguousarray(proposal, dtype=source_struct_dtype)
+    out = np.zeros(out_bytes, dtype=np.uint8)
+    n_out = 8 * MAXBANDS + response_struct_dtype.itemsize * nb
+    max_gauss = ns * int(np.max(patch.n_psf_per_source))
+    rc = L.fpx_evaluate_sync(d_slot, n_tiles, max_gauss, cfg, 0, None, d_prop, prop.ctypes.data,
+                             prop.nbytes, d_out, out.ctypes.data, n_out, None)
+    assert rc == 0, L.fpx_last_error()
+    chi2 = out[:8 * nb].view(np.float64).copy()
+    derivs = out[8 * MAXBANDS:n_out].view(np.float32).reshape(nb, MAXSOURCES, NPARAMS)[:, :ns]
+    resid = np.zeros(patch.npix, dtype=np.float32)
+    dev.d2h(resid, ptrs["residual"])
+    return chi2, derivs.copy(), resid
+
+
+@pytest.mark.parametrize("name", ["multi_small_s2", "padding_s5"])
+def test_raw_ctypes_binding_matches_fixture_and_host_layer(name):
+    from forcepho_amd.proposal import HIPProposer
+    print(f"[gpu] raw ctypes binding on {name}", flush=True)
+    patch, proposal, expected = U.load_golden(name, return_residual=True)
+    dev = RawDevice()
+    try:
+        chi2, derivs, resid = _evaluate_raw(dev, patch, proposal)
+        chi2b, derivsb, residb = _evaluate_raw(dev, patch, proposal)   # a second call
+    finally:
+        dev.free()
+    rel = np.abs(chi2 - expected["chi2"]) / np.abs(expected["chi2"])
+    assert rel.max() <= U.CHI2_RTOL
+    dscale = np.max(np.abs(patch.data)) + 1e-30
+    assert np.max(np.abs(resid - expected["residual"])) <= 2e-6 * dscale
+    np.testing.assert_allclose(chi2b, chi2, rtol=1e-12)
+    np.testing.assert_array_equal(residb, resid)
+    # the torch-backed host layer on the same patch: same kernel, same results
+    patch.send_to_device()
+    c, d, r = HIPProposer(patch=patch, config=5, persistent=False).evaluate_proposal(
+        proposal, patch=patch, unpack=False)
+    np.testing.assert_allclose(chi2, c, rtol=1e-12)
+    np.testing.assert_allclose(derivs, d, rtol=1e-6, atol=1e-6 * np.abs(d).max())
+    np.testing.assert_array_equal(resid, r)
