@@ -306,6 +306,11 @@ __device__ __forceinline__ void model_pass(const float *__restrict__ recs, int w
 #pragma unroll
     for (int i = 0; i < PPL; ++i) {
       if (!(blocks & (1u << i))) continue;  // no lane has pixel i inside
+      // An empty statement the compiler may not speculate: without it the
+      // short body below is if-converted into selects and computed for every
+      // block (2% of the kernel at the multi-band workload).  It emits no
+      // instruction.
+      __asm__ volatile("");
       const float G = on[i] ? __builtin_amdgcn_exp2f(lg[i]) : 0.0f;
       const float H = fmaf(r1.z, dxx[i], fmaf(r1.w, dxy[i], fmaf(r2.x, dyy[i], r1.y)));
       m[i] = fmaf(G * r2.y, H, m[i]);
