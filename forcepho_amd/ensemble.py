@@ -81,6 +81,7 @@ class HMCEnsemble:
         self.p = torch.zeros_like(self.z)
         self.g = torch.zeros_like(self.z)
         self.U = torch.zeros(n, **f64)
+        self.U_start = torch.zeros(n, **f64)    # U at a trajectory's start point
         self.eps = torch.full((n, 1, 1), 0.01, **f64)
         self.resp = b.resp.view(n, MAXBANDS, MAXSOURCES, NPARAMS)
         self.chi2 = b.chi2.view(n, MAXBANDS)
@@ -116,18 +117,20 @@ class HMCEnsemble:
         return out
 
     # ------------------------------------------------------------------
-    def _evaluate_and_update(self, kick=0.0, drift=False, move=False):
+    def _evaluate_and_update(self, kick=0.0, drift=False, move=False, U=None):
         """Records of the current z (bounded transform on the device), one
         batched chi2 launch, then fpx_leapfrog_update: U(z) and dU/dz into
-        self.U / self.g and, with ``move``, p -= kick eps g and (with
-        ``drift``) z += eps p.  Three launches plus the batch's two."""
+        ``U`` (default self.U) / self.g and, with ``move``, p -= kick eps g
+        and (with ``drift``) z += eps p.  Three launches plus the batch's
+        two."""
+        U = self.U if U is None else U
         self.records.write(self.z, lower=self.lo, upper=self.hi)
         self.batch.launch()
         check(lib().fpx_leapfrog_update(
             self.lo.data_ptr(), self.hi.data_ptr(), self.fmask.data_ptr(), self.eps.data_ptr(),
             self.resp.data_ptr(), self.chi2.data_ptr(), self.n_slots, self.S, self.NB,
             float(kick), int(drift), self.z.data_ptr(), self.p.data_ptr() if move else None,
-            self.U.data_ptr(), self.g.data_ptr(),
+            U.data_ptr(), self.g.data_ptr(),
             _torch().cuda.current_stream(self.device).cuda_stream), "fpx_leapfrog_update")
 
     def _potential_and_grad(self):
@@ -135,9 +138,12 @@ class HMCEnsemble:
         self._evaluate_and_update()
 
     def _trajectory(self, n_steps):
-        # potential at the start, half kick, drift; then per step the
-        # potential and a full kick + drift, the last one a half kick
-        self._evaluate_and_update(0.5, drift=True, move=True)
+        # potential at the start (into U_start), half kick, drift; then per
+        # step the potential and a full kick + drift, the last one a half
+        # kick: U then holds the potential at the end point
+        if n_steps < 1:
+            raise ValueError("a trajectory has at least one leapfrog step")
+        self._evaluate_and_update(0.5, drift=True, move=True, U=self.U_start)
         for i in range(n_steps):
             last = i == n_steps - 1
             self._evaluate_and_update(0.5 if last else 1.0, drift=not last, move=True)
@@ -181,12 +187,11 @@ class HMCEnsemble:
         p0 = torch.randn(self.p.shape, generator=generator, dtype=torch.float64,
                          device=self.device) * self.fmask
         z0 = self.z.clone()
-        U0, _ = self.potential()
         self.p.copy_(p0)
         self.trajectory(n_steps, use_graph=use_graph)
-        U1, _ = self.potential()
-        H0 = U0 + 0.5 * (p0 * p0).sum(dim=(1, 2))
-        H1 = U1 + 0.5 * (self.p * self.p).sum(dim=(1, 2))
+        # the trajectory's first and last evaluations are at z0 and at its end
+        H0 = self.U_start + 0.5 * (p0 * p0).sum(dim=(1, 2))
+        H1 = self.U + 0.5 * (self.p * self.p).sum(dim=(1, 2))
         u = torch.rand(self.n_slots, generator=generator, dtype=torch.float64, device=self.device)
         accept = torch.log(u) < (H0 - H1)
         self.z.copy_(torch.where(accept[:, None, None], self.z, z0))

@@ -253,3 +253,31 @@ def test_fused_leapfrog_matches_torch_formulation(ensemble):
         a, b = a.cpu().numpy(), b.cpu().numpy()
         np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-6 * np.abs(b).max())
     assert np.abs(z_f.cpu().numpy() - z0.cpu().numpy()).max() > 0
+
+
+def test_trajectory_end_point_potentials(ensemble):
+    """The Metropolis energies come from the trajectory's own first and last
+    evaluations: U_start is U at the start point and U at the end point,
+    eager and replayed from the recorded graph alike."""
+    import torch
+    ens = ensemble[0]
+    gen = torch.Generator(device=ens.device)
+    gen.manual_seed(33)
+    z0 = ens.z.clone()
+    p0 = torch.randn(ens.p.shape, generator=gen, dtype=torch.float64, device=ens.device) * ens.fmask
+    ens.eps.fill_(0.004)
+    U0 = ens.potential()[0].clone()
+    ends = []
+    for use_graph in (False, True):
+        ens.z.copy_(z0)
+        ens.p.copy_(p0)
+        ens.trajectory(6, use_graph=use_graph)
+        Us, Ue = ens.U_start.clone(), ens.U.clone()
+        U1 = ens.potential()[0].clone()
+        np.testing.assert_allclose(Us.cpu().numpy(), U0.cpu().numpy(), rtol=1e-12)
+        np.testing.assert_allclose(Ue.cpu().numpy(), U1.cpu().numpy(), rtol=1e-12)
+        ends.append(ens.z.clone())
+    np.testing.assert_allclose(ends[0].cpu().numpy(), ends[1].cpu().numpy(), rtol=1e-12, atol=1e-14)
+    ens.z.copy_(z0)
+    with pytest.raises(ValueError):
+        ens.trajectory(0, use_graph=False)
