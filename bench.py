@@ -247,12 +247,15 @@ def autograd_measurement(args, patch, scene):
 
     traj_s, calls = trajectory(model.chi2)
     traj_resid_s, _ = trajectory(model.chi2_from_residual)
+    post_dev_s, post_host_s = posterior_measurement(patch, scene, theta, nb)
     n_evals = patch.n_evals()
     return {"workload": "torch.autograd chi2 of the multi-band patch (fused: one chi2-mode "
                         "launch gives value and gradient)",
             "value_and_grad_per_s": 1.0 / call_s, "ms_per_value_and_grad": call_s * 1e3,
             "residual_path_ms_per_value_and_grad": resid_s * 1e3,
             "residual_path_trajectory_s": traj_resid_s,
+            "fastposterior_ms_per_call": post_dev_s * 1e3,
+            "fastposterior_host_records_ms_per_call": post_host_s * 1e3,
             "evals_per_s": n_evals / call_s,
             "leapfrog_steps": args.leapfrog_steps, "gradient_calls": calls,
             "trajectory_s": traj_s, "trajectories_per_s": 1.0 / traj_s}
@@ -365,6 +368,30 @@ def small_ensemble_measurement(args, n_patches=16, chains=4, side=64, n_bands=3,
             "chains": ens.n_slots, "trajectory_s": traj_s,
             "trajectories_per_s": ens.n_slots / traj_s,
             "us_per_step": traj_s / steps * 1e6, "batch_launch_us": launch_s * 1e6}
+
+
+def posterior_measurement(patch, scene, theta, nb, n=50):
+    """forcepho's own API on the multi-band patch: seconds per
+    FastPosterior.lnprob_and_grad call (bounded transform around theta),
+    with the records built on the device and with Scene.get_proposal on the
+    host."""
+    from forcepho_amd.model import FastPosterior, BoundedTransform
+    from forcepho_amd.proposal import HIPProposer
+    tr = BoundedTransform(*ensemble_bounds(theta, nb))
+    z0 = tr.inverse_transform(theta.reshape(-1))
+    out = []
+    saved, patch.return_residual = patch.return_residual, False   # chi2 and gradient only
+    for device_records in (None, False):
+        post = FastPosterior(HIPProposer(patch=patch), patch, scene=scene, transform=tr,
+                             device_records=device_records)
+        for i in range(3):
+            post.lnprob_and_grad(z0 + 1e-9 * (i + 1))
+        t0 = time.perf_counter()
+        for i in range(n):
+            post.lnprob_and_grad(z0 + 1e-9 * (i + 10))     # a new position every call
+        out.append((time.perf_counter() - t0) / n)
+    patch.return_residual = saved
+    return out[0], out[1]
 
 
 def time_region(fn, steps):

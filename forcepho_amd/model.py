@@ -176,11 +176,19 @@ class Posterior:
 
 
 class FastPosterior(Posterior):
-    """Posterior evaluated by a device proposer on a packed patch."""
+    """Posterior evaluated by a device proposer on a packed patch.
+
+    ``device_records``: build the Source records of each evaluation on the
+    device (fpx_build_sources) instead of through ``Scene.get_proposal`` on
+    the host, and stack the gradient there (fpx_stack_gradients); one copy
+    of chi2 and the gradient comes back.  Default: on when the proposer is
+    the HIP proposer and the scene's amplitude model can be evaluated on the
+    device (log-normal, or one shared spline model).  Evaluations that return
+    residual images (``patch.return_residual``) use the host records."""
 
     def __init__(self, proposer, patch, scene=None, lnprior=None, transform=None,
                  name="", print_interval=1000, verbose=False, debug=False,
-                 logging=False, sampling=True, **kwargs):
+                 logging=False, sampling=True, device_records=None, **kwargs):
         super().__init__(transform=transform, **kwargs)
         self.proposer = proposer
         self.patch = patch
@@ -192,19 +200,70 @@ class FastPosterior(Posterior):
         self.print_interval, self.name = print_interval, name
         if logging:
             self.pos_history, self.lnp_history, self.grad_history = [], [], []
+        self.device_records = device_records
+        self._dev = None
+
+    def _use_device_records(self):
+        if self.device_records is False or getattr(self.patch, "return_residual", False):
+            return False
+        from .proposal import HIPProposer
+        if not isinstance(self.proposer, HIPProposer):
+            return False
+        if self.device_records is None:
+            from .device_scene import device_amplitude_models_ok
+            self.device_records = device_amplitude_models_ok(self.scene)
+        return bool(self.device_records)
+
+    def _device_chi2(self, q):
+        """chi2 summed over bands and d chi2 / dq of scene parameters q from
+        records built on the device: one H2D copy, record build, chi2
+        launch, gradient stacking, one D2H copy."""
+        import torch
+        from .device_scene import DeviceRecords, stack_grad_device
+        from .kernel_limits import MAXBANDS, MAXSOURCES
+        p = self.proposer
+        ns, nb = int(self.patch.n_sources), int(self.patch.n_bands)
+        if self._dev is None:
+            pin = dict(dtype=torch.float64, pin_memory=True)
+            buf = torch.zeros(MAXSOURCES * 264, dtype=torch.uint8, device=p.device)
+            self._dev = dict(
+                records=DeviceRecords([self.scene], [nb], buf, ns, nb),
+                theta_h=torch.zeros(ns * (nb + 6), **pin),
+                theta_d=torch.zeros(ns * (nb + 6), dtype=torch.float64, device=p.device),
+                out_d=torch.zeros(1 + ns * (nb + 6), dtype=torch.float64, device=p.device),
+                out_h=torch.zeros(1 + ns * (nb + 6), **pin))
+        d = self._dev
+        stream = p._stream()
+        d["theta_h"].numpy()[:] = q
+        d["theta_d"].copy_(d["theta_h"], non_blocking=True)
+        d["records"].write(d["theta_d"].view(1, ns, nb + 6))
+        p.launch(d["records"].records, self.patch, stream=stream)
+        g, c = stack_grad_device(p._resp.view(1, MAXBANDS, MAXSOURCES, NPARAMS), ns, nb,
+                                 chi2=p._chi2.view(1, MAXBANDS))
+        d["out_d"][:1].copy_(c)
+        d["out_d"][1:].copy_(g.view(-1))
+        d["out_h"].copy_(d["out_d"], non_blocking=True)
+        stream.synchronize()
+        out = d["out_h"].numpy()
+        return float(out[0]), out[1:].copy()
 
     def evaluate(self, z):
         q = self.make_transform(z)
         self.scene.set_all_source_params(q)
-        proposal = self.scene.get_proposal()
         lpr, lpr_grad = self.lnprior(q)
-        ret = self.proposer.evaluate_proposal(proposal, patch=self.patch)
-        if len(ret) == 3:
-            chi2, chi2_derivs, self._residuals = ret
+        if self._use_device_records():
+            chi2_sum, dchi2 = self._device_chi2(np.asarray(q, dtype=np.float64))
+            ll = self.mhalf * np.array(chi2_sum, dtype=np.float64)
+            ll_grad = self.mhalf * dchi2
         else:
-            chi2, chi2_derivs = ret
-        ll = self.mhalf * np.array(chi2.astype(np.float64).sum(), dtype=np.float64)
-        ll_grad = self.mhalf * self.stack_grad(chi2_derivs)
+            proposal = self.scene.get_proposal()
+            ret = self.proposer.evaluate_proposal(proposal, patch=self.patch)
+            if len(ret) == 3:
+                chi2, chi2_derivs, self._residuals = ret
+            else:
+                chi2, chi2_derivs = ret
+            ll = self.mhalf * np.array(chi2.astype(np.float64).sum(), dtype=np.float64)
+            ll_grad = self.mhalf * self.stack_grad(chi2_derivs)
         self.ncall += 1
         if self.logging:
             self.pos_history.append(z)

@@ -64,7 +64,7 @@ def test_potential_matches_host_posterior(ensemble):
         patch = pats[pi]
         nb = int(patch.n_bands)
         post = FastPosterior(HIPProposer(patch=patch), patch, scene=scs[pi],
-                             lower=bnds[pi][0], upper=bnds[pi][1])
+                             lower=bnds[pi][0], upper=bnds[pi][1], device_records=False)
         z = post.transform.inverse_transform(start[s])
         lnp, grad = post.lnprob_and_grad(z)
         assert Ud[s] == pytest.approx(-lnp, rel=1e-6)

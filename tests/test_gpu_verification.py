@@ -166,7 +166,7 @@ def test_ensemble_on_real_data_with_spline_amplitudes():
     for s, (pi, _) in enumerate(ens.batch.slot_map):
         patch = pats[pi]
         post = FastPosterior(HIPProposer(patch=patch), patch, scene=scs[pi],
-                             lower=bnds[pi][0], upper=bnds[pi][1])
+                             lower=bnds[pi][0], upper=bnds[pi][1], device_records=False)
         lnp, grad = post.lnprob_and_grad(post.transform.inverse_transform(thetas[s]))
         assert Ud[s] == pytest.approx(-lnp, rel=1e-6)
         g = np.concatenate([gd[s, :, :1], gd[s, :, ens.NB:]], axis=1)[:patch.n_sources].reshape(-1)
