@@ -1,7 +1,8 @@
 """torch.autograd over the patch model: the residual image's backward (the
 kernel's gradient pass with dL/dresidual as pixel weights) and the fused chi2
 (value and gradient of one chi2-mode launch) against each other, the float64
-oracle and central differences; a 32-step leapfrog driven by them."""
+oracle, the reference's own CPU kernel and central differences; a 32-step
+leapfrog driven by them."""
 
 import numpy as np
 import pytest
@@ -59,6 +60,16 @@ def test_autograd_matches_kernel_gradient_and_fp64():
     assert np.all(np.abs(g_auto - g64s) <= tol)
     assert np.all(np.abs(g_kernel - g64s) <= tol)
     assert np.all(np.abs(g_fused - g64s) <= tol)
+    # the reference's own CPU kernel on the same proposal (where it is built):
+    # inside the same band, and the autograd gradients within both errors of it
+    mod = oracle.reference_module()
+    if mod is not None:
+        c_ref, d_ref, _ = oracle.evaluate_reference(model.patch, model.proposal(th), module=mod)
+        g_ref = stack_grad(d_ref.astype(np.float64), model.n_sources, model.n_bands)
+        np.testing.assert_allclose(fused.item(), c_ref.sum(), rtol=1e-6)
+        assert np.all(np.abs(g_ref - g64s) <= tol)
+        assert np.all(np.abs(g_auto - g_ref) <= 2 * tol)
+        assert np.all(np.abs(g_fused - g_ref) <= 2 * tol)
 
 
 def test_autograd_central_differences_full_size():
