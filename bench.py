@@ -328,7 +328,9 @@ def small_ensemble_measurement(args, n_patches=16, chains=4, side=64, n_bands=3,
     """A dispatcher-sized ensemble (16 small regions of 64x64 px in 3 bands
     with 4 sources, 4 chains each): time per leapfrog step of the recorded
     trajectory against the batched chi2 launch alone, i.e. what the record
-    build, transform, potential and updates add per step."""
+    build, transform, potential and updates add per step; and a whole HMC
+    transition (momenta, trajectory, Metropolis test) against the trajectory
+    alone."""
     import torch
     from forcepho_amd import scenes
     from forcepho_amd.patches import SimplePatch
@@ -361,13 +363,16 @@ def small_ensemble_measurement(args, n_patches=16, chains=4, side=64, n_bands=3,
     ens.capture(args.leapfrog_steps)
     traj_s = time_region(ens.graph.replay, 10)
     launch_s = time_region(ens.batch.launch, 50)
+    ens.hmc_step(args.leapfrog_steps, generator=gen)
+    hmc_s = time_region(lambda: ens.hmc_step(args.leapfrog_steps, generator=gen), 10)
     steps = args.leapfrog_steps + 1
     return {"workload": f"{n_patches} regions of {side}x{side} px, {n_bands} bands, "
                         f"{n_sources} sources, {chains} chains each, graph-captured "
                         f"{args.leapfrog_steps}-step leapfrog",
             "chains": ens.n_slots, "trajectory_s": traj_s,
             "trajectories_per_s": ens.n_slots / traj_s,
-            "us_per_step": traj_s / steps * 1e6, "batch_launch_us": launch_s * 1e6}
+            "us_per_step": traj_s / steps * 1e6, "batch_launch_us": launch_s * 1e6,
+            "hmc_transition_ms": hmc_s * 1e3, "trajectory_ms": traj_s * 1e3}
 
 
 def posterior_measurement(patch, scene, theta, nb, n=50):

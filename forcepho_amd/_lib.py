@@ -73,6 +73,7 @@ _SIGNATURES = {
     "fpx_leapfrog_update": (_c_int, [_p, _p, _p, _p, _p, _p, _c_int, _c_int, _c_int,
                                      ctypes.c_double, _c_int, _p, _p, _p, _p, _p]),
     "fpx_stack_gradients": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _p, _p, _p]),
+    "fpx_hmc_metropolis": (_c_int, [_p, _p, _p, _p, _p, _p, _c_int, _c_int, _p, _p, _p]),
     "fpx_probe_valu": (_c_int, [_c_int, _c_int, _p, _c_int, _p, _p]),
 }
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)

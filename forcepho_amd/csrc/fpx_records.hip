@@ -291,6 +291,50 @@ __global__ __launch_bounds__(kStepThreads) void leapfrog_kernel(
   }
 }
 
+// Metropolis test of one HMC transition per chain (one workgroup per chain):
+// H = U + |p|^2 / 2 at the start (U0, p0) and at the end (U1, p1) of the
+// trajectory; accept when ln u < H0 - H1 (a NaN energy rejects); a rejected
+// chain's position goes back to z0.
+__global__ __launch_bounds__(kStepThreads) void metropolis_kernel(
+    const double *__restrict__ U0, const double *__restrict__ U1,
+    const double *__restrict__ p0, const double *__restrict__ p1,
+    const double *__restrict__ u, const double *__restrict__ z0, int n, double *__restrict__ z,
+    int32_t *__restrict__ accept) {
+  __shared__ double part[2][kStepThreads / 64];
+  __shared__ int ok;
+  const int k = blockIdx.x;
+  const size_t base = (size_t)k * n;
+  double k0 = 0.0, k1 = 0.0;
+  for (int i = threadIdx.x; i < n; i += kStepThreads) {
+    k0 += p0[base + i] * p0[base + i];
+    k1 += p1[base + i] * p1[base + i];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    k0 += __shfl_xor(k0, off, 64);
+    k1 += __shfl_xor(k1, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    part[0][threadIdx.x >> 6] = k0;
+    part[1][threadIdx.x >> 6] = k1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int w = 0; w < kStepThreads / 64; ++w) {
+      s0 += part[0][w];
+      s1 += part[1][w];
+    }
+    const double H0 = U0[k] + 0.5 * s0;
+    const double H1 = U1[k] + 0.5 * s1;
+    ok = log(u[k]) < H0 - H1 ? 1 : 0;  // false for NaN
+    accept[k] = ok;
+  }
+  __syncthreads();
+  if (!ok)
+    for (int i = threadIdx.x; i < n; i += kStepThreads) z[base + i] = z0[base + i];
+}
+
 int check_launch(const char *where) {
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? FPX_OK : report_hip_error(e, where);
@@ -383,6 +427,19 @@ int fpx_leapfrog_update(const double *lower, const double *upper, const double *
                      reinterpret_cast<const float *>(resp), chi2, n_src_max, n_bands_max, kick,
                      drift ? 1 : 0, z, p, U, g);
   return fpx::check_launch("leapfrog_kernel");
+}
+
+int fpx_hmc_metropolis(const double *U0, const double *U1, const double *p0,
+                       const double *p1, const double *u, const double *z0, int32_t n_prop,
+                       int32_t n_per, double *z, int32_t *accept, void *stream) {
+  using fpx::report_error;
+  if (n_prop < 0 || n_per < 0) return report_error(FPX_E_ARG, "negative size");
+  if (!U0 || !U1 || !p0 || !p1 || !u || !z0 || !z || !accept)
+    return report_error(FPX_E_ARG, "NULL array");
+  if (n_prop == 0) return FPX_OK;
+  hipLaunchKernelGGL(fpx::metropolis_kernel, dim3(n_prop), dim3(fpx::kStepThreads), 0,
+                     (hipStream_t)stream, U0, U1, p0, p1, u, z0, n_per, z, accept);
+  return fpx::check_launch("metropolis_kernel");
 }
 
 }  // extern "C"

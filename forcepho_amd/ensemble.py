@@ -182,20 +182,24 @@ class HMCEnsemble:
 
     def hmc_step(self, n_steps, generator=None, use_graph=True):
         """One HMC transition of every chain: fresh momenta, a leapfrog
-        trajectory, Metropolis accept/reject.  Returns acceptance flags."""
+        trajectory, Metropolis accept/reject (fpx_hmc_metropolis: the
+        energies at the trajectory's two end points, rejected chains back to
+        their start).  Returns acceptance flags (bool device tensor)."""
         torch = _torch()
         p0 = torch.randn(self.p.shape, generator=generator, dtype=torch.float64,
-                         device=self.device) * self.fmask
+                         device=self.device)
+        p0.mul_(self.fmask)
         z0 = self.z.clone()
         self.p.copy_(p0)
         self.trajectory(n_steps, use_graph=use_graph)
-        # the trajectory's first and last evaluations are at z0 and at its end
-        H0 = self.U_start + 0.5 * (p0 * p0).sum(dim=(1, 2))
-        H1 = self.U + 0.5 * (self.p * self.p).sum(dim=(1, 2))
         u = torch.rand(self.n_slots, generator=generator, dtype=torch.float64, device=self.device)
-        accept = torch.log(u) < (H0 - H1)
-        self.z.copy_(torch.where(accept[:, None, None], self.z, z0))
-        return accept
+        accept = torch.empty(self.n_slots, dtype=torch.int32, device=self.device)
+        check(lib().fpx_hmc_metropolis(
+            self.U_start.data_ptr(), self.U.data_ptr(), p0.data_ptr(), self.p.data_ptr(),
+            u.data_ptr(), z0.data_ptr(), self.n_slots, self.S * self.K, self.z.data_ptr(),
+            accept.data_ptr(), torch.cuda.current_stream(self.device).cuda_stream),
+            "fpx_hmc_metropolis")
+        return accept.bool()
 
 
 def torch_sigmoid(x):

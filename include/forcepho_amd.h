@@ -369,6 +369,18 @@ int fpx_leapfrog_update(const double *lower, const double *upper, const double *
                         double kick, int32_t drift, double *z, double *p, double *U,
                         double *g, void *stream);
 
+/* Metropolis test of one HMC transition of n_prop chains of n_per
+ * parameters each (no counterpart in the reference, whose sampler tests one
+ * chain on the host): H = U + |p|^2 / 2 at the start (U0, p0) and the end
+ * (U1, p1) of the trajectory; accept[k] = ln u[k] < H0 - H1 (a NaN energy
+ * rejects); a rejected chain's z is set back to z0.
+ *   U0, U1, u : device double [n_prop]; p0, p1, z0, z : device double
+ *   [n_prop][n_per]; accept : device int32 [n_prop] */
+int fpx_hmc_metropolis(const double *U0, const double *U1, const double *p0,
+                       const double *p1, const double *u, const double *z0,
+                       int32_t n_prop, int32_t n_per, double *z, int32_t *accept,
+                       void *stream);
+
 /* Throughput probe of the VALU pipes used by the hot loop: kind 0 = FP32
  * FMA, kind 1 = v_exp_f32, 8 independent chains of `iters` steps per lane.
  * Writes one float per thread to out (device, n_threads floats): lane 0 of

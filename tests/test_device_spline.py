@@ -138,3 +138,12 @@ def test_leapfrog_update_argument_checks():
     assert call(z=None) == (-1, "NULL array")
     assert call(drift=1, mom=None) == (-1, "drift needs the momenta")
     assert call(n=0)[0] == 0
+
+
+def test_hmc_metropolis_argument_checks():
+    L = _lib.lib()
+    p = ctypes.c_void_p(8)
+    f = L.fpx_hmc_metropolis
+    assert _rc(f, p, p, p, p, p, p, -1, 4, p, p, None)[0] == -1
+    assert _rc(f, p, p, p, p, None, p, 1, 4, p, p, None) == (-1, "NULL array")
+    assert _rc(f, p, p, p, p, p, p, 0, 4, p, p, None)[0] == 0

@@ -281,3 +281,36 @@ def test_trajectory_end_point_potentials(ensemble):
     ens.z.copy_(z0)
     with pytest.raises(ValueError):
         ens.trajectory(0, use_graph=False)
+
+
+def test_metropolis_kernel_matches_torch():
+    """fpx_hmc_metropolis against the test written in torch: accept where
+    ln u < H0 - H1 with H = U + |p|^2 / 2, rejected chains back to z0, a NaN
+    energy rejects."""
+    import torch
+    from forcepho_amd._lib import lib, check
+    rng = np.random.default_rng(17)
+    n, m = 37, 53
+    dev = dict(dtype=torch.float64, device="cuda")
+    U0 = torch.tensor(rng.normal(0, 2, n), **dev)
+    U1 = U0 + torch.tensor(rng.normal(0, 1, n), **dev)
+    U1[3] = float("nan")
+    p0 = torch.tensor(rng.normal(size=(n, m)), **dev)
+    p1 = p0 + torch.tensor(rng.normal(0, 0.1, (n, m)), **dev)
+    u = torch.tensor(rng.uniform(size=n), **dev)
+    z0 = torch.tensor(rng.normal(size=(n, m)), **dev)
+    z = torch.tensor(rng.normal(size=(n, m)), **dev)
+    z_in = z.clone()
+    acc = torch.empty(n, dtype=torch.int32, device="cuda")
+    check(lib().fpx_hmc_metropolis(U0.data_ptr(), U1.data_ptr(), p0.data_ptr(), p1.data_ptr(),
+                                   u.data_ptr(), z0.data_ptr(), n, m, z.data_ptr(),
+                                   acc.data_ptr(), torch.cuda.current_stream().cuda_stream),
+          "fpx_hmc_metropolis")
+    H0 = U0 + 0.5 * (p0 * p0).sum(dim=1)
+    H1 = U1 + 0.5 * (p1 * p1).sum(dim=1)
+    want = (torch.log(u) < (H0 - H1)).cpu().numpy()
+    got = acc.cpu().numpy().astype(bool)
+    assert 0 < want.sum() < n and not want[3]
+    np.testing.assert_array_equal(got, want)
+    expect_z = torch.where(torch.tensor(want, device="cuda")[:, None], z_in, z0)
+    np.testing.assert_array_equal(z.cpu().numpy(), expect_z.cpu().numpy())
