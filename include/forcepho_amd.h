@@ -370,8 +370,9 @@ int fpx_leapfrog_update(const double *lower, const double *upper, const double *
                         double *g, void *stream);
 
 /* Metropolis test of one HMC transition of n_prop chains of n_per
- * parameters each (no counterpart in the reference, whose sampler tests one
- * chain on the host): H = U + |p|^2 / 2 at the start (U0, p0) and the end
+ * parameters each (no counterpart in the reference, whose NUTS sampler,
+ * littlemcmc, picks among the points of one chain's trajectory on the
+ * host): H = U + |p|^2 / 2 at the start (U0, p0) and the end
  * (U1, p1) of the trajectory; accept[k] = ln u[k] < H0 - H1 (a NaN energy
  * rejects); a rejected chain's z is set back to z0.
  *   U0, U1, u : device double [n_prop]; p0, p1, z0, z : device double
