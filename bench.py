@@ -32,6 +32,7 @@ FLOP_PER_EVAL = 117          # FP32 flop per pair inside the cutoff, both passes
 FLOP_OUTSIDE = 22            # flop per pair beyond it: dx, dy, F dx, dx^T F dx in each pass
 PEAK_FP32 = 157.3e12         # MI355X FP32 vector peak, FLOP/s
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "chi2_grad_kernel_multi_band_traffic.txt")
+BATCH_TRAFFIC_FILE = os.path.join(REPO, "profiles", "chi2_grad_kernel_batched_uneven_traffic.txt")
 
 
 def parse():
@@ -128,17 +129,17 @@ def roofline_line(base, traffic_bytes, seconds, n_inside, probe):
     return line
 
 
-def traffic(cfg):
-    """HBM bytes per launch of the multi-band workload for kernel configuration
-    cfg, from the committed FETCH_SIZE / WRITE_SIZE summary (None if that
-    configuration was not profiled)."""
+def traffic(cfg, path=TRAFFIC_FILE):
+    """HBM bytes per launch of a workload for kernel configuration cfg, from
+    its committed FETCH_SIZE / WRITE_SIZE summary (the multi-band workload's
+    by default; None if that configuration was not profiled)."""
     import ctypes
     from forcepho_amd._lib import lib, check
     w, p = ctypes.c_int32(), ctypes.c_int32()
     check(lib().fpx_config_info(cfg, ctypes.byref(w), ctypes.byref(p), None), "fpx_config_info")
     key = f"{w.value}x{p.value}"
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             rows = [json.loads(l) for l in f if l.startswith("{")]
     except OSError:
         return None
@@ -464,7 +465,10 @@ def batched_measurement(args, plain):
         "stamp_side_range": [min(sides), max(sides)],
         "evals_per_s": n_evals / batch_s,
         "ms_per_step": batch_s * 1e3,
-        "roofline": roofline(n_evals, n_inside, batch_s),
+        # measured traffic applies to the profiled workload (512 patches, seed 0) only
+        "roofline": dict(roofline(n_evals, n_inside, batch_s),
+                         traffic=(traffic(ev.config, BATCH_TRAFFIC_FILE)
+                                  if args.batch_patches == 512 and args.seed == 0 else None)),
         "kernel_config": "{}{}".format(ev.config, "p" if ev.persistent else ""),
         "config_ms": {f"{c}{'p' if p else ''}": t * 1e3 for (c, p), t in tuned.items()},
         "sequential_plain_launches": {"evals_per_s": n_evals / seq_s, "ms": seq_s * 1e3},
