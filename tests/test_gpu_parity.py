@@ -233,3 +233,33 @@ def test_full_size_multi_band_against_fp64():
     err = U.grad_errors(d, ref["dchi2"], ref["mass"], ref["smass"])
     print("full-size gradient error / tolerance: max", err.max(), "median", np.median(err))
     assert err.max() <= 1.0
+
+
+@pytest.mark.parametrize("name", ["multi_small_s2", "padding_s5"])
+def test_pixel_order_independence(name):
+    """Pixels shuffled at random within each exposure (tiles then hold
+    scattered pixels): chi2, gradients and the residual image (put back in
+    the original order) agree with the packed order, up to float32
+    summation order."""
+    from forcepho_amd.proposal import HIPProposer
+    patch, proposal, expected = U.load_golden(name, return_residual=True)
+    patch.send_to_device()
+    c0, d0, r0 = HIPProposer(patch=patch).evaluate_proposal(proposal, patch=patch, unpack=False)
+    shuffled, _, _ = U.load_golden(name, return_residual=True)
+    rng = np.random.default_rng(4)
+    perm = np.arange(shuffled.npix)
+    for e in range(int(shuffled.n_exp)):
+        s, n = int(shuffled.exposure_start[e]), int(shuffled.exposure_N[e])
+        perm[s:s + n] = s + rng.permutation(n)
+    for f in ("xpix", "ypix", "data", "ierr"):
+        setattr(shuffled, f, np.ascontiguousarray(getattr(shuffled, f)[perm]))
+    shuffled.send_to_device()
+    c1, d1, r1 = HIPProposer(patch=shuffled).evaluate_proposal(proposal, patch=shuffled,
+                                                                unpack=False)
+    np.testing.assert_allclose(c1, c0, rtol=1e-6)
+    _, g64, mass, smass = U.fp64_reference(patch, proposal)
+    assert U.grad_errors(d1, g64, mass, smass).max() <= 1.0
+    assert np.abs(d1 - d0).max() <= 1e-4 * np.abs(d0).max()
+    back = np.empty_like(r1)
+    back[perm] = r1
+    assert np.abs(back - r0).max() <= 2e-6 * np.abs(patch.data).max()
