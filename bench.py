@@ -33,6 +33,8 @@ FLOP_OUTSIDE = 22            # flop per pair beyond it: dx, dy, F dx, dx^T F dx 
 PEAK_FP32 = 157.3e12         # MI355X FP32 vector peak, FLOP/s
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "chi2_grad_kernel_multi_band_traffic.txt")
 BATCH_TRAFFIC_FILE = os.path.join(REPO, "profiles", "chi2_grad_kernel_batched_uneven_traffic.txt")
+VALU_FILE = os.path.join(REPO, "profiles", "chi2_grad_kernel_multi_band_valu.txt")
+BATCH_VALU_FILE = os.path.join(REPO, "profiles", "chi2_grad_kernel_batched_uneven_valu.txt")
 
 
 def parse():
@@ -129,10 +131,9 @@ def roofline_line(base, traffic_bytes, seconds, n_inside, probe):
     return line
 
 
-def traffic(cfg, path=TRAFFIC_FILE):
-    """HBM bytes per launch of a workload for kernel configuration cfg, from
-    its committed FETCH_SIZE / WRITE_SIZE summary (the multi-band workload's
-    by default; None if that configuration was not profiled)."""
+def profile_record(cfg, path):
+    """The JSON record of kernel configuration cfg in a committed rocprof
+    summary (None if the file or that configuration's record is absent)."""
     import ctypes
     from forcepho_amd._lib import lib, check
     w, p = ctypes.c_int32(), ctypes.c_int32()
@@ -145,8 +146,32 @@ def traffic(cfg, path=TRAFFIC_FILE):
         return None
     for r in rows:
         if r.get("kernel") == key:
-            return r["hbm_bytes_per_launch"]
+            return r
     return None
+
+
+def traffic(cfg, path=TRAFFIC_FILE):
+    """HBM bytes per launch of a workload for kernel configuration cfg, from
+    its committed FETCH_SIZE / WRITE_SIZE summary (the multi-band workload's
+    by default; None if that configuration was not profiled)."""
+    r = profile_record(cfg, path)
+    return None if r is None else r["hbm_bytes_per_launch"]
+
+
+def executed(cfg, seconds, path=VALU_FILE):
+    """FP32 flop and exponentials the kernel issues per launch (rocprof's
+    SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F32 x 64 lanes, from the committed
+    summary of the workload) over this run's kernel time: the work the
+    moment-form kernel actually does, next to the reference algorithm's
+    flops that the roofline credits.  None if not profiled."""
+    r = profile_record(cfg, path)
+    if r is None:
+        return None
+    tflops = r["executed_flop_per_launch"] / seconds / 1e12
+    return {"flop_per_launch": r["executed_flop_per_launch"], "tflops": tflops,
+            "frac": tflops * 1e12 / PEAK_FP32, "exp_per_launch": r["exp_per_launch"],
+            "exp_per_s": r["exp_per_launch"] / seconds,
+            "valu_wave_insts_per_launch": r["valu"]}
 
 
 def build_uneven(n_patches, seed, with_scenes=False):
@@ -457,6 +482,8 @@ def batched_measurement(args, plain):
     sides = [int(np.sqrt(p.original_sizes[0])) for p in patches]
     desc = (f"{len(patches)} uneven patches (1-20 sources, 1-8 bands x 1-4 exposures, "
             f"48-256 px stamps, 3-6 PSF Gaussians, 9 radii) in one persistent launch")
+    # the committed counter summaries describe the profiled workload (512 patches, seed 0) only
+    profiled = args.batch_patches == 512 and args.seed == 0
     return {
         "workload": desc,
         "n_patches": len(patches),
@@ -465,10 +492,10 @@ def batched_measurement(args, plain):
         "stamp_side_range": [min(sides), max(sides)],
         "evals_per_s": n_evals / batch_s,
         "ms_per_step": batch_s * 1e3,
-        # measured traffic applies to the profiled workload (512 patches, seed 0) only
         "roofline": dict(roofline(n_evals, n_inside, batch_s),
-                         traffic=(traffic(ev.config, BATCH_TRAFFIC_FILE)
-                                  if args.batch_patches == 512 and args.seed == 0 else None)),
+                         traffic=(traffic(ev.config, BATCH_TRAFFIC_FILE) if profiled else None),
+                         executed=(executed(ev.config, batch_s, BATCH_VALU_FILE)
+                                   if profiled else None)),
         "kernel_config": "{}{}".format(ev.config, "p" if ev.persistent else ""),
         "config_ms": {f"{c}{'p' if p else ''}": t * 1e3 for (c, p), t in tuned.items()},
         "sequential_plain_launches": {"evals_per_s": n_evals / seq_s, "ms": seq_s * 1e3},
@@ -549,6 +576,8 @@ def main():
     proposal = scene.get_proposal()
     proposal["q"] *= np.float32(0.97)   # off the truth: gradients are generic
     n_evals = patch.n_evals()
+    # the committed counter summaries describe the multi-band workload at seed 0 only
+    profiled = args.workload == "multi" and args.seed == 0
 
     # pick the kernel configuration (tile shape, persistent or not) by measurement
     raw = np.ascontiguousarray(proposal).view(np.uint8).reshape(-1)
@@ -621,8 +650,10 @@ def main():
         "config": {"workload": desc, "evals_per_step": n_evals, "path": "plain",
                    "kernel_config": best, "config_ms": {str(k): v * 1e3 for k, v in timings.items()},
                    "wall_s": wall},
-        "roofline": roofline_line(roofline(n_evals, n_inside, kernel_s), traffic(prop.config),
-                                  kernel_s, n_inside, probe),
+        "roofline": dict(roofline_line(roofline(n_evals, n_inside, kernel_s),
+                                       traffic(prop.config) if profiled else None,
+                                       kernel_s, n_inside, probe),
+                         executed=executed(prop.config, kernel_s) if profiled else None),
         "single_plain": single,
         "batched": batched,
         "autograd": autograd,
