@@ -509,7 +509,9 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
     float m[PPL];
 #pragma unroll
     for (int i = 0; i < PPL; ++i) m[i] = 0.0f;
-    for (int c = 0; c < n_chunks; ++c) {
+    // chunks in reverse, so that the first (full) chunk is the one left in
+    // LDS for the gradient pass, which then rebuilds only the others
+    for (int c = n_chunks - 1; c >= 0; --c) {
       const int n_act = chunk_size(c);
       build_records<W>(P, props, pstart, gcal, band, n_psf, c * gauss_cap, n_act, l);
       model_pass<PPL>(l.recs, wave, W, n_act, xp, yp, m);
@@ -552,10 +554,10 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
   wave_has_weight = wave_any(wave_has_weight);
 
   unsigned char *wrow = l.rows + wave * kRowBytes;
-  // last chunk first: after the model pass its records are still in LDS
-  for (int c = n_chunks - 1; c >= 0; --c) {
+  // first chunk first: after the model pass its records are still in LDS
+  for (int c = 0; c < n_chunks; ++c) {
     const int n_act = chunk_size(c);
-    if (c != n_chunks - 1 || vjp)
+    if (c != 0 || vjp)
       build_records<W>(P, props, pstart, gcal, band, n_psf, c * gauss_cap, n_act, l);
     if (wave_has_weight)
       grad_pass<PPL>(l.recs, wave, W, n_act, xp, yp, w2, l.cwl, wrow, lane);
