@@ -121,11 +121,12 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask) {
 //   rad2     kRad2 floats              the patch's radius-bin variances
 //   rows     W x kRowBytes             per-wave accumulators: chi2 (double)
 //                                       then MAXSOURCES*NPARAMS gradient sums
-//   index    index_cap + 1 ints        cull flags, then the survivors'
+//   index    index_cap + 1 uint16      cull flags, then the survivors'
 //                                       (source, component) indices in order
-//                                       and their count
+//                                       and their count (all below 2^16)
 // ------------------------------------------------------------------------
 constexpr int kRowBytes = 8 + MAXSOURCES * NPARAMS * 4;
+static_assert(MAXSOURCES * FPX_MAX_PSF_PER_SOURCE < 65536, "LDS index entries are 16-bit");
 
 constexpr int kRad2 = 16;  // >= MAXRADII
 constexpr int kExpoFloats = MAXSOURCES * (kGeoFloats + 4) + kRad2;
@@ -134,7 +135,7 @@ __host__ __device__ constexpr size_t lds_bytes_for(int waves, int ppl, int gauss
                                                    int index_cap) {
   return (size_t)gauss_cap * kRec * sizeof(float) + (size_t)waves * ppl * 64 * sizeof(float) +
          (size_t)kExpoFloats * sizeof(float) + (size_t)waves * kRowBytes +
-         (size_t)(index_cap + 1) * sizeof(int);
+         (size_t)(index_cap + 1) * sizeof(uint16_t);
 }
 
 struct Lds {
@@ -144,7 +145,7 @@ struct Lds {
   float *cwl;
   float *rad2;
   unsigned char *rows;
-  int *index;
+  uint16_t *index;
 };
 
 __device__ __forceinline__ Lds carve(unsigned char *smem, int waves, int ppl, int gauss_cap) {
@@ -155,7 +156,7 @@ __device__ __forceinline__ Lds carve(unsigned char *smem, int waves, int ppl, in
   l.cwl = l.geo + MAXSOURCES * kGeoFloats;
   l.rad2 = l.cwl + MAXSOURCES * 4;
   l.rows = reinterpret_cast<unsigned char *>(l.rad2 + kRad2);
-  l.index = reinterpret_cast<int *>(l.rows + waves * kRowBytes);
+  l.index = reinterpret_cast<uint16_t *>(l.rows + waves * kRowBytes);
   return l;
 }
 
@@ -186,7 +187,7 @@ template <int W>
 __device__ int cull_tile(const FpxPatch *__restrict__ P, int pstart, int n_src, int n_psf,
                          const Box &box, const Lds &l, int wave, int lane) {
   const int n = n_src * n_psf;
-  int *index = l.index;
+  uint16_t *index = l.index;
   __syncthreads();  // previous users of the index are done; geometry is in LDS
   for (int lg = threadIdx.x; lg < n; lg += W * 64) {
     const int s = lg / n_psf;
