@@ -2,18 +2,22 @@
 
 One step = one chi2 + gradient evaluation of the whole workload with every
 input already resident in device memory (patch pixels and metadata, the
-proposal); outputs stay on the device.  Prints one JSON line.
+proposals); outputs stay on the device.  Prints one JSON line.
 
-Workloads:
-  multi    (default) one patch of 8 bands x 2 exposures x 128x128 px, 15
-           sources, 9 radii x 4 PSF Gaussians = 1.416e8 pixel-Gaussian pairs
-  single   1 band x 1 exposure x 64x64 px, 1 source, 9 x 3 Gaussians
+The headline (value, ms_per_step, roofline) is the batched workload: 512
+uneven patches (1-20 sources, 1-8 bands x 1-4 exposures, square stamps of
+side 48-256 px, 3-6 PSF Gaussians, 9 radii) evaluated in one persistent
+launch, whose "vs plain" figure is the same 512 patches evaluated by one
+plain launch each.  The line also carries:
+  plain     one patch of 8 bands x 2 exposures x 128x128 px, 15 sources, 9
+            radii x 4 PSF Gaussians (1.416e8 pixel-Gaussian pairs), one
+            launch per evaluation
+  single_plain   1 band x 1 exposure x 64x64 px, 1 source, 9 x 3 Gaussians
+  parity    the multi-band patch's gradients and chi2 against the reference
+            CPU kernel and the float64 oracle (computed once, untimed)
+  autograd, ensemble, valu_probe, cpu_baseline(_parallel)
 
-The line also carries a "batched" object: --batch-patches uneven patches
-evaluated in one persistent launch, next to the same patches evaluated by one
-plain launch each.
-
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload multi]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 
 import argparse
@@ -40,17 +44,19 @@ BATCH_VALU_FILE = os.path.join(REPO, "profiles", "chi2_grad_kernel_batched_uneve
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="multi", choices=["multi", "single"])
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed batched launches (the headline)")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=200,
+                    help="timed evaluations of the plain multi-band patch")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--config", type=int, default=None, help="kernel config (default: tuned)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (counter calibration)")
     ap.add_argument("--batch-patches", type=int, default=512,
-                    help="uneven patches in the batched measurement (0 skips it)")
-    ap.add_argument("--batch-steps", type=int, default=5)
+                    help="uneven patches of the batched (headline) workload")
+    ap.add_argument("--no-parity", action="store_true", help="skip the parity object")
     ap.add_argument("--no-probe", action="store_true", help="skip the VALU rate probe")
     ap.add_argument("--no-single", action="store_true",
                     help="skip the single-patch plain measurement")
@@ -330,13 +336,24 @@ def ensemble_measurement(args):
     ens.batch.autotune(repeats=1, configs=[2, 6, 7], persistent=(True,))
     gen = torch.Generator(device=ens.device)
     gen.manual_seed(args.seed)
-    ens.eps.fill_(0.005)
-    ens.p.copy_(torch.randn(ens.p.shape, generator=gen, dtype=torch.float64,
-                            device=ens.device) * ens.fmask)
+    # per-chain step sizes (untimed): the starting heuristic, two 8-step
+    # rounds, then two rounds of the recorded trajectory itself
     t1 = time.perf_counter()
+    ens.eps.fill_(0.005)
+    ens.find_step_size(generator=gen)
+    ens.adapt_step_size(2, 8, generator=gen)
+    t2 = time.perf_counter()
     ens.capture(args.leapfrog_steps)
-    capture_s = time.perf_counter() - t1
+    capture_s = time.perf_counter() - t2
+    ens.adapt_step_size(2, args.leapfrog_steps, generator=gen, use_graph=True)
+    adapt_s = time.perf_counter() - t1 - capture_s
+    p0 = torch.randn(ens.p.shape, generator=gen, dtype=torch.float64,
+                     device=ens.device) * ens.fmask
+    ens.p.copy_(p0)
     traj_s = time_region(ens.graph.replay, 1)
+    dH = ((ens.U + 0.5 * (ens.p * ens.p).sum(dim=(1, 2)))
+          - (ens.U_start + 0.5 * (p0 * p0).sum(dim=(1, 2)))).cpu().numpy()
+    accept = np.where(np.isfinite(dH), np.minimum(1.0, np.exp(-np.clip(dH, -50, None))), 0.0)
     n_evals = ens.batch.n_evals() * (args.leapfrog_steps + 1)
     small = small_ensemble_measurement(args)
     return {"workload": f"{ens.n_slots} chains ({args.ensemble_chains} per patch) on "
@@ -346,8 +363,11 @@ def ensemble_measurement(args):
             "trajectories_per_s": ens.n_slots / traj_s,
             "evals_per_s": n_evals / traj_s,
             "kernel_config": "{}{}".format(ens.batch.config, "p" if ens.batch.persistent else ""),
-            "host_build_s": build_s, "warmup_and_capture_s": capture_s,
-            "small_regions": small}
+            "mean_accept_prob": float(accept.mean()),
+            "finite_energy_fraction": float(np.isfinite(dH).mean()),
+            "median_abs_energy_error": float(np.median(np.abs(dH))),
+            "host_build_s": build_s, "step_size_adaptation_s": adapt_s,
+            "warmup_and_capture_s": capture_s, "small_regions": small}
 
 
 def small_ensemble_measurement(args, n_patches=16, chains=4, side=64, n_bands=3, n_sources=4):
@@ -384,6 +404,8 @@ def small_ensemble_measurement(args, n_patches=16, chains=4, side=64, n_bands=3,
     gen = torch.Generator(device=ens.device)
     gen.manual_seed(args.seed)
     ens.eps.fill_(0.005)
+    ens.find_step_size(generator=gen)
+    ens.adapt_step_size(4, 8, generator=gen)
     ens.p.copy_(torch.randn(ens.p.shape, generator=gen, dtype=torch.float64,
                             device=ens.device) * ens.fmask)
     ens.capture(args.leapfrog_steps)
@@ -440,14 +462,35 @@ def time_region(fn, steps):
     return a.elapsed_time(b) * 1e-3 / steps
 
 
-def batched_measurement(args, plain):
-    """Batched persistent evaluation of --batch-patches uneven patches in one
-    launch, against the same patches evaluated one plain launch at a time
-    (device-resident launches, and the full evaluate_proposal API with host
-    proposals in and results out)."""
+def timed_region(fn, steps, warmup):
+    """The timed region of the contract: warmup untimed calls, then exactly
+    ``steps`` back-to-back calls between two HIP events on the launch
+    stream, synchronised on both sides (no per-step markers: they would add
+    their own time to every step).  Returns (seconds, wall seconds)."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    s = torch.cuda.current_stream()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record(s)
+    for _ in range(steps):
+        fn()
+    end.record(s)
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) * 1e-3, time.perf_counter() - t0
+
+
+def batched_measurement(args, plain_prop):
+    """The headline: --batch-patches uneven patches evaluated in one
+    persistent launch (autotuned tile shape), timed over --steps launches;
+    then the tile kernel alone (its average launch time feeds the roofline),
+    and the same patches evaluated by one plain launch each (device-resident
+    inputs, the plain proposer's configuration) and through evaluate_proposal
+    per patch (host proposal in, results out)."""
     import torch
     from forcepho_amd.batch import BatchEvaluator
-    from forcepho_amd.proposal import HIPProposer
     t0 = time.perf_counter()
     patches, proposals = build_uneven(args.batch_patches, args.seed + 1)
     build_s = time.perf_counter() - t0
@@ -456,25 +499,28 @@ def batched_measurement(args, plain):
     tuned = ev.autotune(repeats=2)
     n_evals = ev.n_evals()
     n_inside = int(ev.count_pairs().sum())
-    ev.launch()
-    batch_s = time_region(ev.launch, args.batch_steps)
+    region, wall = timed_region(ev.launch, args.steps, args.warmup)
+    batch_s = region / args.steps
     for (chi2, d) in ev.results():
         assert np.all(np.isfinite(chi2)) and np.all(np.isfinite(d))
+    # the dominant kernel alone: back-to-back tile launches, then one output pass
+    kernel_s = time_region(lambda: ev.launch(outputs=False), max(2, args.steps // 2))
+    ev.finalize()
+    torch.cuda.synchronize()
 
-    # one plain launch per patch, same kernel configuration as the plain bench
-    prop = HIPProposer(config=plain.config, persistent=plain.persistent)
+    # one plain launch per patch, the plain proposer's kernel configuration
     pdevs = [torch.from_numpy(np.ascontiguousarray(p).view(np.uint8).reshape(-1).copy()).cuda()
              for p in proposals]
 
     def sequential():
         for patch, pd in zip(patches, pdevs):
-            prop.launch(pd, patch)
+            plain_prop.launch(pd, patch)
     sequential()
     seq_s = time_region(sequential, 2)
 
     def sequential_api():
         for patch, pr in zip(patches, proposals):
-            prop.evaluate_proposal(pr, patch=patch)
+            plain_prop.evaluate_proposal(pr, patch=patch)
     sequential_api()
     t1 = time.perf_counter()
     sequential_api()
@@ -484,26 +530,136 @@ def batched_measurement(args, plain):
             f"48-256 px stamps, 3-6 PSF Gaussians, 9 radii) in one persistent launch")
     # the committed counter summaries describe the profiled workload (512 patches, seed 0) only
     profiled = args.batch_patches == 512 and args.seed == 0
+    line = roofline(n_evals, n_inside, kernel_s)
+    line["traffic"] = traffic(ev.config, BATCH_TRAFFIC_FILE) if profiled else None
+    if line["traffic"] is not None:
+        line["hbm_gb_per_s"] = line["traffic"] / kernel_s / 1e9
+        line["hbm_frac"] = line["hbm_gb_per_s"] / 8000.0
+    line["executed"] = executed(ev.config, kernel_s, BATCH_VALU_FILE) if profiled else None
     return {
-        "workload": desc,
-        "n_patches": len(patches),
-        "evals_per_step": n_evals,
-        "pixels": int(sum(int(p.npix) for p in patches)),
-        "stamp_side_range": [min(sides), max(sides)],
-        "evals_per_s": n_evals / batch_s,
+        "value": n_evals / batch_s,
+        "region_s": region,
+        "wall_s": wall,
         "ms_per_step": batch_s * 1e3,
-        "roofline": dict(roofline(n_evals, n_inside, batch_s),
-                         traffic=(traffic(ev.config, BATCH_TRAFFIC_FILE) if profiled else None),
-                         executed=(executed(ev.config, batch_s, BATCH_VALU_FILE)
-                                   if profiled else None)),
-        "kernel_config": "{}{}".format(ev.config, "p" if ev.persistent else ""),
-        "config_ms": {f"{c}{'p' if p else ''}": t * 1e3 for (c, p), t in tuned.items()},
-        "sequential_plain_launches": {"evals_per_s": n_evals / seq_s, "ms": seq_s * 1e3},
-        "sequential_evaluate_proposal": {"evals_per_s": n_evals / api_s, "ms": api_s * 1e3},
-        "speedup_vs_sequential_plain": seq_s / batch_s,
-        "speedup_vs_sequential_evaluate_proposal": api_s / batch_s,
-        "host_build_s": build_s,
+        "config": {"workload": desc, "n_patches": len(patches), "evals_per_step": n_evals,
+                   "pixels": int(sum(int(p.npix) for p in patches)),
+                   "stamp_side_range": [min(sides), max(sides)], "path": "batched persistent",
+                   "kernel_config": "{}{}".format(ev.config, "p" if ev.persistent else ""),
+                   "config_ms": {f"{c}{'p' if p else ''}": t * 1e3 for (c, p), t in tuned.items()},
+                   "host_build_s": build_s},
+        "roofline": line,
+        "vs_plain": {
+            "workload": "the same patches, one plain launch each (device-resident inputs)",
+            "sequential_plain_launches": {"evals_per_s": n_evals / seq_s, "ms": seq_s * 1e3},
+            "sequential_evaluate_proposal": {"evals_per_s": n_evals / api_s, "ms": api_s * 1e3},
+            "speedup_vs_sequential_plain": seq_s / batch_s,
+            "speedup_vs_sequential_evaluate_proposal": api_s / batch_s},
     }
+
+
+def plain_measurement(args):
+    """The multi-band patch (8 bands x 2 exposures x 128x128 px, 15
+    sources) evaluated one plain launch at a time: autotuned tile shape, then
+    --plain-steps back-to-back evaluations, then the tile kernel alone."""
+    import torch
+    from forcepho_amd.proposal import HIPProposer
+    patch, scene, desc = build_workload("multi", args.seed)
+    patch.return_residual = False
+    patch.send_to_device()
+    proposal = scene.get_proposal()
+    proposal["q"] *= np.float32(0.97)   # off the truth: gradients are generic
+    n_evals = patch.n_evals()
+    raw = np.ascontiguousarray(proposal).view(np.uint8).reshape(-1)
+    pdev = torch.from_numpy(raw.copy()).cuda()
+    prop = HIPProposer(patch=patch)
+    configs = [args.config] if args.config is not None else None
+    tuned = prop.autotune(pdev, patch, repeats=10, configs=configs)
+    region, wall = timed_region(lambda: prop.launch(pdev, patch), args.plain_steps, 10)
+    kernel_s = time_region(lambda: prop.launch(pdev, patch, outputs=False), args.plain_steps)
+    prop.finalize()
+    chi2, derivs = prop.evaluate_proposal(proposal, patch=patch)
+    assert np.all(np.isfinite(chi2)) and np.all(np.isfinite(derivs))
+    n_inside = prop.count_pairs(pdev, patch)
+    # the committed counter summaries describe the multi-band workload at seed 0 only
+    profiled = args.seed == 0
+    out = {"workload": desc, "evals_per_step": n_evals, "path": "plain",
+           "kernel_config": f"{prop.config}{'p' if prop.persistent else ''}",
+           "config_ms": {f"{c}{'p' if p else ''}": t * 1e3 for (c, p), t in tuned.items()},
+           "evals_per_s": n_evals * args.plain_steps / region,
+           "ms_per_step": region / args.plain_steps * 1e3, "steps": args.plain_steps,
+           "roofline": dict(roofline_line(roofline(n_evals, n_inside, kernel_s),
+                                          traffic(prop.config) if profiled else None,
+                                          kernel_s, n_inside, None),
+                            executed=executed(prop.config, kernel_s) if profiled else None)}
+    return out, prop, patch, scene, proposal
+
+
+def parity_measurement():
+    """Parity of the multi-band workload at the north-star bar, once and
+    untimed: the patch of tests/golden/make_fullsize.py (the benchmark's
+    scene with data rendered on the host, every q scaled by 0.97) evaluated
+    by the HIP kernel, by the reference's CPU kernel (oracle/_ref, the
+    checker) and from the committed float64 oracle values of the same patch.
+    Relative chi2 errors, and gradient errors per entry and relative to the
+    largest |gradient| of the band, as p50 / p99 / max, for GPU vs the
+    reference kernel, GPU vs float64 and the reference kernel vs float64."""
+    import importlib.util
+    import oracle
+    from forcepho_amd.proposal import HIPProposer
+    path = os.path.join(REPO, "tests", "golden", "make_fullsize.py")
+    spec = importlib.util.spec_from_file_location("make_fullsize", path)
+    F = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(F)
+    patch, proposal = F.build()
+    fp64 = np.load(F.OUT)
+    if not np.allclose(F.data_digest(patch), fp64["digest"], rtol=1e-12):
+        return {"error": "patch differs from the committed float64 fixture"}
+    patch.return_residual = False
+    patch.send_to_device()
+    prop = HIPProposer(patch=patch)
+    c_gpu, d_gpu = prop.evaluate_proposal(proposal, patch=patch)
+    mod = oracle.reference_module()
+    if mod is None:
+        return {"error": "the reference CPU kernel is not built (oracle/_ref)"}
+    c_ref, d_ref, _ = oracle.evaluate_reference(patch, proposal, module=mod)
+    d_ref = d_ref.astype(np.float64)
+    c64, g64 = fp64["chi2"], fp64["dchi2"]
+
+    def q3(x):
+        x = np.asarray(x, dtype=np.float64).reshape(-1)
+        return {"p50": float(np.percentile(x, 50)), "p99": float(np.percentile(x, 99)),
+                "max": float(x.max())}
+
+    def entry(g, h):
+        return np.abs(g - h) / np.where(h != 0, np.abs(h), np.inf)
+
+    def band(g, h):
+        m = np.abs(h).max(axis=(1, 2), keepdims=True)
+        return np.abs(g - h) / np.where(m > 0, m, np.inf)
+
+    pairs = {"gpu_vs_reference": (d_gpu, d_ref, c_gpu, c_ref),
+             "gpu_vs_fp64": (d_gpu, g64, c_gpu, c64),
+             "reference_vs_fp64": (d_ref, g64, c_ref, c64)}
+    return {"workload": "multi-band patch (8 bands x 2 exposures x 128x128 px, 15 sources), "
+                        "host-rendered data, proposal q x 0.97",
+            "bar": "chi2 <= 1e-6 relative, gradients <= 1e-4 of the band's largest |dchi2/dp|, "
+                   "against the reference CPU kernel",
+            "chi2_rel_max": {k: float(np.max(np.abs(c - h) / np.abs(h)))
+                             for k, (_, _, c, h) in pairs.items()},
+            "grad_rel_per_entry": {k: q3(entry(g, h)) for k, (g, h, _, _) in pairs.items()},
+            "grad_rel_of_band_max": {k: q3(band(g, h)) for k, (g, h, _, _) in pairs.items()}}
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(patch, proposal, n_evals, budget_s=12.0):
@@ -566,30 +722,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import torch
-    from forcepho_amd._lib import lib
-    from forcepho_amd.proposal import HIPProposer
-
     torch.cuda.set_device(0)
-    patch, scene, desc = build_workload(args.workload, args.seed)
-    patch.return_residual = False
-    patch.send_to_device()
-    proposal = scene.get_proposal()
-    proposal["q"] *= np.float32(0.97)   # off the truth: gradients are generic
-    n_evals = patch.n_evals()
-    # the committed counter summaries describe the multi-band workload at seed 0 only
-    profiled = args.workload == "multi" and args.seed == 0
-
-    # pick the kernel configuration (tile shape, persistent or not) by measurement
-    raw = np.ascontiguousarray(proposal).view(np.uint8).reshape(-1)
-    pdev = torch.from_numpy(raw.copy()).cuda()
-    prop = HIPProposer(patch=patch)
-    if args.config is not None:
-        tuned = prop.autotune(pdev, patch, repeats=10, configs=[args.config])
-    else:
-        tuned = prop.autotune(pdev, patch, repeats=10)
-    timings = {f"{c}{'p' if p else ''}": t for (c, p), t in tuned.items()}
-    best = f"{prop.config}{'p' if prop.persistent else ''}"
-
     if args.copy_mb:
         # a plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE
         # counters calibrate the chi2 kernel's
@@ -598,71 +731,54 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    for _ in range(args.warmup):
-        prop.launch(pdev, patch)
-    torch.cuda.synchronize()
-    # the timed region: K back-to-back evaluations between two HIP events on
-    # the launch stream, synchronised on both sides (no per-step markers in
-    # the stream: they would add their own time to every step)
-    s = torch.cuda.current_stream()
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    start.record(s)
-    for i in range(args.steps):
-        prop.launch(pdev, patch)
-    end.record(s)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    region = start.elapsed_time(end) * 1e-3
-    # the dominant kernel alone: back-to-back tile launches (fpx_evaluate_tiles)
-    # bracketed by HIP events on the launch stream, then one output pass
-    kernel_s = time_region(lambda: prop.launch(pdev, patch, outputs=False), args.steps)
-    prop.finalize()
 
-    # outputs of the timed evaluations: one more evaluation through the full API
-    chi2, derivs = prop.evaluate_proposal(proposal, patch=patch)
-    assert np.all(np.isfinite(chi2)) and np.all(np.isfinite(derivs))
-
-    value = n_evals * args.steps / region
-    n_inside = prop.count_pairs(pdev, patch)
+    plain, plain_prop, patch, scene, proposal = plain_measurement(args)
+    print(f"plain multi-band: {plain['ms_per_step']:.4f} ms per evaluation", file=sys.stderr,
+          flush=True)
+    head = batched_measurement(args, plain_prop)
+    print(f"batched: {head['ms_per_step']:.3f} ms per launch", file=sys.stderr, flush=True)
     probe = None if args.no_probe else valu_probe()
+    if probe is not None:
+        for r in (head["roofline"], plain["roofline"]):
+            n_in = r["pairs_inside_cutoff"]
+            r["exp_per_s"] = 2.0 * n_in / (r["kernel_ms"] * 1e-3)
+            r["exp_frac_of_measured_peak"] = r["exp_per_s"] / probe["exp_per_s"]
     single = None if args.no_single else single_plain_measurement(args)
-    batched = batched_measurement(args, prop) if args.batch_patches > 0 else None
-    if batched is not None and single is not None:
-        batched["speedup_vs_single_patch_plain_launches"] = (
-            batched["evals_per_s"] / single["launch_evals_per_s"])
+    if single is not None:
+        head["vs_plain"]["speedup_vs_single_patch_plain_launches"] = (
+            head["value"] / single["launch_evals_per_s"])
+    parity = None if args.no_parity else parity_measurement()
     ensemble = ensemble_measurement(args) if args.ensemble_patches > 0 else None
     autograd = None if args.no_autograd else autograd_measurement(args, patch, scene)
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": region / args.steps * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": "synthetic (seeded scene, model + Gaussian noise at S/N~50)",
-        "config": {"workload": desc, "evals_per_step": n_evals, "path": "plain",
-                   "kernel_config": best, "config_ms": {str(k): v * 1e3 for k, v in timings.items()},
-                   "wall_s": wall},
-        "roofline": dict(roofline_line(roofline(n_evals, n_inside, kernel_s),
-                                       traffic(prop.config) if profiled else None,
-                                       kernel_s, n_inside, probe),
-                         executed=executed(prop.config, kernel_s) if profiled else None),
+        "data": "synthetic (seeded scenes, model + Gaussian noise at S/N~50)",
+        "config": dict(head["config"], wall_s=head["wall_s"]),
+        "roofline": head["roofline"],
+        "vs_plain": head["vs_plain"],
+        "plain": plain,
         "single_plain": single,
-        "batched": batched,
+        "parity": parity,
         "autograd": autograd,
         "ensemble": ensemble,
         "valu_probe": probe,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(patch, proposal, n_evals)
-        line["cpu_baseline_parallel"] = cpu_baseline_parallel(patch, proposal, n_evals)
+        model = cpu_model()
+        n_evals = patch.n_evals()
+        line["cpu_baseline"] = dict(cpu_baseline(patch, proposal, n_evals), cpu_model=model)
+        line["cpu_baseline_parallel"] = dict(cpu_baseline_parallel(patch, proposal, n_evals),
+                                             cpu_model=model)
     print(json.dumps(line), flush=True)
 
 

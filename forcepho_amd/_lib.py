@@ -13,6 +13,7 @@ import numpy as np
 from .build import LIB_PATH
 
 __all__ = ["lib", "check", "FpxError", "TILE_DTYPE", "SLOT_DTYPE", "SLOT_MODEL_ONLY",
+           "STATUS_LIMIT", "STATUS_RADIUS_BIN",
            "STRUCT_PATCH", "STRUCT_SOURCE", "STRUCT_RESPONSE", "STRUCT_PSFGAUSS",
            "STRUCT_TILE", "STRUCT_SLOT", "STRUCT_AMPMODEL", "EXPORTED_SYMBOLS",
            "AmplitudeModel", "AMP_LOGNORMAL", "AMP_SPLINE", "MAX_SPLINE_DEGREE"]
@@ -25,11 +26,13 @@ TILE_DTYPE = np.dtype([("exposure", np.int32), ("band", np.int32),
                        ("pix_start", np.int32), ("pix_count", np.int32)])
 SLOT_DTYPE = np.dtype([("patch", np.uintp), ("proposal", np.uintp), ("tiles", np.uintp),
                        ("pixel_weight", np.uintp), ("pchi2", np.uintp),
-                       ("pdchi2_dp", np.uintp), ("accum", np.uintp), ("ticket", np.uintp),
+                       ("pdchi2_dp", np.uintp), ("accum", np.uintp), ("status", np.uintp),
                        ("n_bands", np.int32), ("n_tiles", np.int32),
                        ("flags", np.int32), ("reserved", np.int32)], align=True)
 
 SLOT_MODEL_ONLY = 1    # FpxSlot.flags: model pass only (FPX_SLOT_MODEL_ONLY)
+STATUS_LIMIT = 1       # FpxSlot.status bit: a tile broke a size limit (FPX_STATUS_LIMIT)
+STATUS_RADIUS_BIN = 2  # FpxSlot.status bit: radius bin outside [0, n_radii)
 
 AMP_LOGNORMAL, AMP_SPLINE = 0, 1   # FpxAmplitudeModel.kind
 MAX_SPLINE_DEGREE = 5              # FPX_MAX_SPLINE_DEGREE
@@ -63,7 +66,13 @@ _SIGNATURES = {
     "fpx_device_info": (_c_int, [_p, _p, _p]),
     "fpx_evaluate_proposal": (_c_int, [_p, _c_int, _c_int, _c_int, _p]),
     "fpx_evaluate_sync": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _p, _p, _p, _c_size,
-                                   _p, _p, _c_size, _p]),
+                                   _p, _p, _c_size, _p, _p]),
+    "fpx_bind": (_c_int, [_p, _c_int, _c_int, _c_int, _p]),
+    "fpx_evaluate": (_c_int, [_p, _p, _p, _p, _p]),
+    "fpx_evaluate_host": (_c_int, [_p, _p, _c_int, _p, _p, _p]),
+    "fpx_evaluator_status": (_c_int, [_p, _p]),
+    "fpx_evaluator_info": (_c_int, [_p, _p, _p, _p]),
+    "fpx_release": (_c_int, [_p]),
     "fpx_evaluate_batch": (_c_int, [_p, _c_int, _p, _c_int, _c_int, _c_int, _c_int, _p, _p]),
     "fpx_evaluate_tiles": (_c_int, [_p, _c_int, _p, _c_int, _c_int, _c_int, _c_int, _p, _p]),
     "fpx_finalize": (_c_int, [_p, _c_int, _p]),

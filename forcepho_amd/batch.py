@@ -16,7 +16,7 @@ configuration (tile shape) on the batch and keeps the fastest.
 import numpy as np
 
 from .kernel_limits import MAXBANDS, MAXSOURCES, NPARAMS
-from ._lib import lib, check, SLOT_DTYPE
+from ._lib import lib, check, FpxError, SLOT_DTYPE
 from .proposal import source_struct_dtype
 
 __all__ = ["BatchEvaluator"]
@@ -59,15 +59,16 @@ class BatchEvaluator:
         self.n_acc = L.fpx_accum_doubles(MAXBANDS)
         n = self.n_slots
         self.accum = torch.zeros(n * self.n_acc, dtype=torch.float64, device=dev)
-        self.tickets = torch.zeros(n, dtype=torch.int32, device=dev)
+        # per-slot status words (FpxSlot.status): nonzero when the kernel left
+        # a tile of the slot out (checked by results())
+        self.status = torch.zeros(n, dtype=torch.int32, device=dev)
         self.queue = torch.zeros(2, dtype=torch.int32, device=dev)
         self.chi2 = torch.zeros(n * MAXBANDS, dtype=torch.float64, device=dev)
         self.resp = torch.zeros(n * MAXBANDS * _RESP, dtype=torch.float32, device=dev)
         self.proposals = torch.zeros(n * MAXSOURCES * _SRC_BYTES, dtype=torch.uint8, device=dev)
-        self.max_gauss = max(p.max_gauss() for p in self.patches)
         for p in self.patches:
-            if int(p.n_sources) > MAXSOURCES or int(p.n_bands) > MAXBANDS:
-                raise ValueError("patch exceeds MAXSOURCES/MAXBANDS")
+            p.check_limits()
+        self.max_gauss = max(p.max_gauss() for p in self.patches)
         self.set_config(DEFAULT_CONFIG if config is None else config)
 
     # ------------------------------------------------------------------
@@ -97,7 +98,7 @@ class BatchEvaluator:
             slots[k]["pchi2"] = base(self.chi2) + s * MAXBANDS * 8
             slots[k]["pdchi2_dp"] = base(self.resp) + s * MAXBANDS * _RESP * 4
             slots[k]["accum"] = base(self.accum) + s * self.n_acc * 8
-            slots[k]["ticket"] = base(self.tickets) + s * 4
+            slots[k]["status"] = base(self.status) + s * 4
             slots[k]["n_bands"] = int(patch.n_bands)
             slots[k]["n_tiles"] = nt
             n_tiles[k] = nt
@@ -127,21 +128,34 @@ class BatchEvaluator:
             host[s, :raw.size] = raw.reshape(-1)
         self.proposals.copy_(torch.from_numpy(host.reshape(-1)), non_blocking=False)
 
-    def launch(self, stream=None):
-        """Queue the batched evaluation (asynchronous)."""
+    def launch(self, stream=None, outputs=True):
+        """Queue the batched evaluation (asynchronous).  ``outputs=False``
+        launches the tile kernel alone (fpx_evaluate_tiles): its sums stay in
+        the slots' accumulators until ``finalize`` or a launch with
+        outputs."""
         torch = _torch()
         s = stream or torch.cuda.current_stream(self.device)
-        check(lib().fpx_evaluate_batch(self.slots_dev.data_ptr(), self.n_slots,
-                                       self.offsets_dev.data_ptr(), self.n_items, self.max_gauss,
-                                       self.config, int(self.persistent),
-                                       self.queue.data_ptr(), s.cuda_stream),
-              "fpx_evaluate_batch")
+        fn = lib().fpx_evaluate_batch if outputs else lib().fpx_evaluate_tiles
+        check(fn(self.slots_dev.data_ptr(), self.n_slots, self.offsets_dev.data_ptr(),
+                 self.n_items, self.max_gauss, self.config, int(self.persistent),
+                 self.queue.data_ptr(), s.cuda_stream),
+              "fpx_evaluate_batch" if outputs else "fpx_evaluate_tiles")
+
+    def finalize(self, stream=None):
+        """Write every slot's outputs from its accumulator and zero it
+        (fpx_finalize)."""
+        torch = _torch()
+        s = stream or torch.cuda.current_stream(self.device)
+        check(lib().fpx_finalize(self.slots_dev.data_ptr(), self.n_slots, s.cuda_stream),
+              "fpx_finalize")
 
     def results(self):
         """Synchronise and return per-slot (chi2 [n_bands], derivs
-        [n_bands, n_sources, 7]) in slot order."""
+        [n_bands, n_sources, 7]) in slot order.  Raises FpxError (after
+        clearing the words) if the kernel left a tile of some slot out."""
         torch = _torch()
         torch.cuda.current_stream(self.device).synchronize()
+        self.check_status()
         chi2 = self.chi2.view(self.n_slots, MAXBANDS).cpu().numpy()
         resp = self.resp.view(self.n_slots, MAXBANDS, MAXSOURCES, NPARAMS).cpu().numpy()
         out = []
@@ -149,6 +163,17 @@ class BatchEvaluator:
             nb, ns = int(self.patches[p].n_bands), int(self.patches[p].n_sources)
             out.append((chi2[s, :nb].copy(), resp[s, :nb, :ns].astype(np.float64)))
         return out
+
+    def check_status(self):
+        """Raise FpxError, naming the slots, if any slot's status word is set
+        (the kernel left a tile of that slot out and wrote NaN chi2); the
+        words are cleared first."""
+        st = self.status.cpu().numpy()
+        bad = np.flatnonzero(st)
+        if len(bad):
+            self.status.zero_()
+            raise FpxError(f"the kernel left tiles out in slots {bad[:10].tolist()} "
+                           f"(status bits {sorted(set(st[bad].tolist()))})")
 
     def count_pairs(self):
         """Per slot (slot order), the (pixel, ImageGaussian) pairs inside the

@@ -1,0 +1,283 @@
+// fpx_evaluator.hip -- bound patches (FpxEvaluator): the reference's one-call
+// EvaluateProposal(patch, proposal, pchi2, pdchi2_dp) launch
+// (forcepho/src/compute_gaussians_kernel.cu:212-214) on a patch whose tile
+// table, FP64 accumulator, work queue and status word the library owns.
+//
+// fpx_bind reads the packed patch's struct and index arrays back once,
+// checks every limit the kernel relies on and plans the tile table; an
+// evaluation then passes its slot to the kernels by value, so it touches no
+// host memory, allocates nothing and can be recorded into a graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "forcepho_amd.h"
+#include "fpx_internal.h"
+
+// One device allocation per evaluator, each part 256-byte aligned:
+//   io     status word (uint32, 8 bytes with padding), pchi2 double[MAXBANDS],
+//          FpxResponse[MAXBANDS] (fpx_evaluate_host's outputs: one copy
+//          brings the status and the outputs back)
+//   queue  uint32[2] work-queue counter of persistent launches
+//   prop   FpxSource[MAXSOURCES] (fpx_evaluate_host's proposal)
+//   accum  double[fpx_accum_doubles(n_bands)]
+//   tiles  FpxTile[n_tiles]
+// The pinned host block mirrors io and prop.
+struct FpxEvaluator {
+  int device = 0;
+  const FpxPatch *patch = nullptr;
+  int32_t n_bands = 0, n_sources = 0, max_gauss = 0, cfg = 0, persistent = 0, n_tiles = 0;
+  unsigned char *block = nullptr;
+  unsigned char *host = nullptr;
+  uint32_t *status = nullptr;
+  double *chi2 = nullptr;
+  FpxResponse *resp = nullptr;
+  uint32_t *queue = nullptr;
+  FpxSource *prop = nullptr;
+  double *accum = nullptr;
+  FpxTile *tiles = nullptr;
+};
+
+namespace {
+
+constexpr size_t kIoChi2 = 8;
+constexpr size_t kIoResp = kIoChi2 + sizeof(double) * MAXBANDS;
+constexpr size_t kIoBytes = kIoResp + sizeof(FpxResponse) * MAXBANDS;
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+int arg_err(const std::string &msg) { return fpx::report_error(FPX_E_ARG, msg.c_str()); }
+
+#define EV_HIP(call)                                              \
+  do {                                                            \
+    hipError_t _e = (call);                                       \
+    if (_e != hipSuccess) return fpx::report_hip_error(_e, #call); \
+  } while (0)
+
+template <typename T>
+int read_back(std::vector<T> &dst, const T *src, size_t n) {
+  dst.assign(n, T());
+  if (n == 0) return FPX_OK;
+  if (!src) return arg_err("a patch array the kernel reads is NULL");
+  EV_HIP(hipMemcpy(dst.data(), src, n * sizeof(T), hipMemcpyDeviceToHost));
+  return FPX_OK;
+}
+
+int check_device(const FpxEvaluator *ev) {
+  int dev = -1;
+  EV_HIP(hipGetDevice(&dev));
+  if (dev != ev->device) return arg_err("the evaluator was bound on another device");
+  return FPX_OK;
+}
+
+void release(FpxEvaluator *ev) {
+  if (!ev) return;
+  if (ev->block) (void)hipFree(ev->block);
+  if (ev->host) (void)hipHostFree(ev->host);
+  delete ev;
+}
+
+FpxSlot slot_of(const FpxEvaluator *ev, const FpxSource *prop, double *pchi2, FpxResponse *resp) {
+  FpxSlot s;
+  std::memset(&s, 0, sizeof(s));
+  s.patch = ev->patch;
+  s.proposal = prop;
+  s.tiles = ev->tiles;
+  s.pixel_weight = nullptr;
+  s.pchi2 = pchi2;
+  s.pdchi2_dp = reinterpret_cast<float *>(resp);
+  s.accum = ev->accum;
+  s.status = ev->status;
+  s.n_bands = ev->n_bands;
+  s.n_tiles = ev->n_tiles;
+  return s;
+}
+
+int bind(const FpxPatch *patch_dev, int32_t n_bands, int32_t cfg, int32_t persistent,
+         FpxEvaluator *ev) {
+  if (!patch_dev) return arg_err("NULL patch");
+  if (n_bands < 1 || n_bands > MAXBANDS) return arg_err("n_bands out of range [1, MAXBANDS]");
+  if (cfg < 0) cfg = fpx::kDefaultConfig;
+  const int ppt = fpx::config_pixels(cfg);
+  if (ppt <= 0) return arg_err("invalid kernel configuration");
+  EV_HIP(hipGetDevice(&ev->device));
+  FpxPatch P;
+  EV_HIP(hipMemcpy(&P, patch_dev, sizeof(P), hipMemcpyDeviceToHost));
+  if (P.n_sources < 1 || P.n_sources > MAXSOURCES)
+    return arg_err("n_sources out of range [1, MAXSOURCES]");
+  if (P.n_radii < 1 || P.n_radii > MAXRADII) return arg_err("n_radii out of range [1, MAXRADII]");
+  if (!P.data || !P.ierr || !P.xpix || !P.ypix || !P.rad2 || !P.D || !P.crpix || !P.crval ||
+      !P.CW || !P.G)
+    return arg_err("a patch array the kernel reads is NULL");
+  std::vector<int16_t> band_start, band_N;
+  std::vector<int32_t> n_psf;
+  int rc = read_back(band_start, P.band_start, n_bands);
+  if (!rc) rc = read_back(band_N, P.band_N, n_bands);
+  if (!rc) rc = read_back(n_psf, P.n_psf_per_source, n_bands);
+  if (rc) return rc;
+  int n_exp = 0, max_psf = 0;
+  for (int b = 0; b < n_bands; ++b) {
+    if (band_start[b] < 0 || band_N[b] < 0) return arg_err("negative band_start or band_N");
+    if (n_psf[b] < 1 || n_psf[b] > FPX_MAX_PSF_PER_SOURCE)
+      return arg_err("n_psf_per_source out of range [1, FPX_MAX_PSF_PER_SOURCE]");
+    n_exp = std::max(n_exp, band_start[b] + band_N[b]);
+    max_psf = std::max(max_psf, n_psf[b]);
+  }
+  std::vector<int32_t> exp_start, exp_N, psf_start;
+  rc = read_back(exp_start, P.exposure_start, n_exp);
+  if (!rc) rc = read_back(exp_N, P.exposure_N, n_exp);
+  if (!rc) rc = read_back(psf_start, P.psfgauss_start, n_exp);
+  if (rc) return rc;
+  // the PSF entries the kernel reads: psfgauss[psfgauss_start[e] + p],
+  // p < n_psf_per_source of e's band (every source uses that block)
+  long psf_extent = 0;
+  for (int b = 0; b < n_bands; ++b)
+    for (int k = 0; k < band_N[b]; ++k) {
+      const int e = band_start[b] + k;
+      if (exp_start[e] < 0 || exp_N[e] < 0) return arg_err("negative exposure start or size");
+      if (psf_start[e] < 0) return arg_err("negative psfgauss_start");
+      psf_extent = std::max<long>(psf_extent, (long)psf_start[e] + n_psf[b]);
+    }
+  std::vector<FpxPSFSourceGaussian> psf;
+  rc = read_back(psf, P.psfgauss, (size_t)psf_extent);
+  if (rc) return rc;
+  for (int b = 0; b < n_bands; ++b)
+    for (int k = 0; k < band_N[b]; ++k) {
+      const int e = band_start[b] + k;
+      for (int p = 0; p < n_psf[b]; ++p) {
+        const int bin = psf[psf_start[e] + p].sersic_radius_bin;
+        if (bin < 0 || bin >= P.n_radii)
+          return arg_err("a PSF entry's sersic_radius_bin is outside [0, n_radii)");
+      }
+    }
+  // tile table
+  const int n_tiles = fpx_plan_tiles(exp_start.data(), exp_N.data(), band_start.data(),
+                                     band_N.data(), n_bands, ppt, nullptr, 0);
+  if (n_tiles < 0) return n_tiles;
+  std::vector<FpxTile> tiles((size_t)std::max(n_tiles, 1));
+  if (n_tiles > 0) {
+    rc = fpx_plan_tiles(exp_start.data(), exp_N.data(), band_start.data(), band_N.data(), n_bands,
+                        ppt, tiles.data(), n_tiles);
+    if (rc < 0) return rc;
+  }
+  ev->patch = patch_dev;
+  ev->n_bands = n_bands;
+  ev->n_sources = P.n_sources;
+  ev->max_gauss = P.n_sources * max_psf;
+  ev->cfg = cfg;
+  ev->persistent = persistent ? 1 : 0;
+  ev->n_tiles = n_tiles;
+  // device block
+  const size_t o_queue = align256(kIoBytes);
+  const size_t o_prop = o_queue + 256;
+  const size_t o_accum = o_prop + align256(sizeof(FpxSource) * MAXSOURCES);
+  const size_t o_tiles = o_accum + align256(sizeof(double) * fpx_accum_doubles(n_bands));
+  const size_t bytes = o_tiles + align256(sizeof(FpxTile) * tiles.size());
+  EV_HIP(hipMalloc((void **)&ev->block, bytes));
+  EV_HIP(hipMemset(ev->block, 0, o_tiles));
+  EV_HIP(hipMemcpy(ev->block + o_tiles, tiles.data(), sizeof(FpxTile) * tiles.size(),
+                   hipMemcpyHostToDevice));
+  ev->status = reinterpret_cast<uint32_t *>(ev->block);
+  ev->chi2 = reinterpret_cast<double *>(ev->block + kIoChi2);
+  ev->resp = reinterpret_cast<FpxResponse *>(ev->block + kIoResp);
+  ev->queue = reinterpret_cast<uint32_t *>(ev->block + o_queue);
+  ev->prop = reinterpret_cast<FpxSource *>(ev->block + o_prop);
+  ev->accum = reinterpret_cast<double *>(ev->block + o_accum);
+  ev->tiles = reinterpret_cast<FpxTile *>(ev->block + o_tiles);
+  // pinned host mirror of io, then the proposal staging
+  EV_HIP(hipHostMalloc((void **)&ev->host, align256(kIoBytes) + sizeof(FpxSource) * MAXSOURCES,
+                       hipHostMallocDefault));
+  return FPX_OK;
+}
+
+int read_status(FpxEvaluator *ev, hipStream_t s, uint32_t status) {
+  if (status == 0) return FPX_OK;
+  EV_HIP(hipMemsetAsync(ev->status, 0, sizeof(uint32_t), s));
+  EV_HIP(hipStreamSynchronize(s));
+  return arg_err(fpx::status_text(status));
+}
+
+}  // namespace
+
+extern "C" {
+
+int fpx_bind(const FpxPatch *patch_dev, int32_t n_bands, int32_t cfg, int32_t persistent,
+             FpxEvaluator **out) {
+  if (!out) return arg_err("NULL output handle");
+  *out = nullptr;
+  FpxEvaluator *ev = new FpxEvaluator();
+  const int rc = bind(patch_dev, n_bands, cfg, persistent, ev);
+  if (rc) {
+    release(ev);
+    return rc;
+  }
+  *out = ev;
+  return FPX_OK;
+}
+
+int fpx_evaluate(FpxEvaluator *ev, const FpxSource *proposal_dev, double *pchi2_dev,
+                 FpxResponse *pdchi2_dp_dev, void *stream) {
+  if (!ev || !proposal_dev || !pchi2_dev || !pdchi2_dp_dev)
+    return arg_err("NULL evaluator, proposal or output");
+  const int rc = check_device(ev);
+  if (rc) return rc;
+  return fpx::launch_one(slot_of(ev, proposal_dev, pchi2_dev, pdchi2_dp_dev), ev->n_tiles,
+                         ev->max_gauss, ev->cfg, ev->persistent, ev->queue, stream);
+}
+
+int fpx_evaluate_host(FpxEvaluator *ev, const FpxSource *proposal_host, int32_t n_sources,
+                      double *chi2_host, FpxResponse *resp_host, void *stream) {
+  if (!ev || !proposal_host || !chi2_host) return arg_err("NULL evaluator, proposal or output");
+  if (n_sources != ev->n_sources)
+    return arg_err("proposal length differs from the bound patch's n_sources");
+  int rc = check_device(ev);
+  if (rc) return rc;
+  const hipStream_t s = (hipStream_t)stream;
+  unsigned char *stage = ev->host + align256(kIoBytes);
+  const size_t pbytes = sizeof(FpxSource) * (size_t)n_sources;
+  std::memcpy(stage, proposal_host, pbytes);
+  EV_HIP(hipMemcpyAsync(ev->prop, stage, pbytes, hipMemcpyHostToDevice, s));
+  rc = fpx::launch_one(slot_of(ev, ev->prop, ev->chi2, ev->resp), ev->n_tiles, ev->max_gauss,
+                       ev->cfg, ev->persistent, ev->queue, stream);
+  if (rc) return rc;
+  const size_t out_bytes = kIoResp + (resp_host ? sizeof(FpxResponse) * ev->n_bands : 0);
+  EV_HIP(hipMemcpyAsync(ev->host, ev->block, out_bytes, hipMemcpyDeviceToHost, s));
+  EV_HIP(hipStreamSynchronize(s));
+  uint32_t status = 0;
+  std::memcpy(&status, ev->host, sizeof(status));
+  rc = read_status(ev, s, status);
+  if (rc) return rc;
+  std::memcpy(chi2_host, ev->host + kIoChi2, sizeof(double) * ev->n_bands);
+  if (resp_host) std::memcpy(resp_host, ev->host + kIoResp, sizeof(FpxResponse) * ev->n_bands);
+  return FPX_OK;
+}
+
+int fpx_evaluator_status(FpxEvaluator *ev, void *stream) {
+  if (!ev) return arg_err("NULL evaluator");
+  int rc = check_device(ev);
+  if (rc) return rc;
+  const hipStream_t s = (hipStream_t)stream;
+  EV_HIP(hipStreamSynchronize(s));
+  uint32_t status = 0;
+  EV_HIP(hipMemcpy(&status, ev->status, sizeof(status), hipMemcpyDeviceToHost));
+  return read_status(ev, s, status);
+}
+
+int fpx_evaluator_info(const FpxEvaluator *ev, int32_t *n_tiles, int32_t *cfg,
+                       int32_t *max_gauss) {
+  if (!ev) return arg_err("NULL evaluator");
+  if (n_tiles) *n_tiles = ev->n_tiles;
+  if (cfg) *cfg = ev->cfg;
+  if (max_gauss) *max_gauss = ev->max_gauss;
+  return FPX_OK;
+}
+
+int fpx_release(FpxEvaluator *ev) {
+  release(ev);
+  return FPX_OK;
+}
+
+}  // extern "C"

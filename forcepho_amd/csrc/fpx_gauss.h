@@ -162,7 +162,9 @@ __device__ __forceinline__ void inverse_sigma(float Sxx, float Sxy, float Syy, f
   fyy = Sxx * idet;
 }
 
-// Centre and F of ImageGaussian (s, p) of exposure e.
+// Centre and F of ImageGaussian (s, p) of exposure e.  The radius bin is
+// clamped into the patch's table (the evaluation kernel instead refuses a
+// bin outside it, through the slot's status word).
 __device__ __forceinline__ void image_gaussian_shape(const FpxPatch *__restrict__ P,
                                                      const FpxSource *__restrict__ props,
                                                      int e, int s, int p, float &xc, float &yc,
@@ -170,8 +172,10 @@ __device__ __forceinline__ void image_gaussian_shape(const FpxPatch *__restrict_
   SourceGeometry g;
   source_geometry<false>(P, props + s, e, s, g);
   const FpxPSFSourceGaussian psf = P->psfgauss[P->psfgauss_start[e] + p];
+  const int n_radii = min(max(P->n_radii, 1), MAXRADII);
+  const int bin = min(max(psf.sersic_radius_bin, 0), n_radii - 1);
   float Sxx, Sxy, Syy;
-  convolved_sigma(g.T, P->rad2[psf.sersic_radius_bin], psf, Sxx, Sxy, Syy);
+  convolved_sigma(g.T, P->rad2[bin], psf, Sxx, Sxy, Syy);
   inverse_sigma(Sxx, Sxy, Syy, fxx, fxy, fyy);
   xc = g.xs + psf.xcen;
   yc = g.ys + psf.ycen;

@@ -49,7 +49,7 @@
 #include "fpx_gauss.h"
 #include "fpx_internal.h"
 
-static_assert(sizeof(FpxPatch) == 152, "Patch layout (patch.cc:131-208)");
+static_assert(sizeof(FpxPatch) == 152, "Patch layout (patch.cc:23-100)");
 static_assert(offsetof(FpxPatch, n_sources) == 72, "Patch.n_sources offset");
 static_assert(offsetof(FpxPatch, rad2) == 80, "Patch.rad2 offset");
 static_assert(offsetof(FpxPatch, psfgauss_start) == 144, "Patch.psfgauss_start offset");
@@ -59,7 +59,7 @@ static_assert(offsetof(FpxSource, mixture_amplitudes) == 144, "Source.mixture_am
 static_assert(offsetof(FpxSource, damplitude_drh) == 184, "Source.damplitude_drh");
 static_assert(offsetof(FpxSource, damplitude_dnsersic) == 224, "Source.damplitude_dnsersic");
 static_assert(sizeof(FpxResponse) == 840, "Response layout (proposal.cu:33-39)");
-static_assert(sizeof(FpxPSFSourceGaussian) == 28, "PSFSourceGaussian layout");
+static_assert(sizeof(FpxPSFSourceGaussian) == 28, "PSFSourceGaussian layout (patch.cc:103-119)");
 static_assert(sizeof(FpxTile) == 16, "tile layout");
 static_assert(sizeof(FpxSlot) == 80, "slot layout");
 
@@ -173,27 +173,33 @@ struct Box {
   float x0, x1, y0, y1;
 };
 
+// Radius-bin variance of a PSF entry from the LDS copy.  Only entries whose
+// bin the cull has checked against [0, n_radii) get here.
+__device__ __forceinline__ float bin_covar(const Lds &l, const FpxPSFSourceGaussian &psf) {
+  return l.rad2[psf.sersic_radius_bin];
+}
+
 // Cull the n_src x n_psf Gaussians of the exposure against the tile's box:
 // phase 1 flags every Gaussian (one lane each), phase 2 compacts the flags
 // into the list of survivors in (source, component) order with one
-// wavefront (ballot + prefix count).  Returns the number of survivors.
-// Radius-bin variance of a PSF entry from the LDS copy (bins outside the
-// table, which valid input never has, read its last entry).
-__device__ __forceinline__ float bin_covar(const Lds &l, const FpxPSFSourceGaussian &psf) {
-  return l.rad2[min(max(psf.sersic_radius_bin, 0), MAXRADII - 1)];
-}
-
+// wavefront (ballot + prefix count).  Returns the number of survivors.  A
+// PSF entry whose radius bin lies outside [0, n_radii) never survives and
+// sets FPX_STATUS_RADIUS_BIN in the slot's status word.
 template <int W>
 __device__ int cull_tile(const FpxPatch *__restrict__ P, int pstart, int n_src, int n_psf,
-                         const Box &box, const Lds &l, int wave, int lane) {
+                         int n_radii, const Box &box, const Lds &l, int wave, int lane,
+                         uint32_t *status) {
   const int n = n_src * n_psf;
   uint16_t *index = l.index;
   __syncthreads();  // previous users of the index are done; geometry is in LDS
   for (int lg = threadIdx.x; lg < n; lg += W * 64) {
     const int s = lg / n_psf;
     const FpxPSFSourceGaussian psf = P->psfgauss[pstart + lg - s * n_psf];
-    index[lg] = gauss_reaches(load_geometry(l.geo + s * kGeoFloats), psf, bin_covar(l, psf),
-                              box.x0, box.x1, box.y0, box.y1) ? 1 : 0;
+    const bool bin_ok = psf.sersic_radius_bin >= 0 && psf.sersic_radius_bin < n_radii;
+    if (!bin_ok && status != nullptr) atomicOr(status, (uint32_t)FPX_STATUS_RADIUS_BIN);
+    index[lg] = bin_ok && gauss_reaches(load_geometry(l.geo + s * kGeoFloats), psf,
+                                        bin_covar(l, psf), box.x0, box.x1, box.y0, box.y1)
+                    ? 1 : 0;
   }
   __syncthreads();
   if (wave == 0) {
@@ -446,7 +452,7 @@ __device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int wa
 template <int W, int PPL>
 __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPatch *P,
                           const FpxSource *props, int e, int band, int n_src, int n_psf,
-                          int gauss_cap, unsigned char *smem) {
+                          int n_radii, int gauss_cap, unsigned char *smem) {
   // The tile's Gaussians that reach it are processed in chunks of gauss_cap
   // records (one chunk when they all fit).
   constexpr int kPix = 64 * PPL;
@@ -468,7 +474,6 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
 #pragma unroll
     for (int c = 0; c < 4; ++c) l.cwl[4 * j + c] = P->CW[4 * (n_src * e + j) + c];
   }
-  const int n_radii = min(__builtin_amdgcn_readfirstlane(P->n_radii), MAXRADII);
   for (int j = tid; j < n_radii; j += W * 64) l.rad2[j] = P->rad2[j];
   const int pstart = __builtin_amdgcn_readfirstlane(P->psfgauss_start[e]);
   const float gcal = uniform_f(P->G[e]);
@@ -503,7 +508,8 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
     box.y1 = uniform_f(wave_max(y1));
   }
 
-  const int n_surv = cull_tile<W>(P, pstart, n_src, n_psf, box, l, wave, lane);
+  const int n_surv = cull_tile<W>(P, pstart, n_src, n_psf, n_radii, box, l, wave, lane,
+                                  slot.status);
   const int n_chunks = (n_surv + gauss_cap - 1) / gauss_cap;
   auto chunk_size = [&](int c) { return min(gauss_cap, n_surv - c * gauss_cap); };
   if (!vjp) {
@@ -590,15 +596,18 @@ __device__ void finish_tile(const FpxSlot &slot, int band, int n_src, unsigned c
 }
 
 // Outputs of every slot from its accumulator, which is left zero: pchi2[b]
-// and the float Response of each band.  Grid (slot, chunk of kFinalizeThreads
-// accumulator values): every thread moves at most one value, so the latency
-// of the agent-scope load and store is paid once, not once per value.
+// and the float Response of each band; pchi2 is NaN in every band of a slot
+// whose status word is set (a tile broke a limit and was left out).  Grid
+// (slot, chunk of kFinalizeThreads accumulator values): every thread moves
+// at most one value, so the latency of the agent-scope load and store is
+// paid once, not once per value.  slots == NULL: the one slot `one`, passed
+// by value.
 constexpr int kFinalizeThreads = 256;
 constexpr int kFinalizeChunks = (MAXBANDS * kAcc + kFinalizeThreads - 1) / kFinalizeThreads;
 
 __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(
-    const FpxSlot *__restrict__ slots, int n_slots) {
-  const FpxSlot slot = slots[blockIdx.x];
+    const FpxSlot *__restrict__ slots, FpxSlot one, int n_slots) {
+  const FpxSlot slot = slots != nullptr ? slots[blockIdx.x] : one;
   const int nb = min(slot.n_bands, MAXBANDS);
   const int j = blockIdx.y * kFinalizeThreads + threadIdx.x;
   if (j < nb * kAcc) {
@@ -606,7 +615,11 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(
     const int b = j / kAcc;
     const int k = j - b * kAcc;
     if (k == 0) {
-      slot.pchi2[b] = v;
+      const uint32_t st = slot.status == nullptr
+                              ? 0u
+                              : __hip_atomic_load(slot.status, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+      slot.pchi2[b] = st == 0u ? v : __builtin_nan("");
     } else {
       slot.pdchi2_dp[b * (MAXSOURCES * NPARAMS) + (k - 1)] = (float)v;
     }
@@ -627,21 +640,34 @@ __device__ void process_tile(const FpxSlot &slot, int local, int gauss_cap, int 
   const int e = __builtin_amdgcn_readfirstlane(tile.exposure);
   const int band = __builtin_amdgcn_readfirstlane(tile.band);
   const int n_src = __builtin_amdgcn_readfirstlane(P->n_sources);
-  const int n_psf = __builtin_amdgcn_readfirstlane(P->n_psf_per_source[band]);
-  // A tile whose sizes break the kernel's assumptions contributes nothing
-  // (every lane sees the same values, so the whole workgroup agrees).
-  const bool ok = n_src >= 1 && n_src <= MAXSOURCES && n_psf >= 1 && n_src * n_psf <= index_cap &&
-                  n_psf <= FPX_MAX_PSF_PER_SOURCE && band >= 0 && band < slot.n_bands &&
-                  band < MAXBANDS && tile.pix_count >= 0 && tile.pix_count <= kPix;
-  if (ok) tile_work<W, PPL>(slot, tile, P, props, e, band, n_src, n_psf, gauss_cap, smem);
+  const int n_radii = __builtin_amdgcn_readfirstlane(P->n_radii);
+  const bool band_ok = band >= 0 && band < slot.n_bands && band < MAXBANDS;
+  const int n_psf = band_ok ? __builtin_amdgcn_readfirstlane(P->n_psf_per_source[band]) : 0;
+  // A tile whose sizes break the kernel's assumptions contributes nothing and
+  // sets FPX_STATUS_LIMIT in the slot's status word, which makes the output
+  // kernel write NaN chi2 (every lane sees the same values, so the whole
+  // workgroup agrees).
+  const bool ok = band_ok && n_src >= 1 && n_src <= MAXSOURCES && n_psf >= 1 &&
+                  n_psf <= FPX_MAX_PSF_PER_SOURCE && n_src * n_psf <= index_cap &&
+                  n_radii >= 1 && n_radii <= MAXRADII && tile.pix_count >= 0 &&
+                  tile.pix_count <= kPix;
+  if (!ok && threadIdx.x == 0 && slot.status != nullptr)
+    atomicOr(slot.status, (uint32_t)FPX_STATUS_LIMIT);
+  if (ok)
+    tile_work<W, PPL>(slot, tile, P, props, e, band, n_src, n_psf, n_radii, gauss_cap, smem);
   __syncthreads();
   finish_tile<W>(slot, ok ? band : -1, ok ? n_src : 0, smem, gauss_cap, PPL);
 }
 
-template <int W, int PPL>
+// kOne: the single slot `one`, passed by value (no slot table in device
+// memory; slots is not read).  A compile-time variant rather than a runtime
+// choice, which would keep both slots' fields live (8 more VGPRs at 8x4, one
+// wavefront per SIMD fewer).
+template <int W, int PPL, bool kOne>
 __global__ __launch_bounds__(W * 64) void chi2_grad_kernel(
-    const FpxSlot *__restrict__ slots, int n_slots, const int32_t *__restrict__ item_offsets,
-    int n_items, int gauss_cap, int index_cap, int persistent, uint32_t *__restrict__ queue) {
+    const FpxSlot *__restrict__ slots, FpxSlot one, int n_slots,
+    const int32_t *__restrict__ item_offsets, int n_items, int gauss_cap, int index_cap,
+    int persistent, uint32_t *__restrict__ queue) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_item;
   for (int it = 0;; ++it) {
@@ -658,7 +684,7 @@ __global__ __launch_bounds__(W * 64) void chi2_grad_kernel(
     if (item >= n_items) break;
     int si = 0;
     int local = item;
-    if (n_slots > 1) {
+    if (!kOne && n_slots > 1) {
       int lo = 0, hi = n_slots - 1;  // largest si with item_offsets[si] <= item
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -667,7 +693,7 @@ __global__ __launch_bounds__(W * 64) void chi2_grad_kernel(
       si = lo;
       local = item - item_offsets[si];
     }
-    const FpxSlot slot = slots[si];
+    const FpxSlot slot = kOne ? one : slots[si];
     process_tile<W, PPL>(slot, local, gauss_cap, index_cap, smem);
   }
   if (persistent) {
@@ -798,6 +824,17 @@ int hip_err(hipError_t e, const char *where) {
   return set_err(FPX_E_HIP, std::string(where) + ": " + hipGetErrorString(e));
 }
 
+// Text of a nonzero slot status word (FPX_STATUS_* bits).
+std::string status_message(uint32_t status) {
+  std::string m = "the kernel left tiles out:";
+  if (status & FPX_STATUS_LIMIT)
+    m += " a tile broke a size limit (n_sources, n_psf_per_source, n_radii, band, pixel count, "
+         "or max_gauss below n_sources x n_psf_per_source);";
+  if (status & FPX_STATUS_RADIUS_BIN) m += " a PSF entry's radius bin is outside [0, n_radii);";
+  if (status & ~(uint32_t)(FPX_STATUS_LIMIT | FPX_STATUS_RADIUS_BIN)) m += " unknown status bits;";
+  return m;
+}
+
 #define FPX_HIP(call)                                  \
   do {                                                 \
     hipError_t _e = (call);                            \
@@ -812,13 +849,18 @@ constexpr int kLdsPerWave = 10 * 1024;
 // query; launches use the kernel's reported size
 constexpr int kStaticLds = 16;
 
+// A tile shape and its two kernels: fn[0] reads a slot table, fn[1] takes
+// one slot by value.
 struct KernelCfg {
   int waves;
   int ppl;
-  const void *fn;
+  const void *fn[2];
 };
 
-#define FPX_CFG(W, P) {W, P, reinterpret_cast<const void *>(&fpx::chi2_grad_kernel<W, P>)}
+#define FPX_CFG(W, P)                                                               \
+  {W, P,                                                                            \
+   {reinterpret_cast<const void *>(&fpx::chi2_grad_kernel<W, P, false>),            \
+    reinterpret_cast<const void *>(&fpx::chi2_grad_kernel<W, P, true>)}}
 const KernelCfg kConfigs[] = {
     FPX_CFG(4, 1), FPX_CFG(4, 2), FPX_CFG(4, 4), FPX_CFG(8, 8),
     FPX_CFG(16, 4), FPX_CFG(8, 4), FPX_CFG(2, 4), FPX_CFG(4, 8),
@@ -832,13 +874,14 @@ struct DeviceState {
   int n_cu = 0;
   int lds_max = 0;
   int clock_khz = 0;
-  size_t lds_set[kNumConfigs] = {};
-  // resident workgroups per CU, cached per configuration and LDS size, so that
-  // a launch recorded into a graph issues no occupancy query
-  size_t occ_lds[kNumConfigs] = {};
-  int occ_per_cu[kNumConfigs] = {};
+  // per kernel, index 2 * configuration + (one slot by value)
+  size_t lds_set[2 * kNumConfigs] = {};
+  // resident workgroups per CU, cached per kernel and LDS size, so that a
+  // launch recorded into a graph issues no occupancy query
+  size_t occ_lds[2 * kNumConfigs] = {};
+  int occ_per_cu[2 * kNumConfigs] = {};
   size_t census_lds = 0;
-  int static_lds[kNumConfigs];  // static LDS of each kernel, -1 until queried
+  int static_lds[2 * kNumConfigs];  // static LDS of each kernel, -1 until queried
   DeviceState() {
     for (int &v : static_lds) v = -1;
   }
@@ -894,37 +937,40 @@ int pick_gauss_cap(const KernelCfg &kc, int max_gauss, int lds_max, int static_l
 // Workgroups of configuration cfg with `lds` bytes of dynamic LDS that fit
 // one CU at once (cached, so that a launch recorded into a graph makes no
 // occupancy query).
-int groups_per_cu(DeviceState *st, int cfg, size_t lds, int *per_cu) {
+int groups_per_cu(DeviceState *st, int cfg, int one, size_t lds, int *per_cu) {
+  const int k = 2 * cfg + one;
   {
     std::lock_guard<std::mutex> lock(g_mu);
-    if (st->occ_lds[cfg] == lds && st->occ_per_cu[cfg] > 0) {
-      *per_cu = st->occ_per_cu[cfg];
+    if (st->occ_lds[k] == lds && st->occ_per_cu[k] > 0) {
+      *per_cu = st->occ_per_cu[k];
       return FPX_OK;
     }
   }
   int n = 0;
-  FPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kConfigs[cfg].fn,
+  FPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kConfigs[cfg].fn[one],
                                                        kConfigs[cfg].waves * 64, lds));
   if (n < 1) n = 1;
   std::lock_guard<std::mutex> lock(g_mu);
-  st->occ_lds[cfg] = lds;
-  st->occ_per_cu[cfg] = n;
+  st->occ_lds[k] = lds;
+  st->occ_per_cu[k] = n;
   *per_cu = n;
   return FPX_OK;
 }
 
 // Dynamic LDS bytes and records per chunk of a launch of configuration cfg.
-int launch_lds(DeviceState *st, int cfg, int max_gauss, size_t *lds, int *cap, int *index_cap) {
+int launch_lds(DeviceState *st, int cfg, int one, int max_gauss, size_t *lds, int *cap,
+               int *index_cap) {
   const KernelCfg &kc = kConfigs[cfg];
+  const int k = 2 * cfg + one;
   int static_lds = 0;
   {
     std::lock_guard<std::mutex> lock(g_mu);
-    if (st->static_lds[cfg] < 0) {
+    if (st->static_lds[k] < 0) {
       hipFuncAttributes attr;
-      FPX_HIP(hipFuncGetAttributes(&attr, kc.fn));
-      st->static_lds[cfg] = (int)attr.sharedSizeBytes;
+      FPX_HIP(hipFuncGetAttributes(&attr, kc.fn[one]));
+      st->static_lds[k] = (int)attr.sharedSizeBytes;
     }
-    static_lds = st->static_lds[cfg];
+    static_lds = st->static_lds[k];
   }
   *cap = pick_gauss_cap(kc, max_gauss, st->lds_max, static_lds);
   if (*cap < 1) return set_err(FPX_E_ARG, "max_gauss too large for the LDS of one workgroup");
@@ -933,13 +979,17 @@ int launch_lds(DeviceState *st, int cfg, int max_gauss, size_t *lds, int *cap, i
   return FPX_OK;
 }
 
-// Launch the tile kernel (and, with finalize, the output pass after it).
-int launch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
-           int32_t n_items, int32_t max_gauss, int32_t cfg, int32_t persistent,
-           uint32_t *queue, void *stream, bool finalize_outputs = true) {
+// Launch the tile kernel (and, with finalize, the output pass after it) over
+// the slot table `slots` (device) or, when slots is NULL, the single slot
+// *one (host struct, passed by value as a kernel argument).
+int launch(const FpxSlot *slots, const FpxSlot *one, int32_t n_slots,
+           const int32_t *item_offsets, int32_t n_items, int32_t max_gauss, int32_t cfg,
+           int32_t persistent, uint32_t *queue, void *stream, bool finalize_outputs = true) {
   int rc = check_cfg(cfg);
   if (rc) return rc;
-  if (!slots || n_slots <= 0 || n_items < 0) return set_err(FPX_E_ARG, "empty or invalid slot list");
+  if ((!slots && !one) || n_slots <= 0 || n_items < 0)
+    return set_err(FPX_E_ARG, "empty or invalid slot list");
+  if (!slots && n_slots != 1) return set_err(FPX_E_ARG, "a slot passed by value is one slot");
   if (n_slots > 1 && !item_offsets) return set_err(FPX_E_ARG, "item_offsets required for n_slots > 1");
   if (max_gauss <= 0 || max_gauss > MAXSOURCES * FPX_MAX_PSF_PER_SOURCE)
     return set_err(FPX_E_ARG, "max_gauss out of range [1, MAXSOURCES*FPX_MAX_PSF_PER_SOURCE]");
@@ -947,7 +997,10 @@ int launch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
   DeviceState *st = nullptr;
   rc = current_device_state(&st);
   if (rc) return rc;
-  void *fargs[] = {(void *)&slots, (void *)&n_slots};
+  FpxSlot byval;
+  std::memset(&byval, 0, sizeof(byval));
+  if (!slots) byval = *one;
+  void *fargs[] = {(void *)&slots, (void *)&byval, (void *)&n_slots};
   const void *finalize = reinterpret_cast<const void *>(&fpx::finalize_kernel);
   if (n_items == 0) {  // no pixels: chi2 and gradients are zero
     if (finalize_outputs)
@@ -956,29 +1009,31 @@ int launch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
     return FPX_OK;
   }
   const KernelCfg &kc = kConfigs[cfg];
+  const int v = slots ? 0 : 1;
   int cap = 0, index_cap = 0;
   size_t lds = 0;
-  rc = launch_lds(st, cfg, max_gauss, &lds, &cap, &index_cap);
+  rc = launch_lds(st, cfg, v, max_gauss, &lds, &cap, &index_cap);
   if (rc) return rc;
   {
     std::lock_guard<std::mutex> lock(g_mu);
-    if (st->lds_set[cfg] < lds) {
-      FPX_HIP(hipFuncSetAttribute(kc.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      st->lds_set[cfg] = lds;
+    if (st->lds_set[2 * cfg + v] < lds) {
+      FPX_HIP(hipFuncSetAttribute(kc.fn[v], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds));
+      st->lds_set[2 * cfg + v] = lds;
     }
   }
   const int threads = kc.waves * 64;
   int grid = n_items;
   if (persistent) {
     int per_cu = 0;
-    rc = groups_per_cu(st, cfg, lds, &per_cu);
+    rc = groups_per_cu(st, cfg, v, lds, &per_cu);
     if (rc) return rc;
     grid = std::min<long>((long)per_cu * st->n_cu, (long)n_items);
   }
-  void *args[] = {(void *)&slots,     (void *)&n_slots,    (void *)&item_offsets,
-                  (void *)&n_items,   (void *)&cap,        (void *)&index_cap,
-                  (void *)&persistent, (void *)&queue};
-  FPX_HIP(hipLaunchKernel(kc.fn, dim3(grid), dim3(threads), args, lds, (hipStream_t)stream));
+  void *args[] = {(void *)&slots,     (void *)&byval,      (void *)&n_slots,
+                  (void *)&item_offsets, (void *)&n_items, (void *)&cap,
+                  (void *)&index_cap, (void *)&persistent, (void *)&queue};
+  FPX_HIP(hipLaunchKernel(kc.fn[v], dim3(grid), dim3(threads), args, lds, (hipStream_t)stream));
   if (finalize_outputs)
     FPX_HIP(hipLaunchKernel(finalize, dim3(n_slots, fpx::kFinalizeChunks),
                             dim3(fpx::kFinalizeThreads), fargs, 0, (hipStream_t)stream));
@@ -990,6 +1045,15 @@ int launch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
 namespace fpx {
 int report_error(int code, const char *msg) { return set_err(code, msg); }
 int report_hip_error(hipError_t e, const char *where) { return hip_err(e, where); }
+int launch_one(const FpxSlot &slot, int32_t n_tiles, int32_t max_gauss, int32_t cfg,
+               int32_t persistent, uint32_t *queue, void *stream) {
+  return launch(nullptr, &slot, 1, nullptr, n_tiles, max_gauss, cfg, persistent ? 1 : 0, queue,
+                stream);
+}
+std::string status_text(uint32_t status) { return status_message(status); }
+int config_pixels(int32_t cfg) {
+  return (cfg >= 0 && cfg < kNumConfigs) ? 64 * kConfigs[cfg].ppl : 0;
+}
 }  // namespace fpx
 
 extern "C" {
@@ -1080,9 +1144,9 @@ int fpx_resident_groups(int32_t cfg, int32_t max_gauss, int32_t *groups) {
   if (rc) return rc;
   int cap = 0, index_cap = 0, per_cu = 0;
   size_t lds = 0;
-  rc = launch_lds(st, cfg, max_gauss, &lds, &cap, &index_cap);
+  rc = launch_lds(st, cfg, 0, max_gauss, &lds, &cap, &index_cap);
   if (rc) return rc;
-  rc = groups_per_cu(st, cfg, lds, &per_cu);
+  rc = groups_per_cu(st, cfg, 0, lds, &per_cu);
   if (rc) return rc;
   *groups = per_cu * st->n_cu;
   return FPX_OK;
@@ -1109,13 +1173,13 @@ int fpx_device_info(int32_t *n_cu, int32_t *lds_per_group, int32_t *clock_khz) {
 
 int fpx_evaluate_proposal(const FpxSlot *slot, int32_t n_tiles, int32_t max_gauss,
                           int32_t cfg, void *stream) {
-  return launch(slot, 1, nullptr, n_tiles, max_gauss, cfg, 0, nullptr, stream);
+  return launch(slot, nullptr, 1, nullptr, n_tiles, max_gauss, cfg, 0, nullptr, stream);
 }
 
 int fpx_evaluate_sync(const FpxSlot *slot, int32_t n_tiles, int32_t max_gauss, int32_t cfg,
                       int32_t persistent, uint32_t *queue, void *proposal_dev,
                       const void *proposal_host, size_t proposal_bytes, const void *out_dev,
-                      void *out_host, size_t out_bytes, void *stream) {
+                      void *out_host, size_t out_bytes, uint32_t *status_dev, void *stream) {
   if (!proposal_dev || !proposal_host || !out_dev || !out_host)
     return set_err(FPX_E_ARG, "NULL proposal or output buffer");
   if (proposal_bytes > sizeof(FpxSource) * MAXSOURCES)
@@ -1124,25 +1188,39 @@ int fpx_evaluate_sync(const FpxSlot *slot, int32_t n_tiles, int32_t max_gauss, i
   const hipStream_t s = (hipStream_t)stream;
   if (proposal_bytes > 0)
     FPX_HIP(hipMemcpyAsync(proposal_dev, proposal_host, proposal_bytes, hipMemcpyHostToDevice, s));
-  const int rc = launch(slot, 1, nullptr, n_tiles, max_gauss, cfg, persistent ? 1 : 0, queue, stream);
+  const int rc = launch(slot, nullptr, 1, nullptr, n_tiles, max_gauss, cfg, persistent ? 1 : 0,
+                        queue, stream);
   if (rc) return rc;
   FPX_HIP(hipMemcpyAsync(out_host, out_dev, out_bytes, hipMemcpyDeviceToHost, s));
+  // the status word: from the copied outputs when it lies among them, else
+  // copied on its own
+  const uintptr_t o0 = (uintptr_t)out_dev, sp = (uintptr_t)status_dev;
+  const bool inside = status_dev && sp >= o0 && sp + sizeof(uint32_t) <= o0 + out_bytes;
+  uint32_t status = 0;
+  if (status_dev && !inside)
+    FPX_HIP(hipMemcpyAsync(&status, status_dev, sizeof(status), hipMemcpyDeviceToHost, s));
   FPX_HIP(hipStreamSynchronize(s));
+  if (inside) std::memcpy(&status, (const unsigned char *)out_host + (sp - o0), sizeof(status));
+  if (status != 0) {
+    FPX_HIP(hipMemsetAsync(status_dev, 0, sizeof(uint32_t), s));
+    FPX_HIP(hipStreamSynchronize(s));
+    return set_err(FPX_E_ARG, status_message(status));
+  }
   return FPX_OK;
 }
 
 int fpx_evaluate_batch(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
                        int32_t n_items, int32_t max_gauss, int32_t cfg, int32_t persistent,
                        uint32_t *queue, void *stream) {
-  return launch(slots, n_slots, item_offsets, n_items, max_gauss, cfg, persistent ? 1 : 0,
-                queue, stream);
+  return launch(slots, nullptr, n_slots, item_offsets, n_items, max_gauss, cfg,
+                persistent ? 1 : 0, queue, stream);
 }
 
 int fpx_evaluate_tiles(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
                        int32_t n_items, int32_t max_gauss, int32_t cfg, int32_t persistent,
                        uint32_t *queue, void *stream) {
-  return launch(slots, n_slots, item_offsets, n_items, max_gauss, cfg, persistent ? 1 : 0,
-                queue, stream, false);
+  return launch(slots, nullptr, n_slots, item_offsets, n_items, max_gauss, cfg,
+                persistent ? 1 : 0, queue, stream, false);
 }
 
 int fpx_finalize(const FpxSlot *slots, int32_t n_slots, void *stream) {
@@ -1150,7 +1228,9 @@ int fpx_finalize(const FpxSlot *slots, int32_t n_slots, void *stream) {
   DeviceState *st = nullptr;
   int rc = current_device_state(&st);
   if (rc) return rc;
-  void *fargs[] = {(void *)&slots, (void *)&n_slots};
+  FpxSlot none;
+  std::memset(&none, 0, sizeof(none));
+  void *fargs[] = {(void *)&slots, (void *)&none, (void *)&n_slots};
   FPX_HIP(hipLaunchKernel(reinterpret_cast<const void *>(&fpx::finalize_kernel),
                           dim3(n_slots, fpx::kFinalizeChunks), dim3(fpx::kFinalizeThreads), fargs,
                           0, (hipStream_t)stream));

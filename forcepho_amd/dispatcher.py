@@ -105,11 +105,8 @@ class PatchDispatcher:
         ens.eps.fill_(self.step_size)
         gen = torch.Generator(device=ens.device)
         gen.manual_seed(int(self.seed if self.seed is not None else np.random.randint(2 ** 31)))
-        grow = torch.tensor(1.15, dtype=torch.float64, device=ens.device)
-        shrink = torch.tensor(0.6, dtype=torch.float64, device=ens.device)
-        for _ in range(self.n_warmup):
-            acc = ens.hmc_step(self.n_leapfrog, generator=gen, use_graph=self.use_graph)
-            ens.eps.mul_(torch.where(acc, grow, shrink)[:, None, None])
+        ens.adapt_step_size(self.n_warmup, self.n_leapfrog, generator=gen,
+                            use_graph=self.use_graph)
         sums = [np.zeros_like(np.asarray(t.q, dtype=np.float64)) for t in tasks]
         n_acc = np.zeros(len(tasks))
         for _ in range(self.n_samples):

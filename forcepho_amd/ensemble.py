@@ -202,5 +202,62 @@ class HMCEnsemble:
         return accept.bool()
 
 
+    # ------------------------------------------------------------------
+    def _energy_change(self, z0, p0, n_steps):
+        """H1 - H0 of an eager n_steps trajectory of every chain from (z0,
+        p0), H = U + |p|^2 / 2 (a non-finite energy gives +inf); z and p are
+        left at the trajectory's end."""
+        torch = _torch()
+        self.z.copy_(z0)
+        self.p.copy_(p0)
+        self.trajectory(n_steps, use_graph=False)
+        h0 = self.U_start + 0.5 * (p0 * p0).sum(dim=(1, 2))
+        h1 = self.U + 0.5 * (self.p * self.p).sum(dim=(1, 2))
+        dh = h1 - h0
+        return torch.where(torch.isfinite(dh), dh, torch.full_like(dh, float("inf")))
+
+    def find_step_size(self, generator=None, max_iter=20):
+        """A starting step size per chain by the heuristic of Hoffman & Gelman
+        (2014, Algorithm 4), for every chain at once: from the current
+        position and fresh momenta, double (or halve) eps until the
+        acceptance probability of one leapfrog step crosses 1/2, each chain
+        stopping at its own crossing.  The position is left unchanged.
+        Returns eps (device, per chain)."""
+        import math
+        torch = _torch()
+        z0 = self.z.clone()
+        p0 = torch.randn(self.p.shape, generator=generator, dtype=torch.float64,
+                         device=self.device) * self.fmask
+        half = math.log(0.5)
+        logacc = -self._energy_change(z0, p0, 1)
+        direction = torch.where(logacc > half, 1.0, -1.0).to(torch.float64)
+        active = torch.ones(self.n_slots, dtype=torch.bool, device=self.device)
+        for _ in range(max_iter):
+            factor = torch.where(active, 2.0 ** direction, torch.ones_like(direction))
+            self.eps.mul_(factor[:, None, None])
+            logacc = -self._energy_change(z0, p0, 1)
+            active &= direction * logacc > direction * half
+            if not bool(active.any()):
+                break
+        self.z.copy_(z0)
+        self.p.zero_()
+        return self.eps.view(-1).clone()
+
+    def adapt_step_size(self, n_rounds, n_steps, generator=None, grow=1.15, shrink=0.6,
+                        use_graph=False):
+        """n_rounds HMC transitions of n_steps leapfrog steps, each chain's
+        eps multiplied by ``grow`` after an accepted transition and by
+        ``shrink`` after a rejected one.  Returns the last round's
+        acceptance flags."""
+        torch = _torch()
+        g = torch.tensor(grow, dtype=torch.float64, device=self.device)
+        k = torch.tensor(shrink, dtype=torch.float64, device=self.device)
+        acc = None
+        for _ in range(n_rounds):
+            acc = self.hmc_step(n_steps, generator=generator, use_graph=use_graph)
+            self.eps.mul_(torch.where(acc, g, k)[:, None, None])
+        return acc
+
+
 def torch_sigmoid(x):
     return _torch().sigmoid(x)

@@ -265,6 +265,39 @@ class HIPPatchMixin(DevicePatchMixin):
     def max_gauss(self):
         return int(self.n_sources) * int(np.max(self.n_psf_per_source))
 
+    def check_limits(self):
+        """Raise ValueError unless the packed patch is within the limits the
+        kernel relies on: n_sources, n_bands and n_radii within MAXSOURCES,
+        MAXBANDS and MAXRADII, n_psf_per_source within FPX_MAX_PSF_PER_SOURCE,
+        and every radius bin the kernel reads (the first n_psf_per_source
+        entries from psfgauss_start of each exposure) within [0, n_radii).
+        The reference checks none of these (exceeding them is undefined,
+        forcepho/src/kernel_limits.h); the kernel itself leaves out a tile
+        that breaks one and reports it through the slot's status word.
+        Checked once per upload of the patch struct."""
+        gen = getattr(self, "_struct_generation", None)
+        if gen is not None and getattr(self, "_limits_checked", None) == gen:
+            return
+        from ..kernel_limits import MAXBANDS, MAXRADII, MAXSOURCES, FPX_MAX_PSF_PER_SOURCE
+        ns, nb, nr = int(self.n_sources), int(self.n_bands), int(self.n_radii)
+        if not 1 <= ns <= MAXSOURCES:
+            raise ValueError(f"n_sources {ns} outside [1, MAXSOURCES={MAXSOURCES}]")
+        if not 1 <= nb <= MAXBANDS:
+            raise ValueError(f"n_bands {nb} outside [1, MAXBANDS={MAXBANDS}]")
+        if not 1 <= nr <= MAXRADII:
+            raise ValueError(f"n_radii {nr} outside [1, MAXRADII={MAXRADII}]")
+        n_psf = np.asarray(self.n_psf_per_source[:nb], dtype=np.int64)
+        if n_psf.min() < 1 or n_psf.max() > FPX_MAX_PSF_PER_SOURCE:
+            raise ValueError(f"n_psf_per_source outside [1, {FPX_MAX_PSF_PER_SOURCE}]")
+        bins = np.asarray(self.psfgauss["sersic_bin"])
+        for b in range(nb):
+            for e in range(int(self.band_start[b]), int(self.band_start[b]) + int(self.band_N[b])):
+                s = int(self.psfgauss_start[e])
+                block = bins[s:s + int(n_psf[b])]
+                if len(block) < n_psf[b] or block.min() < 0 or block.max() >= nr:
+                    raise ValueError(f"exposure {e}: PSF radius bins outside [0, n_radii={nr})")
+        self._limits_checked = gen
+
     def free(self):
         self.device_ptrs = {}
         self.device_patch = None

@@ -19,12 +19,14 @@ import sys
 import numpy as np
 
 from .kernel_limits import MAXBANDS, MAXRADII, MAXSOURCES, NPARAMS
-from ._lib import lib, check, SLOT_DTYPE, SLOT_MODEL_ONLY
+from ._lib import lib, check, FpxError, SLOT_DTYPE, SLOT_MODEL_ONLY
 
-# device output block of a proposer: pchi2 (double[MAXBANDS]) then the
-# Responses (float[MAXBANDS][MAXSOURCES][NPARAMS])
-_CHI2_BYTES = MAXBANDS * 8
-_OUT_BYTES = _CHI2_BYTES + MAXBANDS * MAXSOURCES * NPARAMS * 4
+# device output block of a proposer: the slot's status word (uint32, padded
+# to 8 bytes), pchi2 (double[MAXBANDS]), then the Responses
+# (float[MAXBANDS][MAXSOURCES][NPARAMS]); one copy brings all three back
+_STATUS_BYTES = 8
+_CHI2_END = _STATUS_BYTES + MAXBANDS * 8
+_OUT_BYTES = _CHI2_END + MAXBANDS * MAXSOURCES * NPARAMS * 4
 
 __all__ = ["ProposerBase", "HIPProposer", "GPUProposer", "Proposer",
            "source_struct_dtype", "response_struct_dtype", "DEFAULT_CONFIG"]
@@ -46,7 +48,7 @@ response_float_dt = source_float_dt
 response_struct_dtype = np.dtype([("dchi2_dparam", response_float_dt, NPARAMS * MAXSOURCES)],
                                  align=True)
 
-# kernel configuration used when none is given: 4 wavefronts x 4 pixels/lane
+# kernel configuration used when none is given: 8 wavefronts x 4 pixels/lane
 DEFAULT_CONFIG = 5
 
 
@@ -88,12 +90,12 @@ class HIPProposer(ProposerBase):
         dev = self.device
         n_acc = lib().fpx_accum_doubles(MAXBANDS)
         self._accum = torch.zeros(n_acc, dtype=torch.float64, device=dev)
-        self._ticket = torch.zeros(2, dtype=torch.int32, device=dev)
-        # outputs back to back (chi2 doubles, then the Responses) so that one
-        # copy brings them to the host
+        # status word and outputs back to back, so that one copy brings them
+        # to the host
         self._out = torch.zeros(_OUT_BYTES, dtype=torch.uint8, device=dev)
-        self._chi2 = self._out[:_CHI2_BYTES].view(torch.float64)
-        self._resp = self._out[_CHI2_BYTES:].view(torch.float32)
+        self._status = self._out[:4].view(torch.int32)
+        self._chi2 = self._out[_STATUS_BYTES:_CHI2_END].view(torch.float64)
+        self._resp = self._out[_CHI2_END:].view(torch.float32)
         self._prop = torch.zeros(MAXSOURCES * source_struct_dtype.itemsize, dtype=torch.uint8,
                                  device=dev)
         self._slot = torch.zeros(SLOT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
@@ -103,8 +105,9 @@ class HIPProposer(ProposerBase):
         self._prop_h = torch.zeros(self._prop.numel(), dtype=torch.uint8, **pin)
         self._prop_np = self._prop_h.numpy()
         self._out_h = torch.zeros(_OUT_BYTES, dtype=torch.uint8, **pin)
-        self._chi2_np = self._out_h.numpy()[:_CHI2_BYTES].view(np.float64)
-        self._resp_np = self._out_h.numpy()[_CHI2_BYTES:].view(np.float32)
+        self._status_np = self._out_h.numpy()[:4].view(np.uint32)
+        self._chi2_np = self._out_h.numpy()[_STATUS_BYTES:_CHI2_END].view(np.float64)
+        self._resp_np = self._out_h.numpy()[_CHI2_END:].view(np.float32)
         self._slot_key = None
         self._h2d_done = None   # event of the last copy out of the pinned proposal buffer
 
@@ -179,7 +182,7 @@ class HIPProposer(ProposerBase):
             slot["pchi2"] = self._chi2.data_ptr()
             slot["pdchi2_dp"] = self._resp.data_ptr()
             slot["accum"] = self._accum.data_ptr()
-            slot["ticket"] = self._ticket.data_ptr()
+            slot["status"] = self._out.data_ptr()
             slot["n_bands"] = int(patch.n_bands)
             slot["n_tiles"] = n_tiles
             slot["flags"] = flags
@@ -291,12 +294,13 @@ class HIPProposer(ProposerBase):
     def _load(self, proposal, patch, pixel_weight, stream=None, flags=0):
         """Stage the proposal and the slot for ``patch``; returns the tile
         count.  A numpy proposal is copied through pinned memory into the
-        proposer's buffer; a device tensor is read in place by the launch (it
-        must stay alive and unchanged until the launch has run)."""
+        proposer's buffer on ``stream`` (the launch stream); a device tensor
+        is read in place by the launch (it must stay alive and unchanged
+        until the launch has run)."""
         import torch
+        patch.check_limits()
         n_src = int(patch.n_sources)
-        if n_src > MAXSOURCES or int(patch.n_bands) > MAXBANDS:
-            raise ValueError("patch exceeds MAXSOURCES/MAXBANDS")
+        stream = stream or self._stream()
         if isinstance(proposal, np.ndarray):
             if len(proposal) != n_src:
                 raise ValueError(f"proposal has {len(proposal)} sources, patch has {n_src}")
@@ -304,9 +308,10 @@ class HIPProposer(ProposerBase):
             if self._h2d_done is not None:
                 self._h2d_done.synchronize()   # the pinned buffer may still be in flight
             self._prop_np[:raw.size] = raw.reshape(-1)
-            self._prop.copy_(self._prop_h, non_blocking=True)
+            with torch.cuda.stream(stream):
+                self._prop.copy_(self._prop_h, non_blocking=True)
             self._h2d_done = torch.cuda.Event()
-            self._h2d_done.record(stream or self._stream())
+            self._h2d_done.record(stream)
             return self._slot_for(patch, pixel_weight, flags=flags)
         if proposal.device != self.device or not proposal.is_contiguous():
             raise ValueError("device proposals must be contiguous tensors on the proposer's device")
@@ -314,10 +319,21 @@ class HIPProposer(ProposerBase):
             raise ValueError("device proposal holds fewer records than the patch has sources")
         return self._slot_for(patch, pixel_weight, prop_ptr=proposal.data_ptr(), flags=flags)
 
+    def _check_status(self):
+        """Raise FpxError if the status word copied back with the outputs is
+        set (the kernel left tiles out); the device word is cleared first."""
+        st = int(self._status_np[0])
+        if st:
+            self._status_np[0] = 0
+            self._status.zero_()
+            raise FpxError(f"the kernel left tiles out (status {st:#x}): a size limit or a "
+                           "radius bin of the packed patch is out of range")
+
     def evaluate_proposal(self, proposal, patch=None, verbose=False, unpack=True):
         """Evaluate chi2 and dchi2/dparam of ``proposal`` on ``patch``."""
         if patch is None:
             patch = self.patch
+        import torch
         n_bands, n_sources = int(patch.n_bands), int(patch.n_sources)
         if verbose:
             print(f"fpx launch: config {self.config}, {n_bands} bands, {n_sources} sources",
@@ -328,25 +344,27 @@ class HIPProposer(ProposerBase):
             # call (fpx_evaluate_sync) through the pinned staging buffers
             if len(proposal) != n_sources:
                 raise ValueError(f"proposal has {len(proposal)} sources, patch has {n_sources}")
-            if n_sources > MAXSOURCES or n_bands > MAXBANDS:
-                raise ValueError("patch exceeds MAXSOURCES/MAXBANDS")
+            patch.check_limits()
             raw = np.ascontiguousarray(proposal, dtype=source_struct_dtype).view(np.uint8)
             if self._h2d_done is not None:
                 self._h2d_done.synchronize()   # an earlier launch's copy may be in flight
                 self._h2d_done = None
             self._prop_np[:raw.size] = raw.reshape(-1)
             n_tiles = self._slot_for(patch, None)
-            out_bytes = _CHI2_BYTES + n_bands * MAXSOURCES * NPARAMS * 4
+            out_bytes = _CHI2_END + n_bands * MAXSOURCES * NPARAMS * 4
             check(lib().fpx_evaluate_sync(self._slot.data_ptr(), n_tiles, patch.max_gauss(),
                                           self.config, int(self.persistent),
                                           self._queue.data_ptr(), self._prop.data_ptr(),
                                           self._prop_h.data_ptr(), raw.size,
                                           self._out.data_ptr(), self._out_h.data_ptr(),
-                                          out_bytes, stream.cuda_stream), "fpx_evaluate_sync")
+                                          out_bytes, self._out.data_ptr(), stream.cuda_stream),
+                  "fpx_evaluate_sync")
         else:
             self.launch(proposal, patch, stream=stream)
-            self._out_h.copy_(self._out, non_blocking=True)
+            with torch.cuda.stream(stream):
+                self._out_h.copy_(self._out, non_blocking=True)
             stream.synchronize()
+            self._check_status()
         chi_out = self._chi2_np[:n_bands].astype(self.chi_dtype, copy=True)
         derivs = self._resp_np[:n_bands * MAXSOURCES * NPARAMS].reshape(n_bands, MAXSOURCES, NPARAMS)
         chi2_derivs = derivs[:, :n_sources].astype(np.float64)
@@ -372,8 +390,11 @@ class HIPProposer(ProposerBase):
         stream = self._stream()
         self.launch(proposal, patch, pixel_weight=pixel_weight, stream=stream)
         nr = n_bands * MAXSOURCES * NPARAMS
-        self._out_h.copy_(self._out, non_blocking=True)
+        import torch
+        with torch.cuda.stream(stream):
+            self._out_h.copy_(self._out, non_blocking=True)
         stream.synchronize()
+        self._check_status()
         d = self._resp_np[:nr].reshape(n_bands, MAXSOURCES, NPARAMS)
         return d[:, :n_sources].astype(np.float64)
 

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define FPX_ABI_VERSION 1
+#define FPX_ABI_VERSION 2
 
 /* ------------------------------------------------------------------------ */
 /* Wire structs.  Byte layouts equal the reference's (checked by           */
@@ -30,7 +30,7 @@ extern "C" {
 /* ------------------------------------------------------------------------ */
 
 /* One PSF Gaussian, already tagged with the Sersic radius bin it convolves.
- * Replaces PSFSourceGaussian (forcepho/src/patch.cc:211-227); field order is
+ * Replaces PSFSourceGaussian (forcepho/src/patch.cc:103-119); field order is
  * storage.PSF_COLS + sersic_bin (forcepho/patches/storage.py:27-28). 28 bytes. */
 typedef struct FpxPSFSourceGaussian {
     float amp;
@@ -40,7 +40,7 @@ typedef struct FpxPSFSourceGaussian {
 } FpxPSFSourceGaussian;
 
 /* Pixel data and per-exposure metadata of one patch, all arrays device
- * resident.  Replaces Patch (forcepho/src/patch.cc:131-208; numpy dtype
+ * resident.  Replaces Patch (forcepho/src/patch.cc:23-100; numpy dtype
  * forcepho/patches/patch.py:44-73). 152 bytes.
  *   data, ierr, xpix, ypix, residual : float[npix], exposures concatenated in
  *       band order, each exposure padded to a multiple of FPX_WAVE pixels;
@@ -114,7 +114,13 @@ typedef struct FpxTile {
  * many (many patches, many HMC chains on one patch, ...).  80 bytes.
  *   accum  : double[fpx_accum_doubles(n_bands)], zero before the first launch;
  *            the kernel leaves it zero again after each launch
- *   ticket : reserved (unused by this version; may be NULL)
+ *   status : NULL, or a device uint32 word, zero before the launch.  A tile
+ *            that breaks a limit (FPX_STATUS_* bits below) is left out and
+ *            ORs its bit into the word; the output kernel then writes NaN
+ *            into pchi2 of every band of the slot.  The word keeps its bits
+ *            until the caller clears it (fpx_evaluate_sync clears it and
+ *            returns FPX_E_ARG).  The reference has no such checks
+ *            (exceeding its limits is undefined, kernel_limits.h).
  *   pixel_weight : NULL for chi2 mode.  Non-NULL selects vector-Jacobian mode:
  *            no model pass, pchi2 is written as 0 and pdchi2_dp receives
  *            sum_pix pixel_weight[pix] * d model[pix] / d param.
@@ -131,7 +137,7 @@ typedef struct FpxSlot {
     double *pchi2;        /* device double[n_bands] */
     float *pdchi2_dp;     /* device FpxResponse[n_bands] */
     double *accum;
-    uint32_t *ticket;
+    uint32_t *status;
     int32_t n_bands;
     int32_t n_tiles;
     int32_t flags;
@@ -139,6 +145,13 @@ typedef struct FpxSlot {
 } FpxSlot;
 
 #define FPX_SLOT_MODEL_ONLY 1
+
+/* Bits of FpxSlot.status. */
+#define FPX_STATUS_LIMIT 1u       /* a tile broke a size limit: n_sources, n_psf_per_source of
+                                     its band, n_radii, its band index, its pixel count (tile
+                                     table planned for a larger tile), or max_gauss below
+                                     n_sources x n_psf_per_source */
+#define FPX_STATUS_RADIUS_BIN 2u  /* a PSF entry's sersic_radius_bin is outside [0, n_radii) */
 
 /* A mixture-amplitude model that fpx_build_sources evaluates on the device:
  * the mixture amplitudes of a source and their derivatives with respect to
@@ -256,6 +269,9 @@ int fpx_evaluate_proposal(const FpxSlot *slot, int32_t n_tiles,
  * proposal buffer), the two kernels of fpx_evaluate_proposal (or the
  * persistent form, with queue) run, and out_bytes from out_dev (the slot's
  * pchi2 / pdchi2_dp region) are copied to out_host before the call returns.
+ * status_dev is the slot's status word or NULL; when it lies inside the
+ * copied region its value is read from the copy, else it is copied back on
+ * its own.  A nonzero status is cleared and the call returns FPX_E_ARG.
  * The PyCUDA call EvaluateProposal(patch, cuda.In(proposal),
  * cuda.Out(chi2), cuda.Out(derivs)) of forcepho/proposal.py:133-136 as one
  * native call.  Host buffers may be pinned (hipHostMalloc) or pageable. */
@@ -263,7 +279,75 @@ int fpx_evaluate_sync(const FpxSlot *slot, int32_t n_tiles, int32_t max_gauss,
                       int32_t cfg, int32_t persistent, uint32_t *queue,
                       void *proposal_dev, const void *proposal_host,
                       size_t proposal_bytes, const void *out_dev, void *out_host,
-                      size_t out_bytes, void *stream);
+                      size_t out_bytes, uint32_t *status_dev, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Bound patches: the reference's one-call launch, with the tile table,     */
+/* accumulator, queue and status word owned by the library.                 */
+/* ------------------------------------------------------------------------ */
+
+/* A patch bound for evaluation (opaque): its tile table, FP64 accumulator,
+ * work queue, status word and staging buffers on one device. */
+typedef struct FpxEvaluator FpxEvaluator;
+
+/* Bind a device-resident patch for evaluation on the current device.
+ * patch_dev is the device address of an FpxPatch whose arrays are device
+ * resident (what forcepho's send_patchstruct_to_device uploads,
+ * forcepho/patches/device_patch.py:310-341) and n_bands the number of bands
+ * (the reference's grid size, forcepho/proposal.py:133-136).  The call reads
+ * the struct and its index arrays back (synchronously), checks every limit
+ * the kernel relies on (n_sources in [1, MAXSOURCES], n_bands in
+ * [1, MAXBANDS], n_radii in [1, MAXRADII], n_psf_per_source in
+ * [1, FPX_MAX_PSF_PER_SOURCE], every radius bin the kernel reads in
+ * [0, n_radii), exposure starts and sizes non-negative), plans the tile
+ * table for kernel configuration cfg (-1: the library default) and
+ * allocates the evaluator's device buffers.  Evaluations read the struct at
+ * patch_dev each time, so swapping pixel buffers (swap_on_device,
+ * forcepho/patches/device_patch.py:343-364) needs no new bind; bind again
+ * when the packed metadata changes (another scene, other exposures).
+ *   persistent : 1 = a resident grid pulling tiles from a queue,
+ *                0 = one workgroup per tile
+ * No counterpart in the reference, whose launch needs no set-up beyond the
+ * uploaded struct; the tile table is what this kernel adds. */
+int fpx_bind(const FpxPatch *patch_dev, int32_t n_bands, int32_t cfg, int32_t persistent,
+             FpxEvaluator **out);
+
+/* chi2 and dchi2/dparam of one proposal on a bound patch, asynchronous on
+ * stream.  Replaces the kernel EvaluateProposal(patch, proposal, pchi2,
+ * pdchi2_dp) (forcepho/src/compute_gaussians_kernel.cu:212-214, launched
+ * <<<n_bands, 1024, 48000>>> by forcepho/proposal.py:133-136).
+ *   proposal_dev  : device FpxSource[n_sources]
+ *   pchi2_dev     : device double[n_bands]
+ *   pdchi2_dp_dev : device FpxResponse[n_bands]
+ * The residual image is written when the patch struct's residual is not
+ * NULL.  No allocation and no synchronisation with the host; calls on one
+ * evaluator share its buffers and must be ordered on one stream.  A limit
+ * broken after the bind (the struct at patch_dev changed) leaves NaN in
+ * pchi2 and sets the evaluator's status (fpx_evaluator_status). */
+int fpx_evaluate(FpxEvaluator *ev, const FpxSource *proposal_dev, double *pchi2_dev,
+                 FpxResponse *pdchi2_dp_dev, void *stream);
+
+/* The same evaluation with host buffers, synchronous: n_sources records of
+ * proposal_host in, chi2_host[n_bands] and resp_host[n_bands] out (resp_host
+ * may be NULL), the status checked and cleared (FPX_E_ARG if a tile was left
+ * out).  forcepho's GPUProposer.evaluate_proposal kernel call with its
+ * cuda.In / cuda.Out arguments (forcepho/proposal.py:133-136) as one native
+ * call; n_sources must be the bound patch's. */
+int fpx_evaluate_host(FpxEvaluator *ev, const FpxSource *proposal_host, int32_t n_sources,
+                      double *chi2_host, FpxResponse *resp_host, void *stream);
+
+/* Synchronise stream, then read and clear the evaluator's status word:
+ * FPX_OK if every tile of the evaluations since the last check was
+ * evaluated, else FPX_E_ARG (fpx_last_error names the broken limit). */
+int fpx_evaluator_status(FpxEvaluator *ev, void *stream);
+
+/* Tile count, kernel configuration and max_gauss (n_sources x the largest
+ * n_psf_per_source) of a bound patch; any output may be NULL. */
+int fpx_evaluator_info(const FpxEvaluator *ev, int32_t *n_tiles, int32_t *cfg,
+                       int32_t *max_gauss);
+
+/* Free an evaluator (after its last evaluation has completed). */
+int fpx_release(FpxEvaluator *ev);
 
 /* Many evaluations in one launch (uneven patches, many chains).
  *   slots        : device FpxSlot[n_slots]

@@ -30,6 +30,23 @@ GRAD_FLOOR = 4e-6
 MODEL_EPS = 1e-6
 
 
+# The parity bar against the reference CPU kernel itself: |g - g_ref| at most
+# this fraction of the largest |g_ref| of the same band (per-entry relative
+# errors are unbounded where a gradient crosses zero; the reference's own
+# float32 gradients sit within 1.5e-5 of each band's largest float64 entry on
+# every fixture and 4.3e-5 on the full-size multi-band patch).
+GRAD_BANDMAX_RTOL = 1e-4
+
+
+def bandmax_rel(g, g_ref):
+    """|g - g_ref| over the largest |g_ref| of the same band (axis 0)."""
+    g = np.asarray(g, dtype=np.float64)
+    g_ref = np.asarray(g_ref, dtype=np.float64)
+    axes = tuple(range(1, g_ref.ndim))
+    m = np.abs(g_ref).max(axis=axes, keepdims=True) if axes else np.abs(g_ref)
+    return np.abs(g - g_ref) / np.where(m > 0, m, 1e-300)
+
+
 def fp64_reference(patch, proposal):
     """float64 chi2, gradient, the noise-weighted magnitude sum|w dC/dtheta|
     and the signal-weighted magnitude sum 2 ierr^2 |d| |dC/dtheta|."""

@@ -31,7 +31,7 @@ def test_every_declared_symbol_is_exported_and_bound():
 
 def test_struct_sizes_match_dtypes():
     L = _lib.lib()
-    assert L.fpx_abi_version() == 1
+    assert L.fpx_abi_version() == 2
     assert L.fpx_struct_size(_lib.STRUCT_PATCH) == PatchBase.pdt.itemsize == 152
     assert L.fpx_struct_size(_lib.STRUCT_SOURCE) == source_struct_dtype.itemsize == 264
     assert L.fpx_struct_size(_lib.STRUCT_RESPONSE) == response_struct_dtype.itemsize == 840
@@ -45,7 +45,7 @@ def test_struct_sizes_match_dtypes():
 
 def test_reference_field_offsets():
     """Offsets of the reference's Patch and Source structs
-    (forcepho/src/patch.cc:131-208, forcepho/src/proposal.cu:13-27)."""
+    (forcepho/src/patch.cc:23-100, forcepho/src/proposal.cu:13-27)."""
     pdt = PatchBase.pdt
     expect = {"data": 0, "ierr": 8, "xpix": 16, "ypix": 24, "residual": 32,
               "exposure_start": 40, "exposure_N": 48, "band_start": 56, "band_N": 64,
@@ -145,10 +145,64 @@ def test_evaluate_sync_argument_checks():
 
     def call(prop_dev=p, prop_host=p, nbytes=264, out_dev=p, out_host=p, out_bytes=8):
         rc = L.fpx_evaluate_sync(p, 1, 1, 0, 0, None, prop_dev, prop_host, nbytes, out_dev,
-                                 out_host, out_bytes, None)
+                                 out_host, out_bytes, None, None)
         return rc, L.fpx_last_error().decode()
 
     assert call(prop_dev=None) == (-1, "NULL proposal or output buffer")
     assert call(out_host=None)[0] == -1
     assert call(nbytes=264 * 31) == (-1, "proposal larger than MAXSOURCES records")
     assert call(out_bytes=0) == (-1, "no output bytes to copy back")
+
+
+def test_evaluator_argument_checks_need_no_device():
+    """The bound-patch entry points refuse NULL handles and out-of-range
+    arguments before any HIP call."""
+    L = _lib.lib()
+    out = ctypes.c_void_p()
+    assert L.fpx_bind(None, 1, -1, 1, ctypes.byref(out)) == -1
+    assert L.fpx_last_error() == b"NULL patch" and out.value is None
+    assert L.fpx_bind(ctypes.c_void_p(8), 0, -1, 1, ctypes.byref(out)) == -1
+    assert b"n_bands" in L.fpx_last_error()
+    assert L.fpx_bind(ctypes.c_void_p(8), 31, -1, 1, ctypes.byref(out)) == -1
+    assert L.fpx_bind(ctypes.c_void_p(8), 2, 99, 1, ctypes.byref(out)) == -1
+    assert b"configuration" in L.fpx_last_error()
+    assert L.fpx_bind(ctypes.c_void_p(8), 2, -1, 1, None) == -1
+    p = ctypes.c_void_p(8)
+    assert L.fpx_evaluate(None, p, p, p, None) == -1
+    assert L.fpx_evaluate_host(None, p, 1, p, p, None) == -1
+    assert L.fpx_evaluator_status(None, None) == -1
+    assert L.fpx_evaluator_info(None, None, None, None) == -1
+    assert L.fpx_release(None) == 0
+
+
+def test_patch_limit_checks():
+    """check_limits refuses what the kernel would leave out: too many PSF
+    Gaussians per source, a radius bin outside the table, too many sources
+    or radii; a packed synthetic patch passes."""
+    import copy
+    from forcepho_amd import scenes
+    from forcepho_amd.patches import SimplePatch
+    import oracle
+    spec = scenes.spec_single(3)
+    patch, scene, theta = scenes.build_patch(spec, SimplePatch, oracle.render_model)
+    patch.check_limits()
+    bad = copy.deepcopy(patch)
+    bad.psfgauss["sersic_bin"][int(bad.psfgauss_start[0])] = bad.n_radii
+    with np.testing.assert_raises_regex(ValueError, "radius bins"):
+        bad.check_limits()
+    bad = copy.deepcopy(patch)
+    bad.psfgauss["sersic_bin"][int(bad.psfgauss_start[0])] = -1
+    with np.testing.assert_raises_regex(ValueError, "radius bins"):
+        bad.check_limits()
+    bad = copy.deepcopy(patch)
+    bad.n_psf_per_source = np.array([kernel_limits.FPX_MAX_PSF_PER_SOURCE + 1], np.int32)
+    with np.testing.assert_raises_regex(ValueError, "n_psf_per_source"):
+        bad.check_limits()
+    bad = copy.deepcopy(patch)
+    bad.n_sources = kernel_limits.MAXSOURCES + 1
+    with np.testing.assert_raises_regex(ValueError, "n_sources"):
+        bad.check_limits()
+    bad = copy.deepcopy(patch)
+    bad.n_radii = kernel_limits.MAXRADII + 1
+    with np.testing.assert_raises_regex(ValueError, "n_radii"):
+        bad.check_limits()

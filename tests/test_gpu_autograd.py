@@ -60,16 +60,20 @@ def test_autograd_matches_kernel_gradient_and_fp64():
     assert np.all(np.abs(g_auto - g64s) <= tol)
     assert np.all(np.abs(g_kernel - g64s) <= tol)
     assert np.all(np.abs(g_fused - g64s) <= tol)
-    # the reference's own CPU kernel on the same proposal (where it is built):
-    # inside the same band, and the autograd gradients within both errors of it
+    # the reference's own CPU kernel on the same proposal (where it is built)
     mod = oracle.reference_module()
     if mod is not None:
         c_ref, d_ref, _ = oracle.evaluate_reference(model.patch, model.proposal(th), module=mod)
         g_ref = stack_grad(d_ref.astype(np.float64), model.n_sources, model.n_bands)
         np.testing.assert_allclose(fused.item(), c_ref.sum(), rtol=1e-6)
         assert np.all(np.abs(g_ref - g64s) <= tol)
-        assert np.all(np.abs(g_auto - g_ref) <= 2 * tol)
-        assert np.all(np.abs(g_fused - g_ref) <= 2 * tol)
+        # directly against the reference: every entry within its float64
+        # tolerance, and within 1e-4 of the largest |gradient| (one scale for
+        # the stacked vector, whose band fluxes and shared shape parameters
+        # are all in it)
+        for g in (g_auto, g_fused):
+            assert np.all(np.abs(g - g_ref) <= tol)
+            assert np.abs(g - g_ref).max() <= U.GRAD_BANDMAX_RTOL * np.abs(g_ref).max()
 
 
 def test_autograd_central_differences_full_size():

@@ -1,11 +1,14 @@
 """The C ABI bound the way a forcepho maintainer would bind it: ctypes over
 libfpx.so and the HIP runtime, device memory from hipMalloc, no torch.  The
 patch arrays and the Patch struct are uploaded as forcepho's GPUPatchMixin
-uploads them (forcepho/patches/device_patch.py:257-341), the tile table
-comes from fpx_plan_tiles, and each evaluation is one fpx_evaluate_sync call
-(the reference's one PyCUDA call, forcepho/proposal.py:133-136).  Results
-are checked against the golden fixture made by the reference's CPU kernel
-and against this package's own HIPProposer on the same patch."""
+uploads them (forcepho/patches/device_patch.py:257-341).  The bound-patch
+entry points (fpx_bind once per packed patch, then one fpx_evaluate_host
+per evaluation: the reference's one PyCUDA call, forcepho/proposal.py:
+133-136) are the binding INTEGRATION.md shows; the lower-level slot entry
+(tile table from fpx_plan_tiles, one fpx_evaluate_sync per evaluation) is
+checked too.  Results are checked against the golden fixture made by the
+reference's CPU kernel and against this package's own HIPProposer on the
+same patch."""
 
 import ctypes
 
@@ -106,7 +109,7 @@ def _evaluate_raw(dev, patch, proposal, cfg=5):
     n_out = 8 * MAXBANDS + response_struct_dtype.itemsize * nb
     max_gauss = ns * int(np.max(patch.n_psf_per_source))
     rc = L.fpx_evaluate_sync(d_slot, n_tiles, max_gauss, cfg, 0, None, d_prop, prop.ctypes.data,
-                             prop.nbytes, d_out, out.ctypes.data, n_out, None)
+                             prop.nbytes, d_out, out.ctypes.data, n_out, None, None)
     assert rc == 0, L.fpx_last_error()
     chi2 = out[:8 * nb].view(np.float64).copy()
     derivs = out[8 * MAXBANDS:n_out].view(np.float32).reshape(nb, MAXSOURCES, NPARAMS)[:, :ns]
@@ -139,3 +142,89 @@ def test_raw_ctypes_binding_matches_fixture_and_host_layer(name):
     np.testing.assert_allclose(chi2, c, rtol=1e-12)
     np.testing.assert_allclose(derivs, d, rtol=1e-6, atol=1e-6 * np.abs(d).max())
     np.testing.assert_array_equal(resid, r)
+
+
+def _evaluate_bound(dev, patch, proposal, cfg=-1):
+    """The maintainer's binding: fpx_bind on the uploaded struct, then
+    fpx_evaluate_host per call."""
+    L = _lib.lib()
+    d_patch, ptrs = _upload_patch(dev, patch)
+    ev = ctypes.c_void_p()
+    rc = L.fpx_bind(d_patch, int(patch.n_bands), cfg, 1, ctypes.byref(ev))
+    assert rc == 0, L.fpx_last_error()
+    try:
+        nb, ns = int(patch.n_bands), int(patch.n_sources)
+        prop = np.ascontiguousarray(proposal, dtype=source_struct_dtype)
+        chi2 = np.zeros(nb)
+        resp = np.zeros(nb, dtype=response_struct_dtype)
+        out = []
+        for _ in range(2):
+            rc = L.fpx_evaluate_host(ev, prop.ctypes.data, ns, chi2.ctypes.data, resp.ctypes.data,
+                                     None)
+            assert rc == 0, L.fpx_last_error()
+            resid = np.zeros(patch.npix, dtype=np.float32)
+            dev.d2h(resid, ptrs["residual"])
+            derivs = resp["dchi2_dparam"].reshape(nb, MAXSOURCES, NPARAMS)[:, :ns]
+            out.append((chi2.copy(), derivs.astype(np.float64), resid))
+        # the asynchronous form with device buffers, then the status check
+        d_prop = dev.malloc(prop.nbytes)
+        dev.h2d(d_prop, prop)
+        d_chi2 = dev.malloc(8 * nb)
+        d_resp = dev.malloc(response_struct_dtype.itemsize * nb)
+        assert L.fpx_evaluate(ev, d_prop, d_chi2, d_resp, None) == 0, L.fpx_last_error()
+        assert L.fpx_evaluator_status(ev, None) == 0, L.fpx_last_error()
+        c_async = np.zeros(nb)
+        dev.d2h(c_async, d_chi2)
+        r_async = np.zeros(nb, dtype=response_struct_dtype)
+        dev.d2h(r_async, d_resp)
+        out.append((c_async, r_async["dchi2_dparam"].reshape(nb, MAXSOURCES, NPARAMS)[:, :ns]
+                    .astype(np.float64), None))
+        n_tiles, used_cfg, max_gauss = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        assert L.fpx_evaluator_info(ev, ctypes.byref(n_tiles), ctypes.byref(used_cfg),
+                                    ctypes.byref(max_gauss)) == 0
+        assert max_gauss.value == ns * int(np.max(patch.n_psf_per_source))
+        assert n_tiles.value > 0
+    finally:
+        L.fpx_release(ev)
+    return out
+
+
+@pytest.mark.parametrize("name", ["multi_small_s2", "padding_s5", "max_sources_s3"])
+def test_bound_patch_binding_matches_fixture(name):
+    """fpx_bind + fpx_evaluate_host through raw ctypes against the reference
+    CPU kernel's fixture: chi2 to 1e-6, residual to 2e-6 of the data scale,
+    gradients within the float64 band; the asynchronous fpx_evaluate gives
+    the same values."""
+    print(f"[gpu] bound-patch binding on {name}", flush=True)
+    patch, proposal, expected = U.load_golden(name, return_residual=True)
+    dev = RawDevice()
+    try:
+        (c1, d1, r1), (c2, d2, r2), (c3, d3, _) = _evaluate_bound(dev, patch, proposal)
+    finally:
+        dev.free()
+    rel = np.abs(c1 - expected["chi2"]) / np.abs(expected["chi2"])
+    assert rel.max() <= U.CHI2_RTOL
+    dscale = np.max(np.abs(patch.data)) + 1e-30
+    assert np.max(np.abs(r1 - expected["residual"])) <= 2e-6 * dscale
+    _, g64, mass, smass = U.fp64_reference(patch, proposal)
+    assert U.grad_errors(d1, g64, mass, smass).max() <= 1.0
+    np.testing.assert_allclose(c2, c1, rtol=1e-12)
+    np.testing.assert_array_equal(r2, r1)
+    np.testing.assert_allclose(c3, c1, rtol=1e-12)
+    np.testing.assert_allclose(d3, d1, rtol=1e-5, atol=1e-6 * np.abs(d1).max())
+
+
+def test_bind_refuses_out_of_range_radius_bin():
+    """fpx_bind reads the packed metadata back and refuses a PSF entry whose
+    radius bin lies outside the patch's table, before any kernel runs."""
+    patch, proposal, expected = U.load_golden("single_s1", return_residual=True)
+    patch.psfgauss["sersic_bin"][int(patch.psfgauss_start[0])] = int(patch.n_radii)
+    L = _lib.lib()
+    dev = RawDevice()
+    try:
+        d_patch, _ = _upload_patch(dev, patch)
+        ev = ctypes.c_void_p()
+        assert L.fpx_bind(d_patch, int(patch.n_bands), -1, 1, ctypes.byref(ev)) == -1
+        assert b"radius_bin" in L.fpx_last_error() and ev.value is None
+    finally:
+        dev.free()

@@ -314,3 +314,118 @@ def test_metropolis_kernel_matches_torch():
     np.testing.assert_array_equal(got, want)
     expect_z = torch.where(torch.tensor(want, device="cuda")[:, None], z_in, z0)
     np.testing.assert_array_equal(z.cpu().numpy(), expect_z.cpu().numpy())
+
+
+def test_full_size_ensemble_every_slot_and_energy():
+    """The HMC ensemble workload at its stated size: the benchmark's 64
+    uneven patches (seed 0 + 2; 1-20 sources, 1-8 bands x 1-4 exposures,
+    48-256 px stamps) x 64 chains = 4096 slots, per-chain step sizes found
+    and adapted as the benchmark does, then one graph-captured 32-step
+    leapfrog trajectory of every chain replayed once (littlemcmc's per-chain
+    loop over FastPosterior.evaluate, forcepho/fitting.py:254-341 and
+    forcepho/model.py:302-387, as one device-resident ensemble).
+
+    Checks: (1) the trajectory's recorded end-point potential equals a fresh
+    evaluation at the end positions, and the energy errors H1 - H0 are finite
+    with a bounded distribution; (2) at the end positions every one of the
+    4096 slots' batched chi2 and gradient equals that slot's records
+    evaluated alone with the same tile shape (chi2 to 1e-7 relative,
+    gradients to 1e-3 of the slot's largest entry: the bounds of the
+    512-patch batch test); (3) on the five patches with the fewest pairs,
+    the first and last chain against the C restatement (chi2) and the
+    float64 oracle (gradients)."""
+    import time
+    import torch
+    from forcepho_amd.ensemble import HMCEnsemble
+    from forcepho_amd.kernel_limits import MAXSOURCES
+    from forcepho_amd.patches import SimplePatch
+    from forcepho_amd.proposal import HIPProposer
+    from forcepho_amd.sources import source_struct_dtype
+    t0 = time.time()
+    rng = np.random.default_rng(2)
+    pats, scs, bnds, thetas = [], [], [], []
+    for _ in range(64):
+        patch, scene, theta = scenes.build_patch(scenes.spec_uneven(rng), SimplePatch,
+                                                 scenes.render_on_device)
+        patch.return_residual = False
+        patch.send_to_device()
+        pats.append(patch)
+        scs.append(scene)
+        thetas.append(scene.get_all_source_params().copy())
+        bnds.append(_bounds(thetas[-1], int(patch.n_bands)))
+    ens = HMCEnsemble(pats, scs, 64, bnds, config=6, persistent=True)
+    assert ens.n_slots == 4096
+    start = []
+    for pi, c in ens.batch.slot_map:
+        t = thetas[pi].copy()
+        nb = int(pats[pi].n_bands)
+        t.reshape(-1, nb + 6)[:, nb + 2] *= 0.999 - 0.0001 * c    # near the truth
+        start.append(t)
+    ens.set_theta(start)
+    gen = torch.Generator(device=ens.device)
+    gen.manual_seed(0)
+    print(f"[gpu] ensemble of {ens.n_slots} chains built ({time.time() - t0:.1f} s)", flush=True)
+    # per-chain step sizes: the starting heuristic, two 8-step rounds, then
+    # (with the recorded graph) two rounds of the measured 32 steps
+    ens.eps.fill_(0.005)
+    eps = ens.find_step_size(generator=gen).cpu().numpy()
+    ens.adapt_step_size(2, 8, generator=gen)
+    ens.capture(32)
+    print(f"[gpu] trajectory recorded ({time.time() - t0:.1f} s)", flush=True)
+    ens.adapt_step_size(2, 32, generator=gen, use_graph=True)
+    print("[gpu] step sizes: heuristic quantiles (0, 50, 100%)",
+          np.percentile(eps, [0, 50, 100]), "adapted",
+          np.percentile(ens.eps.view(-1).cpu().numpy(), [0, 50, 100]),
+          f"({time.time() - t0:.1f} s)", flush=True)
+    ens.p.copy_(torch.randn(ens.p.shape, generator=gen, dtype=torch.float64,
+                            device=ens.device) * ens.fmask)
+    p0 = ens.p.clone()
+    U0 = ens.potential()[0].clone()
+    ens.graph.replay()
+    torch.cuda.synchronize()
+    print(f"[gpu] trajectory replayed ({time.time() - t0:.1f} s)", flush=True)
+    U_start, U_end, p_end = ens.U_start.clone(), ens.U.clone(), ens.p.clone()
+    np.testing.assert_allclose(U_start.cpu().numpy(), U0.cpu().numpy(), rtol=1e-10)
+    U_fresh = ens.potential()[0]                       # batched launch at the end positions
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(U_end.cpu().numpy(), U_fresh.cpu().numpy(), rtol=1e-10)
+    H0 = (U0 + 0.5 * (p0 * p0).sum(dim=(1, 2))).cpu().numpy()
+    H1 = (U_end + 0.5 * (p_end * p_end).sum(dim=(1, 2))).cpu().numpy()
+    dH = H1 - H0
+    q = [50, 90, 99, 100]
+    print("[gpu] |H1 - H0| quantiles", q, np.percentile(np.abs(dH), q),
+          "mean acceptance", float(np.mean(np.minimum(1.0, np.exp(-dH)))), flush=True)
+
+    # every slot against its own launch with the same tile shape
+    recs = ens.batch.proposals.cpu().numpy().view(source_struct_dtype).reshape(-1, MAXSOURCES)
+    batched = ens.batch.results()
+    plain = HIPProposer(config=ens.batch.config, persistent=ens.batch.persistent)
+    chi2_dev, grad_dev = np.zeros(ens.n_slots), np.zeros(ens.n_slots)
+    for s, (pi, _) in enumerate(ens.batch.slot_map):
+        ns = int(pats[pi].n_sources)
+        c, d = plain.evaluate_proposal(recs[s, :ns].copy(), patch=pats[pi])
+        cb, db = batched[s]
+        chi2_dev[s] = np.max(np.abs(cb - c) / np.abs(c))
+        grad_dev[s] = np.max(np.abs(db - d)) / np.abs(d).max()
+    print(f"[gpu] 4096 slots vs alone: chi2 {chi2_dev.max():.2e}, gradients {grad_dev.max():.2e} "
+          f"({time.time() - t0:.1f} s)", flush=True)
+    assert chi2_dev.max() <= 1e-7 and grad_dev.max() <= 1e-3
+
+    # the smallest patches against the checkers
+    sizes = np.array([p.n_evals() for p in pats])
+    for pi in np.argsort(sizes)[:5]:
+        for c in (0, 63):
+            s = ens.batch.slot_map.index((int(pi), c))
+            ns = int(pats[pi].n_sources)
+            prop = recs[s, :ns].copy()
+            c32, _, _ = oracle.evaluate_cpu(pats[pi], prop)
+            assert np.max(np.abs(batched[s][0] - c32) / np.abs(c32)) <= U.CHI2_RTOL, (pi, c)
+            _, g64, mass, smass = U.fp64_reference(pats[pi], prop)
+            assert U.grad_errors(batched[s][1], g64, mass, smass).max() <= 1.0, (pi, c)
+    print(f"[gpu] smallest patches vs checkers done ({time.time() - t0:.1f} s)", flush=True)
+    # the energy error distribution of the trajectory near the truth with
+    # adapted step sizes: almost every chain finite, most errors small
+    finite = np.isfinite(dH)
+    assert finite.mean() >= 0.99
+    assert np.median(np.abs(dH[finite])) <= 1.0
+    assert np.mean(np.where(finite, np.minimum(1.0, np.exp(-np.clip(dH, -50, None))), 0)) >= 0.6

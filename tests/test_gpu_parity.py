@@ -43,11 +43,15 @@ def test_fixture_parity(name, fp64_cache):
     c64, g64, mass, smass = _fp64(name, patch, proposal, fp64_cache)
     err = U.grad_errors(dchi2, g64, mass, smass)
     assert err.max() <= 1.0, (name, err.max(), np.unravel_index(err.argmax(), err.shape))
-    # and against the reference CPU kernel, whose own float32 error is included
-    err_ref = U.grad_errors(expected["dchi2"], g64, mass, smass)
-    scale = U.GRAD_RTOL * np.abs(g64) + U.GRAD_FLOOR * mass + U.MODEL_EPS * smass
-    both = np.abs(dchi2 - expected["dchi2"]) / np.where(scale > 0, scale, 1e-300)
-    assert np.all(both <= err + err_ref + 1e-9), name
+    # and against the reference CPU kernel directly: within 1e-4 of each
+    # band's largest gradient, and every entry within its float64 tolerance
+    # band (the bound each kernel meets against float64, not their sum)
+    rel_ref = U.bandmax_rel(dchi2, expected["dchi2"])
+    direct = U.grad_errors(dchi2, expected["dchi2"], mass, smass)
+    print(f"{name}: vs reference max {rel_ref.max():.2e} of the band's largest, "
+          f"{direct.max():.3f} of the tolerance; vs float64 {err.max():.3f}", flush=True)
+    assert rel_ref.max() <= U.GRAD_BANDMAX_RTOL, (name, rel_ref.max())
+    assert direct.max() <= 1.0, (name, direct.max())
     # residual image against the reference, relative to the data scale
     dscale = np.max(np.abs(patch.data)) + 1e-30
     assert np.max(np.abs(resid - expected["residual"])) <= 2e-6 * dscale, name
@@ -233,6 +237,28 @@ def test_full_size_multi_band_against_fp64():
     err = U.grad_errors(d, ref["dchi2"], ref["mass"], ref["smass"])
     print("full-size gradient error / tolerance: max", err.max(), "median", np.median(err))
     assert err.max() <= 1.0
+    # the reference's own CPU kernel on the same patch and proposal: the GPU
+    # within 1e-4 of each band's largest gradient of it, and the GPU's error
+    # tail against float64 no worse than twice the reference kernel's own
+    mod = oracle.reference_module()
+    if mod is None:
+        pytest.skip("the reference CPU kernel is not built (oracle/_ref)")
+    c_ref, d_ref, _ = oracle.evaluate_reference(patch, proposal, module=mod)
+    d_ref = d_ref.astype(np.float64)
+    g64 = ref["dchi2"]
+    assert U.bandmax_rel(d, d_ref).max() <= U.GRAD_BANDMAX_RTOL
+    rel = lambda g: np.abs(g - g64) / np.where(g64 != 0, np.abs(g64), 1e-300)
+    q = [50, 99]
+    gpu_tail, ref_tail = np.percentile(rel(d), q), np.percentile(rel(d_ref), q)
+    gpu_band, ref_band = U.bandmax_rel(d, g64).max(), U.bandmax_rel(d_ref, g64).max()
+    gpu_tol = err.max()
+    ref_tol = U.grad_errors(d_ref, g64, ref["mass"], ref["smass"]).max()
+    print(f"per-entry relative error p50/p99: GPU {gpu_tail}, reference {ref_tail}; "
+          f"of the band's largest: GPU {gpu_band:.2e}, reference {ref_band:.2e}; "
+          f"of the tolerance: GPU {gpu_tol:.3f}, reference {ref_tol:.3f}", flush=True)
+    assert gpu_tail[1] <= 2.0 * ref_tail[1]
+    assert gpu_band <= 2.0 * ref_band
+    assert gpu_tol <= max(2.0 * ref_tol, 0.5)
 
 
 @pytest.mark.parametrize("name", ["multi_small_s2", "padding_s5"])
@@ -263,3 +289,70 @@ def test_pixel_order_independence(name):
     back = np.empty_like(r1)
     back[perm] = r1
     assert np.abs(back - r0).max() <= 2e-6 * np.abs(patch.data).max()
+
+
+@pytest.mark.parametrize("persistent", [True, False])
+def test_too_small_max_gauss_is_an_error(persistent):
+    """A max_gauss below n_sources x n_psf_per_source makes the kernel leave
+    every tile out: fpx_evaluate_sync returns FPX_E_ARG and the chi2 it
+    copied back is NaN, not a chi2 missing tiles.  The status word is
+    cleared, so the next evaluation is right again."""
+    import torch
+    from forcepho_amd._lib import lib
+    from forcepho_amd.proposal import HIPProposer, _CHI2_END
+    print(f"[gpu] too-small max_gauss, persistent {persistent}", flush=True)
+    patch, proposal, expected = U.load_golden("multi_small_s2", return_residual=False)
+    patch.send_to_device()
+    prop = HIPProposer(patch=patch, persistent=persistent)
+    chi2, d = prop.evaluate_proposal(proposal, patch=patch)
+    n_tiles = patch.device_tiles(prop.pix_per_tile)[1]
+    nb = int(patch.n_bands)
+    raw = np.ascontiguousarray(proposal).view(np.uint8)
+    L = lib()
+    rc = L.fpx_evaluate_sync(prop._slot.data_ptr(), n_tiles, patch.max_gauss() - 1, prop.config,
+                             int(persistent), prop._queue.data_ptr(), prop._prop.data_ptr(),
+                             raw.ctypes.data, raw.size, prop._out.data_ptr(),
+                             prop._out_h.data_ptr(), _CHI2_END, prop._out.data_ptr(),
+                             torch.cuda.current_stream().cuda_stream)
+    assert rc == -1, rc
+    assert b"size limit" in L.fpx_last_error()
+    assert np.all(np.isnan(prop._chi2_np[:nb]))
+    assert int(prop._status.cpu().item()) == 0          # cleared by the call
+    again, d2 = prop.evaluate_proposal(proposal, patch=patch)
+    np.testing.assert_allclose(again, chi2, rtol=1e-12)
+    rel = np.abs(again - expected["chi2"]) / np.abs(expected["chi2"])
+    assert rel.max() <= U.CHI2_RTOL
+
+
+def test_device_radius_bin_out_of_range_is_an_error():
+    """A radius bin outside [0, n_radii) in the device copy of the PSF block
+    (the host copy, which check_limits reads, is valid) is caught by the
+    kernel's own check: the Gaussian is left out, the status word set, and
+    evaluate_proposal raises; batched slots report it the same way."""
+    import torch
+    from forcepho_amd._lib import FpxError
+    from forcepho_amd.batch import BatchEvaluator
+    from forcepho_amd.proposal import HIPProposer
+    print("[gpu] device radius bin out of range", flush=True)
+    patch, proposal, expected = U.load_golden("single_s1", return_residual=False)
+    patch.send_to_device()
+    prop = HIPProposer(patch=patch)
+    good, _ = prop.evaluate_proposal(proposal, patch=patch)
+    raw = patch.device_ptrs["psfgauss"]
+    off = int(patch.psfgauss_start[0]) * 28 + 24        # sersic_radius_bin of the first entry
+    saved = raw[off:off + 4].clone()
+    raw[off:off + 4].copy_(torch.tensor([int(patch.n_radii)], dtype=torch.int32,
+                                        device=raw.device).view(torch.uint8))
+    with pytest.raises(FpxError, match="left tiles out"):
+        prop.evaluate_proposal(proposal, patch=patch)
+    ev = BatchEvaluator([patch], chains=2)
+    ev.set_proposals([proposal, proposal])
+    ev.launch()
+    with pytest.raises(FpxError, match="slots"):
+        ev.results()
+    raw[off:off + 4].copy_(saved)
+    again, _ = prop.evaluate_proposal(proposal, patch=patch)
+    np.testing.assert_allclose(again, good, rtol=1e-12)
+    ev.launch()
+    (c0, _), (c1, _) = ev.results()
+    np.testing.assert_allclose(c0, good, rtol=1e-12)
