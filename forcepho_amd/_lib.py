@@ -73,6 +73,8 @@ _SIGNATURES = {
     "fpx_evaluator_status": (_c_int, [_p, _p]),
     "fpx_evaluator_info": (_c_int, [_p, _p, _p, _p]),
     "fpx_release": (_c_int, [_p]),
+    "fpx_evaluate_static": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _p, _p, _p, _c_int,
+                                     _c_int, _p]),
     "fpx_evaluate_batch": (_c_int, [_p, _c_int, _p, _c_int, _c_int, _c_int, _c_int, _p, _p]),
     "fpx_evaluate_tiles": (_c_int, [_p, _c_int, _p, _c_int, _c_int, _c_int, _c_int, _p, _p]),
     "fpx_finalize": (_c_int, [_p, _c_int, _p]),

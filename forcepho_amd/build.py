@@ -16,7 +16,7 @@ INCLUDE_DIR = os.path.join(REPO_DIR, "include")
 CSRC_DIR = os.path.join(PKG_DIR, "csrc")
 LIB_PATH = os.path.join(PKG_DIR, "libfpx.so")
 
-SOURCES = ["fpx_kernel.hip", "fpx_records.hip", "fpx_evaluator.hip"]
+SOURCES = ["fpx_kernel.hip", "fpx_records.hip", "fpx_evaluator.hip", "fpx_static.hip"]
 HEADERS = ["fpx_gauss.h", "fpx_internal.h"]
 
 # Packed FP32 (v_pk_*_f32) is disabled for all device code: the compiler

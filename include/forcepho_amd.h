@@ -349,6 +349,30 @@ int fpx_evaluator_info(const FpxEvaluator *ev, int32_t *n_tiles, int32_t *cfg,
 /* Free an evaluator (after its last evaluation has completed). */
 int fpx_release(FpxEvaluator *ev);
 
+/* Bytes of the static device array fpx_evaluate_static works in: a status
+ * word, the queue counter and the FP64 accumulators of MAXBANDS bands. */
+#define FPX_STATIC_SCRATCH_BYTES (256 + MAXBANDS * (1 + MAXSOURCES * NPARAMS) * 8)
+
+/* fpx_evaluate with no allocation at all.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The caller
+ * passes the tile table (fpx_plan_tiles for configuration cfg, copied to
+ * device memory it owns) and the outputs; the accumulator, queue and status
+ * word live in a static device array of FPX_STATIC_SCRATCH_BYTES that every
+ * GPU the library runs on holds until the process ends, cleared by a fill
+ * kernel at the start of every call.
+ *   n_bands   : at most MAXBANDS (larger is refused with FPX_E_ARG)
+ *   max_gauss : n_sources x the largest n_psf_per_source
+ *   cfg       : -1 for the library default
+ * Asynchronous on stream, like fpx_evaluate; a tile that breaks a limit is
+ * left out and pchi2 is written as NaN.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (other streams, other threads)
+ * race and must not be issued. */
+int fpx_evaluate_static(const FpxPatch *patch_dev, const FpxTile *tiles_dev, int32_t n_tiles,
+                        int32_t n_bands, int32_t max_gauss, const FpxSource *proposal_dev,
+                        double *pchi2_dev, FpxResponse *pdchi2_dp_dev, int32_t cfg,
+                        int32_t persistent, void *stream);
+
 /* Many evaluations in one launch (uneven patches, many chains).
  *   slots        : device FpxSlot[n_slots]
  *   item_offsets : device int32[n_slots + 1], prefix sums of the slots'
