@@ -346,6 +346,7 @@ def ensemble_measurement(args):
     ens.capture(args.leapfrog_steps)
     capture_s = time.perf_counter() - t2
     ens.adapt_step_size(2, args.leapfrog_steps, generator=gen, use_graph=True)
+    torch.cuda.synchronize()
     adapt_s = time.perf_counter() - t1 - capture_s
     p0 = torch.randn(ens.p.shape, generator=gen, dtype=torch.float64,
                      device=ens.device) * ens.fmask

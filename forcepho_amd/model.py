@@ -300,4 +300,6 @@ class FastPosterior(Posterior):
         return self._residuals
 
 
-GPUPosterior = FastPosterior
+class GPUPosterior(FastPosterior):
+    """forcepho's name for FastPosterior, kept for backwards compatibility
+    (forcepho/model.py:408-411)."""

@@ -256,28 +256,33 @@ class SuperScene:
         self.n_fixed = 0
 
     def get_circular_scene(self, center):
-        """(radius arcsec, active indices, fixed indices) of the sources about
-        a scene-coordinate center (superscene.py:372-440): up to maxactive
-        sources whose distance + nscale rhalf lies inside maxradius, nearest
-        first; fixed are the next sources reaching into the radius."""
-        kinds = np.array(self.kdt.query_ball_point(center, self.boundary_radius), dtype=int)
-        if len(kinds) == 0 or np.any(self.sourcecat["is_active"][kinds]):
+        """(radius arcsec, active indices, fixed indices) of a circular scene
+        about ``center`` (scene coordinates, arcsec) (superscene.py:372-440).
+
+        A source reaches ``nscale`` half-light radii beyond its distance from
+        the centre.  The sources that reach least far, as many as reach less
+        than ``maxradius`` and at most ``maxactive``, are active; the radius
+        is the furthest of their reaches, at least ``minradius``.  Of the
+        others, those whose near edge (distance less ``nscale`` half-light
+        radii) lies inside the radius are fixed, at most ``maxactive`` and at
+        least the nearest-reaching one.  All three are None when a source
+        within ``boundary_radius`` is active elsewhere or none fits."""
+        near = np.asarray(self.kdt.query_ball_point(center, self.boundary_radius), dtype=int)
+        if near.size == 0 or self.sourcecat["is_active"][near].any():
             return None, None, None
-        rhalf = self.sourcecat["rhalf"][kinds]
-        distance = np.hypot(*(self.scene_coordinates[kinds] - center).T)
-        outer = distance + self.nscale * rhalf
-        inner = distance - self.nscale * rhalf
-        order = np.argsort(outer)
-        n_active = min(self.maxactive, int((outer < self.maxradius).sum()))
-        if n_active == 0:
+        sep = np.hypot(*(self.scene_coordinates[near] - center).T)
+        halo = self.nscale * self.sourcecat["rhalf"][near]
+        rank = np.argsort(sep + halo)
+        reach, edge = (sep + halo)[rank], (sep - halo)[rank]
+        k = min(self.maxactive, int(np.count_nonzero(reach < self.maxradius)))
+        if k == 0:
             return None, None, None
-        active = order[:n_active]
-        rest = order[n_active:]
-        radius = max(self.minradius, outer[active].max())
-        fixed = rest[inner[rest] < radius][:min(self.maxactive, len(rest))]
-        if len(fixed) == 0:
+        radius = max(self.minradius, reach[k - 1])
+        rest = np.arange(k, len(rank))
+        fixed = rest[edge[k:] < radius][:self.maxactive]
+        if fixed.size == 0:
             fixed = rest[:1]
-        return radius, kinds[active], kinds[fixed]
+        return radius, near[rank[:k]], near[rank[fixed]]
 
     def draw_center(self, seed_index=-1):
         """(ra, dec) of a seed source: the given one, or a random draw with
@@ -363,38 +368,48 @@ class LinkedSuperScene(SuperScene):
 
     def grow_scene(self, seed_index, max_fixed=None):
         """(region, active indices, fixed indices) of a scene grown from a
-        seed by whole groups in order of distance (superscene.py:567-626);
-        (seed position, None, None) when a nearby source is active."""
+        seed (superscene.py:567-626); (seed position, None, None) when a
+        source near the seed, or one the growth would take, is active
+        elsewhere.
+
+        Whole friends-of-friends groups join in order of their nearest
+        member's distance from the seed until one does not fit into
+        ``maxactive``: that group then fills the remaining room if the scene
+        is still empty or the scene is not ``strict``, and the growth stops.
+        The region is centred on the mean position of the active sources and
+        reaches ``buffer`` beyond the furthest radius of influence (at least
+        ``minradius``); fixed sources are the nearest others within 1.2
+        times the region radius, at most ``max_fixed`` (default
+        ``maxactive``)."""
         seed = self.original[seed_index]
-        center = self.scene_coordinates[seed_index]
-        candidates = self.find_overlaps(center, self.boundary_radius, sort=True)
-        if np.any(self.sourcecat["is_active"][candidates]):
-            return (seed["ra"], seed["dec"]), None, None
-        active_inds = []
-        for s in candidates:
-            if s in active_inds:
+        busy = ((seed["ra"], seed["dec"]), None, None)
+        near = self.find_overlaps(self.scene_coordinates[seed_index], self.boundary_radius,
+                                  sort=True)
+        if self.sourcecat["is_active"][near].any():
+            return busy
+        chosen, taken = [], set()
+        for s in near:
+            if s in taken:
                 continue
-            n_avail = self.maxactive - len(active_inds)
-            to_add = [m for m in self.grow_source(s) if m not in active_inds]
-            if len(to_add) > n_avail:
-                if (not self.strict) or not active_inds:
-                    active_inds += to_add[:n_avail]
+            group = [int(m) for m in self.grow_source(s) if m not in taken]
+            room = self.maxactive - len(chosen)
+            if len(group) > room:
+                if not (self.strict and chosen):
+                    chosen += group[:room]
                 break
-            active_inds += to_add
-        # a group can reach past the boundary radius; never hand out a source
-        # that another region holds active
-        if np.any(self.sourcecat["is_active"][active_inds]):
-            return (seed["ra"], seed["dec"]), None, None
-        active = self.sourcecat[active_inds]
-        cra, cdec = float(np.mean(active["ra"])), float(np.mean(active["dec"]))
-        center = np.array(self.sky_to_scene(cra, cdec))
-        d = self.scene_coordinates[active_inds] - center
-        radius = max(self.minradius, np.max(np.hypot(*d.T) + self.roi[active_inds] + self.buffer))
-        region = CircularRegion(cra, cdec, radius / 3600.0)
-        max_fixed = self.maxactive if max_fixed is None else max_fixed
-        inregion = self.find_overlaps(center, 1.2 * radius, sort=True)
-        fixed_inds = [int(i) for i in inregion if i not in active_inds][:max_fixed]
-        return region, active_inds, fixed_inds
+            chosen += group
+            taken.update(group)
+        if self.sourcecat["is_active"][chosen].any():
+            return busy
+        rows = self.sourcecat[chosen]
+        ra0, dec0 = float(np.mean(rows["ra"])), float(np.mean(rows["dec"]))
+        middle = np.array(self.sky_to_scene(ra0, dec0))
+        spread = np.hypot(*(self.scene_coordinates[chosen] - middle).T)
+        radius = max(self.minradius, float(np.max(spread + self.roi[chosen] + self.buffer)))
+        limit = self.maxactive if max_fixed is None else max_fixed
+        others = [int(i) for i in self.find_overlaps(middle, 1.2 * radius, sort=True)
+                  if i not in chosen]
+        return CircularRegion(ra0, dec0, radius / 3600.0), chosen, others[:limit]
 
     def checkout_region(self, seed_index=-1, max_fixed=None):
         """(region, active rows, fixed rows or None); ((seed position,
@@ -430,78 +445,82 @@ def make_bounds(active, filternames, shapenames=Galaxy.SHAPE_COLS, unccat=None,
                 n_sig_flux=5.0, dpos=None, n_pix=2, pixscale=0.03,
                 sqrtq_range=(0.4, 1.0), pa_range=(-0.6 * np.pi, 0.6 * np.pi),
                 rhalf_range=(0.03, 0.3), sersic_range=(1.0, 5.0)):
-    """(lower, upper) bounds of every parameter of every source as a
-    structured array, one row per source (superscene.py:805-881)."""
-    pm1 = np.array([-1.0, 1.0])
-    if dpos is None:
-        dpos = np.ones(2) * n_pix * pixscale
-    dra = dpos[0] / 3600.0 / np.cos(np.deg2rad(active["dec"]))
-    ddec = np.array([dpos[1] / 3600.0])
-    cols = list(filternames) + list(shapenames)
-    bcat = np.zeros(len(active), dtype=[("source_index", np.int32)] +
-                    [(c, np.float64, (2,)) for c in cols])
-    bcat["source_index"] = active["source_index"]
-    bcat["q"] = sqrtq_range
-    bcat["pa"] = pa_range
-    bcat["sersic"] = sersic_range
-    bcat["rhalf"] = rhalf_range
-    bcat["ra"] = active["ra"][:, None] + pm1[None, :] * dra[:, None]
-    bcat["dec"] = active["dec"][:, None] + pm1[None, :] * ddec[:, None]
+    """Parameter bounds of each source, a structured array with a
+    ``source_index`` field and a (lower, upper) pair per parameter
+    (superscene.py:805-881).
+
+    Positions: the catalog position plus or minus ``dpos`` arcsec (default
+    ``n_pix`` pixels of ``pixscale``; the ra half-width divided by cos dec).
+    Shapes: the same ranges for every source.  Fluxes: plus or minus
+    ``n_sig_flux`` times the uncertainty catalog's value (no less than 5% of
+    the flux) when ``unccat`` is given, else ``flux_bounds``."""
+    n = len(active)
+    half = np.full(2, n_pix * pixscale) if dpos is None else dpos
+    names = list(filternames) + list(shapenames)
+    out = np.zeros(n, dtype=[("source_index", np.int32)] + [(c, np.float64, (2,)) for c in names])
+    out["source_index"] = active["source_index"]
+    ranges = {"q": sqrtq_range, "pa": pa_range, "sersic": sersic_range, "rhalf": rhalf_range}
+    for name, r in ranges.items():
+        out[name] = r
+    widths = {"ra": half[0] / 3600.0 / np.cos(np.deg2rad(active["dec"])),
+              "dec": np.full(n, half[1] / 3600.0)}
+    for name, w in widths.items():
+        out[name] = np.stack([active[name] - w, active[name] + w], axis=-1)
     for b in filternames:
-        if unccat is not None:
-            sigma = np.maximum(unccat[b], np.abs(0.05 * active[b]))
-            lo, hi = active[b] + pm1[:, None] * n_sig_flux * sigma[None, :]
+        f = np.asarray(active[b], dtype=np.float64)
+        if unccat is None:
+            lo, hi = flux_bounds(f, n_sig_flux)
         else:
-            lo, hi = flux_bounds(np.asarray(active[b], dtype=float), n_sig_flux)
-        bcat[b][:, 0] = lo
-        bcat[b][:, 1] = hi
-    return bcat
+            w = n_sig_flux * np.maximum(unccat[b], np.abs(0.05 * f))
+            lo, hi = f - w, f + w
+        out[b] = np.stack([lo, hi], axis=-1)
+    return out
 
 
 def bounds_vectors(bounds_cat, filternames, shapenames=Galaxy.SHAPE_COLS,
                    reference_coordinates=(0.0, 0.0)):
     """Flat lower and upper bound vectors in the Scene parameter order
-    [fluxes, ra, dec, q, pa, sersic, rhalf] per source, positions relative to
+    (fluxes, ra, dec, q, pa, sersic, rhalf per source), positions relative to
     the patch reference coordinates (superscene.py:884-899)."""
-    bcat = bounds_cat.copy()
-    bcat["ra"] -= reference_coordinates[0]
-    bcat["dec"] -= reference_coordinates[1]
-    cols = list(filternames) + list(shapenames)
-    lower = np.array([[row[c][0] for c in cols] for row in bcat]).reshape(-1)
-    upper = np.array([[row[c][1] for c in cols] for row in bcat]).reshape(-1)
-    return lower, upper
+    shift = {"ra": reference_coordinates[0], "dec": reference_coordinates[1]}
+    names = list(filternames) + list(shapenames)
+    table = np.stack([bounds_cat[c] - shift.get(c, 0.0) for c in names], axis=1)
+    return table[..., 0].reshape(-1), table[..., 1].reshape(-1)
 
 
 def check_bounds(sourcecat, boundscat):
-    """Assert every bounded parameter lies strictly inside its bounds."""
-    for c in boundscat.dtype.names:
-        b = boundscat[c][:]
-        if b.ndim != 2 or b.shape[-1] != 2:
+    """Raise AssertionError unless every parameter with a (lower, upper)
+    bound lies strictly between them, row by row (superscene.py:902-919)."""
+    for name in boundscat.dtype.names:
+        pair = np.asarray(boundscat[name])
+        if pair.ndim != 2 or pair.shape[1] != 2:
             continue
-        assert np.all(sourcecat[c] > b[:, 0]), f"An input value is below lower bound in column '{c}'"
-        assert np.all(sourcecat[c] < b[:, 1]), f"An input value is above bound in column '{c}'"
+        value = np.asarray(sourcecat[name])
+        outside = np.flatnonzero(~((value > pair[:, 0]) & (value < pair[:, 1])))
+        if outside.size:
+            raise AssertionError(f"{name}: rows {outside.tolist()} are not strictly inside "
+                                 "their bounds")
 
 
 def flux_bounds(flux, unc1, snr_max=10, precisions=None):
-    """Flux (lower, upper) of each source (superscene.py:922-965): from the
-    flux precision matrix when given (unc1 sigmas, no narrower than
-    flux / snr_max, upper at least unc1 sigma), else the heuristic with
-    noise sqrt(|flux|) unc1 and a floor (unc1 / 2)^2 around zero."""
+    """(lower, upper) flux bounds of each source (superscene.py:922-965).
+
+    With a precision matrix: ``unc1`` standard deviations either side, the
+    deviation no smaller than flux / ``snr_max``, the upper bound at least
+    ``unc1`` deviations.  Without: a half-width of ``unc1`` times the square
+    root of |flux| (as if the noise at flux 1 were ``unc1``); fluxes at or
+    below f0 = (unc1 / 2)^2 get a lower bound of at most -f0, every upper
+    bound is at least f0, and a zero flux (no width) gets lower bound 0 and
+    the largest upper bound of the band."""
     flux = np.asarray(flux, dtype=float)
     if precisions is not None:
-        sigma = np.maximum(np.sqrt(np.diag(np.linalg.pinv(precisions))), flux / snr_max)
-        lo = flux - unc1 * sigma
-        hi = np.maximum(flux + unc1 * sigma, unc1 * sigma)
-        return lo, hi
+        sd = np.maximum(np.sqrt(np.diag(np.linalg.pinv(precisions))), flux / snr_max)
+        return flux - unc1 * sd, np.maximum(flux + unc1 * sd, unc1 * sd)
+    f0 = (unc1 / 2.0) ** 2
     with np.errstate(divide="ignore", invalid="ignore"):
-        snr = np.sqrt(np.abs(flux)) / unc1
-        noise = np.abs(flux) / snr
-    fmin = (unc1 / 2.0) ** 2
-    lo = flux - noise
-    hi = flux + noise
-    small = flux <= fmin
-    lo[small] = np.minimum(lo[small], -fmin)
-    lo[~np.isfinite(lo)] = 0
-    hi = np.maximum(hi, fmin)
-    hi[~np.isfinite(hi)] = np.nanmax(hi)
+        width = np.abs(flux) / (np.sqrt(np.abs(flux)) / unc1)
+    lo = np.where(flux <= f0, np.minimum(flux - width, -f0), flux - width)
+    hi = np.maximum(flux + width, f0)
+    lo = np.where(np.isfinite(lo), lo, 0.0)
+    hi = np.where(np.isfinite(hi), hi, np.nanmax(hi))
     return lo, hi
