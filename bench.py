@@ -81,16 +81,24 @@ def build_workload(name, seed):
 
 
 def valu_probe():
-    """Measured FP32 FMA and v_exp_f32 rates of the device: ops/s over the
-    launch, cycles per wavefront instruction and the shader clock the
-    wavefronts saw (s_memtime against the 100 MHz s_memrealtime)."""
+    """Measured FP32 FMA and v_exp_f32 rates of the device (fpx_probe_valu):
+    8 dependent chains per lane (each wavefront waits on its results: a
+    latency figure) and 32 FMA / 16 exp chains at 8 wavefronts per SIMD (the
+    issue rate the VALU reaches), as operations/s over the launch, wavefront
+    instructions per CU per cycle at MI355X's 2.4 GHz peak clock, and the
+    shader clock the wavefronts saw (s_memtime against the 100 MHz
+    s_memrealtime)."""
     import torch
     from forcepho_amd._lib import lib, check
     import ctypes
-    n_threads = 256 * 256 * 8
-    out = torch.empty(n_threads, dtype=torch.float32, device="cuda")
+    n_cu = ctypes.c_int32()
+    check(lib().fpx_device_info(ctypes.byref(n_cu), None, None), "fpx_device_info")
     res = {}
-    for kind, name, iters in ((0, "fma", 8192), (1, "exp", 2048)):
+    for kind, name, iters, blocks_per_cu in ((0, "fma_chains8", 8192, 8), (1, "exp_chains8", 2048, 8),
+                                             (2, "fma_chains32", 2000, 32),
+                                             (3, "exp_chains16", 400, 32)):
+        n_threads = 256 * blocks_per_cu * n_cu.value
+        out = torch.empty(n_threads, dtype=torch.float32, device="cuda")
         ops = ctypes.c_double()
         s = torch.cuda.current_stream()
         check(lib().fpx_probe_valu(kind, iters, out.data_ptr(), n_threads, ctypes.byref(ops),
@@ -104,9 +112,12 @@ def valu_probe():
         torch.cuda.synchronize()
         v = out.view(-1, 64).cpu().numpy()
         cyc, ref = np.median(v[:, 0]), np.median(v[:, 1])
-        res[name + "_per_s"] = ops.value / (a.elapsed_time(b) * 1e-3)
-        res[name + "_cycles_per_wave_inst"] = float(cyc / (iters * 8))
+        rate = ops.value / (a.elapsed_time(b) * 1e-3)
+        res[name + "_per_s"] = rate
+        res[name + "_wave_inst_per_cu_cycle"] = rate / 64 / n_cu.value / 2.4e9
         res[name + "_clock_ghz"] = float(cyc / ref * 0.1)
+    res["fma_tflops"] = 2 * res["fma_chains32_per_s"] / 1e12
+    res["exp_per_s"] = res["exp_chains16_per_s"]
     return res
 
 

@@ -478,7 +478,7 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
   const int pstart = __builtin_amdgcn_readfirstlane(P->psfgauss_start[e]);
   const float gcal = uniform_f(P->G[e]);
 
-  float xp[PPL], yp[PPL], w2[PPL];
+  float xp[PPL], yp[PPL], w2[PPL], dat[PPL], ier[PPL];
   int pix[PPL];
   bool valid[PPL];
 #pragma unroll
@@ -488,6 +488,10 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
     pix[i] = tile.pix_start + off;
     xp[i] = valid[i] ? P->xpix[pix[i]] : 0.0f;
     yp[i] = valid[i] ? P->ypix[pix[i]] : 0.0f;
+    // chi2 mode: data and weights requested with the coordinates, so that
+    // their latency overlaps the cull, the records and the model pass
+    dat[i] = (valid[i] && !vjp) ? P->data[pix[i]] : 0.0f;
+    ier[i] = (valid[i] && !vjp) ? P->ierr[pix[i]] : 0.0f;
   }
   Box box;
   {
@@ -533,8 +537,8 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
       float model = 0.0f;
 #pragma unroll
       for (int w = 0; w < W; ++w) model += l.part[(w * PPL + i) * 64 + lane];
-      const float d = valid[i] ? P->data[pix[i]] : 0.0f;
-      const float ie = valid[i] ? P->ierr[pix[i]] : 0.0f;
+      const float d = dat[i];
+      const float ie = ier[i];
       const float r = d - model;
       if (wave == 0 && valid[i] && resid != nullptr) resid[pix[i]] = r;
       const float ri = r * ie;
@@ -605,26 +609,30 @@ __device__ void finish_tile(const FpxSlot &slot, int band, int n_src, unsigned c
 constexpr int kFinalizeThreads = 256;
 constexpr int kFinalizeChunks = (MAXBANDS * kAcc + kFinalizeThreads - 1) / kFinalizeThreads;
 
+// Output value j of a slot (j < n_bands * kAcc) from its accumulator, which
+// is zeroed.
+__device__ __forceinline__ void slot_output(const FpxSlot &slot, int j) {
+  const double v = __hip_atomic_load(slot.accum + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int b = j / kAcc;
+  const int k = j - b * kAcc;
+  if (k == 0) {
+    const uint32_t st = slot.status == nullptr
+                            ? 0u
+                            : __hip_atomic_load(slot.status, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+    slot.pchi2[b] = st == 0u ? v : __builtin_nan("");
+  } else {
+    slot.pdchi2_dp[b * (MAXSOURCES * NPARAMS) + (k - 1)] = (float)v;
+  }
+  __hip_atomic_store(slot.accum + j, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(
     const FpxSlot *__restrict__ slots, FpxSlot one, int n_slots) {
   const FpxSlot slot = slots != nullptr ? slots[blockIdx.x] : one;
   const int nb = min(slot.n_bands, MAXBANDS);
   const int j = blockIdx.y * kFinalizeThreads + threadIdx.x;
-  if (j < nb * kAcc) {
-    const double v = __hip_atomic_load(slot.accum + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int b = j / kAcc;
-    const int k = j - b * kAcc;
-    if (k == 0) {
-      const uint32_t st = slot.status == nullptr
-                              ? 0u
-                              : __hip_atomic_load(slot.status, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
-      slot.pchi2[b] = st == 0u ? v : __builtin_nan("");
-    } else {
-      slot.pdchi2_dp[b * (MAXSOURCES * NPARAMS) + (k - 1)] = (float)v;
-    }
-    __hip_atomic_store(slot.accum + j, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (j < nb * kAcc) slot_output(slot, j);
 }
 
 template <int W, int PPL>
@@ -663,8 +671,16 @@ __device__ void process_tile(const FpxSlot &slot, int local, int gauss_cap, int 
 // memory; slots is not read).  A compile-time variant rather than a runtime
 // choice, which would keep both slots' fields live (8 more VGPRs at 8x4, one
 // wavefront per SIMD fewer).
+//
+// Shapes of up to 4 pixels per lane and 8 wavefronts are held to 128 VGPRs
+// (4 wavefronts per SIMD), where they fit without spilling; the others get
+// what they need.
+template <int W, int PPL>
+constexpr int kMinWavesPerEU = (PPL <= 4 && W <= 8) ? 4 : 1;
+
 template <int W, int PPL, bool kOne>
-__global__ __launch_bounds__(W * 64) void chi2_grad_kernel(
+__global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(kMinWavesPerEU<W, PPL>)))
+void chi2_grad_kernel(
     const FpxSlot *__restrict__ slots, FpxSlot one, int n_slots,
     const int32_t *__restrict__ item_offsets, int n_items, int gauss_cap, int index_cap,
     int persistent, uint32_t *__restrict__ queue) {
@@ -776,27 +792,28 @@ __global__ __launch_bounds__(256) void census_kernel(
 }
 
 // ------------------------------------------------------------------------
-// VALU throughput probes: KIND 0 runs 8 independent FP32 FMA chains, KIND 1
-// 8 independent v_exp_f32 chains.  Lane 0 of every wavefront stores the
-// elapsed shader-clock cycles (s_memtime) and lane 1 the elapsed 100 MHz
-// reference ticks (s_memrealtime) of its loop; other lanes store the chain
-// sum so the loop cannot be removed.
-// ------------------------------------------------------------------------
-template <int KIND>
+// VALU throughput probes: KIND 0 runs C independent FP32 FMA chains per
+// lane, KIND 1 C independent v_exp_f32 chains.  With few chains a wavefront
+// waits on each result (a latency measurement); with 16-32 chains and 8
+// wavefronts per SIMD the issue rate itself is reached.  Lane 0 of every
+// wavefront stores the elapsed shader-clock cycles (s_memtime) and lane 1 the
+// elapsed 100 MHz reference ticks (s_memrealtime) of its loop; other lanes
+// store the chain sum so the loop cannot be removed.
+template <int KIND, int C>
 __global__ __launch_bounds__(256) void probe_kernel(float *out, int iters) {
-  float a[8];
+  float a[C];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) a[k] = (float)((threadIdx.x + k) & 63) * (1.0f / 64.0f);
+  for (int k = 0; k < C; ++k) a[k] = (float)((threadIdx.x + k) & 63) * (1.0f / 64.0f);
   const float b = 0.999f, c = 1e-4f;
   const unsigned long long t0 = __builtin_amdgcn_s_memtime();
   const unsigned long long r0 = __builtin_amdgcn_s_memrealtime();
   for (int i = 0; i < iters; ++i) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) a[k] = KIND == 0 ? fmaf(a[k], b, c) : __builtin_amdgcn_exp2f(-a[k]);
+    for (int k = 0; k < C; ++k) a[k] = KIND == 0 ? fmaf(a[k], b, c) : __builtin_amdgcn_exp2f(-a[k]);
   }
   float s = 0.f;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) s += a[k];
+  for (int k = 0; k < C; ++k) s += a[k];
   const unsigned long long t1 = __builtin_amdgcn_s_memtime();
   const unsigned long long r1 = __builtin_amdgcn_s_memrealtime();
   const int lane = threadIdx.x & 63;
@@ -1275,15 +1292,24 @@ int fpx_probe_valu(int32_t kind, int32_t iters, float *out, int32_t n_threads, d
   int rc = current_device_state(&st);
   if (rc) return rc;
   const dim3 grid(n_threads / 256), block(256);
-  if (kind == 0) {
-    hipLaunchKernelGGL(fpx::probe_kernel<0>, grid, block, 0, (hipStream_t)stream, out, iters);
-  } else if (kind == 1) {
-    hipLaunchKernelGGL(fpx::probe_kernel<1>, grid, block, 0, (hipStream_t)stream, out, iters);
-  } else {
-    return set_err(FPX_E_ARG, "probe kind must be 0 (fma) or 1 (exp)");
+  const hipStream_t s = (hipStream_t)stream;
+  int chains = 8;
+  switch (kind) {
+    case 0: hipLaunchKernelGGL((fpx::probe_kernel<0, 8>), grid, block, 0, s, out, iters); break;
+    case 1: hipLaunchKernelGGL((fpx::probe_kernel<1, 8>), grid, block, 0, s, out, iters); break;
+    case 2:
+      chains = 32;
+      hipLaunchKernelGGL((fpx::probe_kernel<0, 32>), grid, block, 0, s, out, iters);
+      break;
+    case 3:
+      chains = 16;
+      hipLaunchKernelGGL((fpx::probe_kernel<1, 16>), grid, block, 0, s, out, iters);
+      break;
+    default:
+      return set_err(FPX_E_ARG, "probe kind must be 0 (fma x8), 1 (exp x8), 2 (fma x32) or 3 (exp x16)");
   }
   FPX_HIP(hipGetLastError());
-  if (ops) *ops = (double)n_threads * (double)iters * 8.0;
+  if (ops) *ops = (double)n_threads * (double)iters * chains;
   return FPX_OK;
 }
 

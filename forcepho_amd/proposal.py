@@ -209,15 +209,11 @@ class HIPProposer(ProposerBase):
                                            patch.max_gauss(), self.config, int(self.persistent),
                                            self._queue.data_ptr(), stream.cuda_stream),
                   "fpx_evaluate_tiles")
-        elif self.persistent:
+        else:
             check(lib().fpx_evaluate_batch(self._slot.data_ptr(), 1, None, n_tiles,
-                                           patch.max_gauss(), self.config, 1,
+                                           patch.max_gauss(), self.config, int(self.persistent),
                                            self._queue.data_ptr(), stream.cuda_stream),
                   "fpx_evaluate_batch")
-        else:
-            check(lib().fpx_evaluate_proposal(self._slot.data_ptr(), n_tiles, patch.max_gauss(),
-                                              self.config, stream.cuda_stream),
-                  "fpx_evaluate_proposal")
         return n_tiles
 
     def finalize(self):

@@ -491,11 +491,14 @@ int fpx_hmc_metropolis(const double *U0, const double *U1, const double *p0,
                        void *stream);
 
 /* Throughput probe of the VALU pipes used by the hot loop: kind 0 = FP32
- * FMA, kind 1 = v_exp_f32, 8 independent chains of `iters` steps per lane.
- * Writes one float per thread to out (device, n_threads floats): lane 0 of
- * each wavefront the shader-clock cycles of its loop, lane 1 the 100 MHz
- * reference ticks of the same interval.  Returns the operations the launch
- * performs in *ops; the caller may also time the launch. */
+ * FMA and kind 1 = v_exp_f32, 8 independent chains of `iters` steps per lane
+ * (dependency latency shows); kind 2 = FMA with 32 chains and kind 3 =
+ * v_exp_f32 with 16 chains (the issue rate, given 8 wavefronts per SIMD:
+ * n_threads = 256 x 8 x 4 x compute units or more).  Writes one float per
+ * thread to out (device, n_threads floats): lane 0 of each wavefront the
+ * shader-clock cycles of its loop, lane 1 the 100 MHz reference ticks of the
+ * same interval.  Returns the operations the launch performs in *ops; the
+ * caller may also time the launch. */
 int fpx_probe_valu(int32_t kind, int32_t iters, float *out, int32_t n_threads,
                    double *ops, void *stream);
 
