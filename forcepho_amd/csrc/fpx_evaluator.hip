@@ -18,20 +18,29 @@
 #include "fpx_internal.h"
 
 // One device allocation per evaluator, each part 256-byte aligned:
-//   io     status word (uint32, 8 bytes with padding), pchi2 double[MAXBANDS],
-//          FpxResponse[MAXBANDS] (fpx_evaluate_host's outputs: one copy
-//          brings the status and the outputs back)
+//   io     status word (uint32, 8 bytes with padding), then two unused
+//          output areas of the io layout below
 //   queue  uint32[2] work-queue counter of persistent launches
 //   prop   FpxSource[MAXSOURCES] (fpx_evaluate_host's proposal)
 //   accum  double[fpx_accum_doubles(n_bands)]
 //   tiles  FpxTile[n_tiles]
-// The pinned host block mirrors io and prop.
+// and one mapped pinned host block: fpx_evaluate_host's outputs in the io
+// layout (pchi2 double[MAXBANDS] at kIoChi2, FpxResponse[MAXBANDS] at
+// kIoResp), written there by the output kernel itself (no copy back), then
+// the proposal's host staging, then -- for patches of at most
+// kPartialTiles tiles -- one row of sums per tile (FPX_SLOT_TILE_PARTIALS).
+//
+// fpx_evaluate_host on such a small patch is one kernel: the tiles read the
+// proposal in place from the staging and store their sums in the host rows,
+// which the host adds up; the output kernel's launch and the copies would
+// cost more than the evaluation itself.
 struct FpxEvaluator {
   int device = 0;
   const FpxPatch *patch = nullptr;
   int32_t n_bands = 0, n_sources = 0, max_gauss = 0, cfg = 0, persistent = 0, n_tiles = 0;
   unsigned char *block = nullptr;
-  unsigned char *host = nullptr;
+  unsigned char *host = nullptr;      // mapped pinned block (host address)
+  unsigned char *host_dev = nullptr;  // the same block's device address
   uint32_t *status = nullptr;
   double *chi2 = nullptr;
   FpxResponse *resp = nullptr;
@@ -39,11 +48,15 @@ struct FpxEvaluator {
   FpxSource *prop = nullptr;
   double *accum = nullptr;
   FpxTile *tiles = nullptr;
+  size_t o_stage = 0, o_part = 0;  // offsets in the host block
+  std::vector<int32_t> tile_band;  // per tile, when the partials path is used
 };
 
 namespace {
 
 constexpr size_t kIoChi2 = 8;
+constexpr int kPartialTiles = 64;
+constexpr size_t kAccRow = 1 + MAXSOURCES * NPARAMS;  // doubles per tile row
 constexpr size_t kIoResp = kIoChi2 + sizeof(double) * MAXBANDS;
 constexpr size_t kIoBytes = kIoResp + sizeof(FpxResponse) * MAXBANDS;
 
@@ -187,9 +200,18 @@ int bind(const FpxPatch *patch_dev, int32_t n_bands, int32_t cfg, int32_t persis
   ev->prop = reinterpret_cast<FpxSource *>(ev->block + o_prop);
   ev->accum = reinterpret_cast<double *>(ev->block + o_accum);
   ev->tiles = reinterpret_cast<FpxTile *>(ev->block + o_tiles);
-  // pinned host mirror of io, then the proposal staging
-  EV_HIP(hipHostMalloc((void **)&ev->host, align256(kIoBytes) + sizeof(FpxSource) * MAXSOURCES,
-                       hipHostMallocDefault));
+  // mapped pinned host block: the outputs of fpx_evaluate_host, the
+  // proposal staging, and the tile rows of a small patch
+  ev->o_stage = align256(kIoBytes);
+  ev->o_part = ev->o_stage + align256(sizeof(FpxSource) * MAXSOURCES);
+  size_t host_bytes = ev->o_part;
+  if (n_tiles > 0 && n_tiles <= kPartialTiles) {
+    host_bytes += sizeof(double) * kAccRow * n_tiles;
+    ev->tile_band.resize(n_tiles);
+    for (int t = 0; t < n_tiles; ++t) ev->tile_band[t] = tiles[t].band;
+  }
+  EV_HIP(hipHostMalloc((void **)&ev->host, host_bytes, hipHostMallocMapped));
+  EV_HIP(hipHostGetDevicePointer((void **)&ev->host_dev, ev->host, 0));
   return FPX_OK;
 }
 
@@ -198,6 +220,43 @@ int read_status(FpxEvaluator *ev, hipStream_t s, uint32_t status) {
   EV_HIP(hipMemsetAsync(ev->status, 0, sizeof(uint32_t), s));
   EV_HIP(hipStreamSynchronize(s));
   return arg_err(fpx::status_text(status));
+}
+
+// The partials path of fpx_evaluate_host (n_tiles <= kPartialTiles).
+int evaluate_small(FpxEvaluator *ev, double *chi2_host, FpxResponse *resp_host, hipStream_t s) {
+  FpxSlot slot = slot_of(ev, reinterpret_cast<const FpxSource *>(ev->host_dev + ev->o_stage),
+                         nullptr, nullptr);
+  slot.accum = reinterpret_cast<double *>(ev->host_dev + ev->o_part);
+  slot.flags = FPX_SLOT_TILE_PARTIALS;
+  int rc = fpx::launch_one(slot, ev->n_tiles, ev->max_gauss, ev->cfg, ev->persistent, ev->queue, s,
+                           false);
+  if (rc) return rc;
+  EV_HIP(hipStreamSynchronize(s));
+  const double *rows = reinterpret_cast<const double *>(ev->host + ev->o_part);
+  const int nv = 1 + NPARAMS * ev->n_sources;
+  std::vector<double> sum((size_t)ev->n_bands * nv, 0.0);
+  for (int t = 0; t < ev->n_tiles; ++t) {
+    double *dst = sum.data() + (size_t)ev->tile_band[t] * nv;
+    const double *src = rows + (size_t)t * kAccRow;
+    for (int j = 0; j < nv; ++j) dst[j] += src[j];
+  }
+  bool nan = false;
+  for (int b = 0; b < ev->n_bands; ++b) nan |= sum[(size_t)b * nv] != sum[(size_t)b * nv];
+  if (nan) {
+    uint32_t status = 0;
+    EV_HIP(hipMemcpy(&status, ev->status, sizeof(status), hipMemcpyDeviceToHost));
+    rc = read_status(ev, s, status);
+    if (rc) return rc;
+  }
+  for (int b = 0; b < ev->n_bands; ++b) {
+    chi2_host[b] = sum[(size_t)b * nv];
+    if (resp_host) {
+      float *g = resp_host[b].dchi2_dparam;
+      std::memset(g, 0, sizeof(FpxResponse));
+      for (int j = 1; j < nv; ++j) g[j - 1] = (float)sum[(size_t)b * nv + j];
+    }
+  }
+  return FPX_OK;
 }
 
 }  // namespace
@@ -236,21 +295,31 @@ int fpx_evaluate_host(FpxEvaluator *ev, const FpxSource *proposal_host, int32_t 
   int rc = check_device(ev);
   if (rc) return rc;
   const hipStream_t s = (hipStream_t)stream;
-  unsigned char *stage = ev->host + align256(kIoBytes);
+  unsigned char *stage = ev->host + ev->o_stage;
   const size_t pbytes = sizeof(FpxSource) * (size_t)n_sources;
   std::memcpy(stage, proposal_host, pbytes);
+  if (!ev->tile_band.empty()) return evaluate_small(ev, chi2_host, resp_host, s);
   EV_HIP(hipMemcpyAsync(ev->prop, stage, pbytes, hipMemcpyHostToDevice, s));
-  rc = fpx::launch_one(slot_of(ev, ev->prop, ev->chi2, ev->resp), ev->n_tiles, ev->max_gauss,
+  // the output kernel writes pchi2 and the responses into the mapped host
+  // block directly
+  double *chi2_out = reinterpret_cast<double *>(ev->host_dev + kIoChi2);
+  FpxResponse *resp_out = reinterpret_cast<FpxResponse *>(ev->host_dev + kIoResp);
+  rc = fpx::launch_one(slot_of(ev, ev->prop, chi2_out, resp_out), ev->n_tiles, ev->max_gauss,
                        ev->cfg, ev->persistent, ev->queue, stream);
   if (rc) return rc;
-  const size_t out_bytes = kIoResp + (resp_host ? sizeof(FpxResponse) * ev->n_bands : 0);
-  EV_HIP(hipMemcpyAsync(ev->host, ev->block, out_bytes, hipMemcpyDeviceToHost, s));
   EV_HIP(hipStreamSynchronize(s));
-  uint32_t status = 0;
-  std::memcpy(&status, ev->host, sizeof(status));
-  rc = read_status(ev, s, status);
-  if (rc) return rc;
-  std::memcpy(chi2_host, ev->host + kIoChi2, sizeof(double) * ev->n_bands);
+  const double *chi2 = reinterpret_cast<const double *>(ev->host + kIoChi2);
+  // a set status word makes the output kernel write NaN chi2: only then is
+  // the word read (and cleared) from the device
+  bool nan = false;
+  for (int b = 0; b < ev->n_bands; ++b) nan |= chi2[b] != chi2[b];
+  if (nan) {
+    uint32_t status = 0;
+    EV_HIP(hipMemcpy(&status, ev->status, sizeof(status), hipMemcpyDeviceToHost));
+    rc = read_status(ev, s, status);
+    if (rc) return rc;
+  }
+  std::memcpy(chi2_host, chi2, sizeof(double) * ev->n_bands);
   if (resp_host) std::memcpy(resp_host, ev->host + kIoResp, sizeof(FpxResponse) * ev->n_bands);
   return FPX_OK;
 }

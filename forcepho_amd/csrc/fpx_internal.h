@@ -21,10 +21,10 @@ int report_hip_error(hipError_t e, const char *where);
 // Text of a nonzero FpxSlot status word.
 std::string status_text(uint32_t status);
 
-// The tile kernel and the output kernel for one slot passed by value (host
-// struct): no slot table in device memory.
+// The tile kernel and (finalize_outputs) the output kernel for one slot
+// passed by value (host struct): no slot table in device memory.
 int launch_one(const FpxSlot &slot, int32_t n_tiles, int32_t max_gauss, int32_t cfg,
-               int32_t persistent, uint32_t *queue, void *stream);
+               int32_t persistent, uint32_t *queue, void *stream, bool finalize_outputs = true);
 // Pixels per tile of kernel configuration cfg (0 if cfg is invalid).
 int config_pixels(int32_t cfg);
 

@@ -577,13 +577,34 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
 
 // Add the tile's per-wave rows to the slot accumulator (band < 0: nothing
 // to add).  No fence or completion count: the outputs are written by
-// finalize_kernel, launched after this kernel on the same stream.
+// finalize_kernel, launched after this kernel on the same stream.  With
+// FPX_SLOT_TILE_PARTIALS the sums are stored instead at
+// accum[local * kAcc + j] (tile `local`'s own row, no atomics), for the
+// caller to add up; a tile left out (band < 0), or one that finds the slot's
+// status word set, stores NaN as its chi2.
 template <int W>
-__device__ void finish_tile(const FpxSlot &slot, int band, int n_src, unsigned char *smem,
-                            int gauss_cap, int ppl) {
+__device__ void finish_tile(const FpxSlot &slot, int local, int band, int n_src,
+                            unsigned char *smem, int gauss_cap, int ppl) {
   const int tid = threadIdx.x;
   const Lds l = carve(smem, W, ppl, gauss_cap);
-  if (band >= 0) {
+  if (__builtin_amdgcn_readfirstlane(slot.flags) & FPX_SLOT_TILE_PARTIALS) {
+    double *acc = slot.accum + (size_t)local * kAcc;
+    const uint32_t st = slot.status == nullptr
+                            ? 0u
+                            : __hip_atomic_load(slot.status, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+    const bool left_out = band < 0 || st != 0u;
+    const int nv = 1 + NPARAMS * n_src;
+    for (int j = tid; j < nv; j += W * 64) {
+      double v = 0.0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        unsigned char *row = l.rows + w * kRowBytes;
+        v += (j == 0) ? row_chi2(row) : (double)row_grad(row)[j - 1];
+      }
+      acc[j] = (j == 0 && left_out) ? __builtin_nan("") : v;
+    }
+  } else if (band >= 0) {
     double *acc = slot.accum + (size_t)band * kAcc;
     const int nv = 1 + NPARAMS * n_src;
     for (int j = tid; j < nv; j += W * 64) {
@@ -664,7 +685,7 @@ __device__ void process_tile(const FpxSlot &slot, int local, int gauss_cap, int 
   if (ok)
     tile_work<W, PPL>(slot, tile, P, props, e, band, n_src, n_psf, n_radii, gauss_cap, smem);
   __syncthreads();
-  finish_tile<W>(slot, ok ? band : -1, ok ? n_src : 0, smem, gauss_cap, PPL);
+  finish_tile<W>(slot, local, ok ? band : -1, ok ? n_src : 0, smem, gauss_cap, PPL);
 }
 
 // kOne: the single slot `one`, passed by value (no slot table in device
@@ -1063,9 +1084,9 @@ namespace fpx {
 int report_error(int code, const char *msg) { return set_err(code, msg); }
 int report_hip_error(hipError_t e, const char *where) { return hip_err(e, where); }
 int launch_one(const FpxSlot &slot, int32_t n_tiles, int32_t max_gauss, int32_t cfg,
-               int32_t persistent, uint32_t *queue, void *stream) {
+               int32_t persistent, uint32_t *queue, void *stream, bool finalize_outputs) {
   return launch(nullptr, &slot, 1, nullptr, n_tiles, max_gauss, cfg, persistent ? 1 : 0, queue,
-                stream);
+                stream, finalize_outputs);
 }
 std::string status_text(uint32_t status) { return status_message(status); }
 int config_pixels(int32_t cfg) {

@@ -145,6 +145,16 @@ typedef struct FpxSlot {
 } FpxSlot;
 
 #define FPX_SLOT_MODEL_ONLY 1
+/* FpxSlot.flags bit (chi2 mode, one slot per launch, no output kernel):
+ * every tile stores its own sums, without atomics, at
+ * accum[tile * (1 + MAXSOURCES * NPARAMS) + j] -- j = 0 its chi2, j = 1 +
+ * NPARAMS * s + k its gradient of source s, parameter k, for j < 1 + NPARAMS
+ * * n_sources -- and the caller adds the tiles of each band (accum then
+ * holds n_tiles rows; it may be mapped host memory).  A tile left out, or
+ * one that finds the status word set, stores NaN as its chi2.  For small
+ * patches, where the output kernel's launch would cost more than the
+ * evaluation; no counterpart in the reference. */
+#define FPX_SLOT_TILE_PARTIALS 2
 
 /* Bits of FpxSlot.status. */
 #define FPX_STATUS_LIMIT 1u       /* a tile broke a size limit: n_sources, n_psf_per_source of

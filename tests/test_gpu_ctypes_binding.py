@@ -228,3 +228,77 @@ def test_bind_refuses_out_of_range_radius_bin():
         assert b"radius_bin" in L.fpx_last_error() and ev.value is None
     finally:
         dev.free()
+
+
+def test_bound_patch_host_path_on_many_tiles():
+    """A patch of more than 64 tiles (8 bands x 2 exposures x 64x64 px, 256
+    tiles of 256 px) takes fpx_evaluate_host's copy-and-output-kernel path
+    instead of the per-tile partials of small patches; both agree with the
+    torch-backed proposer."""
+    import torch
+    from forcepho_amd import scenes
+    from forcepho_amd.patches import SimplePatch
+    from forcepho_amd.proposal import HIPProposer
+    import oracle
+    print("[gpu] bound patch, 256 tiles", flush=True)
+    spec = scenes.spec_multi(3, side=64)
+    patch, scene, _ = scenes.build_patch(spec, SimplePatch, oracle.render_model)
+    patch.return_residual = False
+    patch.send_to_device()
+    proposal = scene.get_proposal()
+    proposal["q"] *= np.float32(0.97)
+    L = _lib.lib()
+    ev = ctypes.c_void_p()
+    assert L.fpx_bind(patch.device_patch.data_ptr(), int(patch.n_bands), 5, 1, ctypes.byref(ev)) == 0
+    try:
+        n_tiles = ctypes.c_int32()
+        assert L.fpx_evaluator_info(ev, ctypes.byref(n_tiles), None, None) == 0
+        assert n_tiles.value > 64
+        nb, ns = int(patch.n_bands), int(patch.n_sources)
+        prop = np.ascontiguousarray(proposal, dtype=source_struct_dtype)
+        chi2 = np.zeros(nb)
+        resp = np.zeros(nb, dtype=response_struct_dtype)
+        assert L.fpx_evaluate_host(ev, prop.ctypes.data, ns, chi2.ctypes.data, resp.ctypes.data,
+                                   None) == 0, L.fpx_last_error()
+    finally:
+        L.fpx_release(ev)
+    c, d = HIPProposer(patch=patch, config=5).evaluate_proposal(proposal, patch=patch)
+    np.testing.assert_allclose(chi2, c, rtol=1e-12)
+    g = resp["dchi2_dparam"].reshape(nb, MAXSOURCES, NPARAMS)[:, :ns].astype(np.float64)
+    assert U.bandmax_rel(g, d).max() <= 1e-6
+
+
+def test_bound_patch_small_path_reports_device_radius_bin():
+    """After the bind, a radius bin outside the table written into the
+    device copy of the PSF block: the small-patch path's tiles leave the
+    Gaussian out, store NaN, and fpx_evaluate_host returns FPX_E_ARG (the
+    device status word read back); with the bin restored it is right again."""
+    import torch
+    print("[gpu] bound patch, device radius bin", flush=True)
+    patch, proposal, expected = U.load_golden("single_s1", return_residual=False)
+    patch.send_to_device()
+    L = _lib.lib()
+    ev = ctypes.c_void_p()
+    assert L.fpx_bind(patch.device_patch.data_ptr(), 1, -1, 1, ctypes.byref(ev)) == 0
+    try:
+        prop = np.ascontiguousarray(proposal, dtype=source_struct_dtype)
+        chi2 = np.zeros(1)
+        resp = np.zeros(1, dtype=response_struct_dtype)
+        call = lambda: L.fpx_evaluate_host(ev, prop.ctypes.data, 1, chi2.ctypes.data,
+                                           resp.ctypes.data, None)
+        assert call() == 0
+        good = chi2.copy()
+        raw = patch.device_ptrs["psfgauss"]
+        off = int(patch.psfgauss_start[0]) * 28 + 24
+        saved = raw[off:off + 4].clone()
+        raw[off:off + 4].copy_(torch.tensor([int(patch.n_radii)], dtype=torch.int32,
+                                            device=raw.device).view(torch.uint8))
+        torch.cuda.synchronize()
+        assert call() == -1 and b"radius bin" in L.fpx_last_error()
+        raw[off:off + 4].copy_(saved)
+        torch.cuda.synchronize()
+        assert call() == 0, L.fpx_last_error()
+        np.testing.assert_allclose(chi2, good, rtol=1e-12)
+        assert np.max(np.abs(good - expected["chi2"]) / np.abs(expected["chi2"])) <= U.CHI2_RTOL
+    finally:
+        L.fpx_release(ev)
