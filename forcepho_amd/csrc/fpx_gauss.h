@@ -88,19 +88,47 @@ struct SourceGeometry {
   float xs, ys;
 };
 
+// The inputs of a SourceGeometry: the source's q, pa and sky position, and
+// the exposure's D, CW, crpix and crval for that source.  Loaded apart from
+// the arithmetic so that a caller can issue these loads together with its
+// others and wait once.
+struct GeometryInputs {
+  float q, pa, ra, dec;
+  float D[4], CW[4], crpix[2], crval[2];
+};
+
+// (FloatPtr and SrcPtr: pointers to float and FpxSource in any address
+// space.)
+template <typename FloatPtr, typename SrcPtr>
+__device__ __forceinline__ GeometryInputs load_geometry_inputs(FloatPtr D, FloatPtr CW,
+                                                               FloatPtr crpix, FloatPtr crval,
+                                                               SrcPtr src, int n_sources, int e,
+                                                               int s) {
+  GeometryInputs in;
+  const int base = n_sources * e + s;
+  in.q = src->q;
+  in.pa = src->pa;
+  in.ra = src->ra;
+  in.dec = src->dec;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    in.D[k] = D[4 * base + k];
+    in.CW[k] = CW[4 * base + k];
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    in.crpix[k] = crpix[2 * base + k];
+    in.crval[k] = crval[2 * base + k];
+  }
+  return in;
+}
+
 template <bool kDerivs>
-__device__ __forceinline__ void source_geometry(const FpxPatch *__restrict__ P,
-                                                const FpxSource *__restrict__ src, int e,
-                                                int s, SourceGeometry &g) {
-  const int base = P->n_sources * e + s;
-  const float *Dp = P->D + 4 * base;
-  const float *CWp = P->CW + 4 * base;
-  const float *crpix = P->crpix + 2 * base;
-  const float *crval = P->crval + 2 * base;
-  const float q = src->q;
+__device__ __forceinline__ void geometry_from(const GeometryInputs &in, SourceGeometry &g) {
+  const float q = in.q;
   float sn, cs;
-  sincosf(src->pa, &sn, &cs);
-  const M22 D = {Dp[0], Dp[1], Dp[2], Dp[3]};
+  sincosf(in.pa, &sn, &cs);
+  const M22 D = {in.D[0], in.D[1], in.D[2], in.D[3]};
   const M22 R = {cs, -sn, sn, cs};
   const float iq = 1.0f / q;
   const M22 S = {iq, 0.f, 0.f, q};
@@ -112,10 +140,18 @@ __device__ __forceinline__ void source_geometry(const FpxPatch *__restrict__ P,
     g.dTq = mat_mul(DR, dS);
     g.dTp = mat_mul(mat_mul(D, dR), S);
   }
-  const float sx = src->ra - crval[0];
-  const float sy = src->dec - crval[1];
-  g.xs = CWp[0] * sx + CWp[1] * sy + crpix[0];
-  g.ys = CWp[2] * sx + CWp[3] * sy + crpix[1];
+  const float sx = in.ra - in.crval[0];
+  const float sy = in.dec - in.crval[1];
+  g.xs = in.CW[0] * sx + in.CW[1] * sy + in.crpix[0];
+  g.ys = in.CW[2] * sx + in.CW[3] * sy + in.crpix[1];
+}
+
+template <bool kDerivs>
+__device__ __forceinline__ void source_geometry(const FpxPatch *__restrict__ P,
+                                                const FpxSource *__restrict__ src, int e,
+                                                int s, SourceGeometry &g) {
+  geometry_from<kDerivs>(
+      load_geometry_inputs(P->D, P->CW, P->crpix, P->crval, src, P->n_sources, e, s), g);
 }
 
 // A SourceGeometry kept in kGeoFloats floats (LDS): T, dT/dq, dT/dpa, centre.
@@ -207,9 +243,10 @@ __device__ __forceinline__ bool gauss_reaches(const SourceGeometry &g,
 // variance covar, exposure calibration gcal) in band `band`.  The caller
 // takes psf from the exposure's PSF block whatever the source, as the
 // reference does (compute_gaussians_kernel.cc:100, .cu:77).  kSrc holds the
-// source index (as int bits).
-__device__ __forceinline__ void build_record(const SourceGeometry &g,
-                                             const FpxSource *__restrict__ src,
+// source index (as int bits).  SrcPtr: a pointer to FpxSource in any
+// address space.
+template <typename SrcPtr>
+__device__ __forceinline__ void build_record(const SourceGeometry &g, SrcPtr src,
                                              const FpxPSFSourceGaussian &psf, float covar,
                                              float gcal, int band, int s,
                                              float *__restrict__ out) {

@@ -111,6 +111,82 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask) {
 }
 
 // ------------------------------------------------------------------------
+// Address spaces.  Reads of data that the kernel never writes and that every
+// lane reads at the same address (slot table, tile records, the patch
+// header and its per-exposure scalars) go through the constant address
+// space, i.e. scalar loads: they need no wait on vector memory and land in
+// SGPRs.  Per-lane reads go through the global address space, whose loads
+// take an SGPR base and need no flat-address handling.
+// ------------------------------------------------------------------------
+#define FPX_CONST_AS __attribute__((address_space(4)))
+#define FPX_GLOBAL_AS __attribute__((address_space(1)))
+template <typename T>
+using cptr = const FPX_CONST_AS T *;
+template <typename T>
+using gptr = const FPX_GLOBAL_AS T *;
+template <typename T>
+__device__ __forceinline__ cptr<T> as_const(const T *p) {
+  return (cptr<T>)p;
+}
+template <typename T>
+__device__ __forceinline__ gptr<T> as_global(const T *p) {
+  return (gptr<T>)p;
+}
+
+// A struct read word by word through a pointer of another address space
+// (the implicit copy of a struct binds only generic references).
+template <typename T, typename WordPtr>
+__device__ __forceinline__ T load_words(WordPtr src) {
+  static_assert(sizeof(T) % 4 == 0, "whole words");
+  T v;
+  uint32_t *dst = reinterpret_cast<uint32_t *>(&v);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = src[i];
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T load_const(const T *p, int i) {
+  return load_words<T>((const FPX_CONST_AS uint32_t *)(p + i));
+}
+template <typename T>
+__device__ __forceinline__ T load_global(const T *p, int i) {
+  return load_words<T>((const FPX_GLOBAL_AS uint32_t *)(p + i));
+}
+
+// The fields of a patch that a tile reads, from one scalar read of the
+// header.
+struct PatchView {
+  gptr<float> data, ierr, xpix, ypix, rad2, D, crpix, crval, CW;
+  cptr<float> G;
+  cptr<int32_t> n_psf_per_source, psfgauss_start;
+  const FpxPSFSourceGaussian *psfgauss;  // read with load_global
+  FPX_GLOBAL_AS float *residual;
+  int n_sources, n_radii;
+};
+
+__device__ __forceinline__ PatchView patch_view(const FpxPatch *P) {
+  const cptr<FpxPatch> h = as_const(P);
+  PatchView v;
+  v.data = as_global<float>(h->data);
+  v.ierr = as_global<float>(h->ierr);
+  v.xpix = as_global<float>(h->xpix);
+  v.ypix = as_global<float>(h->ypix);
+  v.rad2 = as_global<float>(h->rad2);
+  v.D = as_global<float>(h->D);
+  v.crpix = as_global<float>(h->crpix);
+  v.crval = as_global<float>(h->crval);
+  v.CW = as_global<float>(h->CW);
+  v.G = as_const<float>(h->G);
+  v.n_psf_per_source = as_const<int32_t>(h->n_psf_per_source);
+  v.psfgauss_start = as_const<int32_t>(h->psfgauss_start);
+  v.psfgauss = h->psfgauss;
+  v.residual = (FPX_GLOBAL_AS float *)h->residual;
+  v.n_sources = h->n_sources;
+  v.n_radii = h->n_radii;
+  return v;
+}
+
+// ------------------------------------------------------------------------
 // LDS carve-up of one workgroup
 //   records  gauss_cap x kRec floats   ImageGaussians of the current chunk
 //                                       of the tile's surviving Gaussians
@@ -184,22 +260,28 @@ __device__ __forceinline__ float bin_covar(const Lds &l, const FpxPSFSourceGauss
 // into the list of survivors in (source, component) order with one
 // wavefront (ballot + prefix count).  Returns the number of survivors.  A
 // PSF entry whose radius bin lies outside [0, n_radii) never survives and
-// sets FPX_STATUS_RADIUS_BIN in the slot's status word.
+// sets FPX_STATUS_RADIUS_BIN in the slot's status word.  psf0 is the PSF
+// entry of Gaussian threadIdx.x (when below n), loaded by the caller ahead of
+// time.
 template <int W>
-__device__ int cull_tile(const FpxPatch *__restrict__ P, int pstart, int n_src, int n_psf,
+__device__ int cull_tile(const FpxPSFSourceGaussian *psfg, int pstart, int n_src, int n_psf,
                          int n_radii, const Box &box, const Lds &l, int wave, int lane,
-                         uint32_t *status) {
+                         uint32_t *status, const FpxPSFSourceGaussian &psf0) {
   const int n = n_src * n_psf;
   uint16_t *index = l.index;
-  __syncthreads();  // previous users of the index are done; geometry is in LDS
-  for (int lg = threadIdx.x; lg < n; lg += W * 64) {
+  auto flag = [&](int lg, const FpxPSFSourceGaussian &psf) {
     const int s = lg / n_psf;
-    const FpxPSFSourceGaussian psf = P->psfgauss[pstart + lg - s * n_psf];
     const bool bin_ok = psf.sersic_radius_bin >= 0 && psf.sersic_radius_bin < n_radii;
     if (!bin_ok && status != nullptr) atomicOr(status, (uint32_t)FPX_STATUS_RADIUS_BIN);
     index[lg] = bin_ok && gauss_reaches(load_geometry(l.geo + s * kGeoFloats), psf,
                                         bin_covar(l, psf), box.x0, box.x1, box.y0, box.y1)
                     ? 1 : 0;
+  };
+  __syncthreads();  // previous users of the index are done; geometry is in LDS
+  if ((int)threadIdx.x < n) flag(threadIdx.x, psf0);
+  for (int lg = threadIdx.x + W * 64; lg < n; lg += W * 64) {
+    const int s = lg / n_psf;
+    flag(lg, load_global(psfg, pstart + lg - s * n_psf));
   }
   __syncthreads();
   if (wave == 0) {
@@ -220,14 +302,14 @@ __device__ int cull_tile(const FpxPatch *__restrict__ P, int pstart, int n_src, 
 
 // Build the records of survivors [first, first + count) into LDS.
 template <int W>
-__device__ void build_records(const FpxPatch *__restrict__ P,
-                              const FpxSource *__restrict__ props, int pstart, float gcal,
+__device__ void build_records(const FpxPSFSourceGaussian *psfg, gptr<FpxSource> props,
+                              int pstart, float gcal,
                               int band, int n_psf, int first, int count, const Lds &l) {
   __syncthreads();  // previous users of the records are done
   for (int i = threadIdx.x; i < count; i += W * 64) {
     const int lg = l.index[first + i];
     const int s = lg / n_psf;
-    const FpxPSFSourceGaussian psf = P->psfgauss[pstart + lg - s * n_psf];
+    const FpxPSFSourceGaussian psf = load_global(psfg, pstart + lg - s * n_psf);
     build_record(load_geometry(l.geo + s * kGeoFloats), props + s, psf, bin_covar(l, psf), gcal,
                  band, s, l.recs + i * kRec);
   }
@@ -287,6 +369,10 @@ constexpr long long kNoCentre = (long long)0x7fc000017fc00001ull;
 // which spreads costly and cheap Gaussians evenly while keeping a wave's
 // records mostly within one source (few per-source flushes).
 constexpr int kRun = 8;
+// Records per run for a chunk of n records: kRun, or fewer when the chunk
+// holds fewer than kRun per wavefront, so that every wavefront gets a share
+// of a small chunk instead of a few wavefronts taking all of it.
+__device__ __forceinline__ int run_length(int n, int W) { return max(1, min(kRun, (n + W - 1) / W)); }
 
 // Pass 1: add this wave's share of records [0, n) to the lane's model.
 template <int PPL>
@@ -296,8 +382,9 @@ __device__ __forceinline__ void model_pass(const float *__restrict__ recs, int w
   const float4 *R4 = reinterpret_cast<const float4 *>(recs);
   float dx[PPL], dy[PPL], dxx[PPL], dyy[PPL], dxy[PPL];
   long long centre = kNoCentre;
-  for (int r0 = wave * kRun; r0 < n; r0 += W * kRun)
-  for (int g = r0; g < min(r0 + kRun, n); ++g) {
+  const int run = run_length(n, W);
+  for (int r0 = wave * run; r0 < n; r0 += W * run)
+  for (int g = r0; g < min(r0 + run, n); ++g) {
     const float4 r0 = R4[g * (kRec / 4) + 0];  // xc yc cxx cxy
     const float4 r1 = R4[g * (kRec / 4) + 1];  // cyy h0 sxx sxy
     const float4 r2 = R4[g * (kRec / 4) + 2];  // syy amp src dA/dflux
@@ -385,8 +472,9 @@ __device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int wa
   int cur = -1;
   float dx[PPL], dy[PPL], dxx[PPL], dyy[PPL], dxy[PPL];
   long long centre = kNoCentre;
-  for (int r0 = wave * kRun; r0 < n; r0 += W * kRun)
-  for (int g = r0; g < min(r0 + kRun, n); ++g) {
+  const int run = run_length(n, W);
+  for (int r0 = wave * run; r0 < n; r0 += W * run)
+  for (int g = r0; g < min(r0 + run, n); ++g) {
     const float4 *rg = R4 + g * (kRec / 4);
     const float4 r0 = rg[0];  // xc yc cxx cxy
     const float4 r1 = rg[1];  // cyy h0 sxx sxy
@@ -449,13 +537,15 @@ __device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int wa
   if (touched) flush_source(cur, aF, aX, aY, aQ, aP, aN, aR, cwl, wrow, lane);
 }
 
+// One tile: returns false when the exposure's n_psf breaks a size limit
+// (FPX_STATUS_LIMIT is then set and the tile contributes nothing), true
+// otherwise, the tile's sums being in the wave rows.
 template <int W, int PPL>
-__device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPatch *P,
-                          const FpxSource *props, int e, int band, int n_src, int n_psf,
-                          int n_radii, int gauss_cap, unsigned char *smem) {
+__device__ bool tile_work(const FpxSlot &slot, const FpxTile &tile, const PatchView &P,
+                          gptr<FpxSource> props, int e, int band, int n_src, int n_radii,
+                          int gauss_cap, int index_cap, unsigned char *smem) {
   // The tile's Gaussians that reach it are processed in chunks of gauss_cap
   // records (one chunk when they all fit).
-  constexpr int kPix = 64 * PPL;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   // the wave index is uniform across the wavefront; saying so puts the
@@ -463,36 +553,72 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const Lds l = carve(smem, W, PPL, gauss_cap);
   const bool vjp = slot.pixel_weight != nullptr;
+  const int pix_count = tile.pix_count;
 
-  for (int j = tid; j < W * kRowBytes / 4; j += W * 64) reinterpret_cast<float *>(l.rows)[j] = 0.0f;
-  // per-source geometry of the exposure, its CW and the radius-bin variances,
-  // once per tile (the cull and the records read them from LDS)
-  for (int j = tid; j < n_src; j += W * 64) {
-    SourceGeometry g;
-    source_geometry<true>(P, props + j, e, j, g);
-    store_geometry(l.geo + j * kGeoFloats, g);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) l.cwl[4 * j + c] = P->CW[4 * (n_src * e + j) + c];
-  }
-  for (int j = tid; j < n_radii; j += W * 64) l.rad2[j] = P->rad2[j];
-  const int pstart = __builtin_amdgcn_readfirstlane(P->psfgauss_start[e]);
-  const float gcal = uniform_f(P->G[e]);
-
+  // Every load that needs only the tile record and the patch header is
+  // requested here, before any of them is waited for, so that the tile pays
+  // one memory latency for all of them: the exposure's n_psf, PSF offset and
+  // calibration, the pixels (with data and weights in chi2 mode, whose
+  // latency then also overlaps the cull, the records and the model pass;
+  // the pixel weights in vjp mode), the radius-bin variances and the
+  // geometry inputs of one source per thread (MAXSOURCES and MAXRADII are
+  // below the 64 threads of a wavefront).  Indices are clamped into range
+  // instead of branched on.
+  const int n_psf = P.n_psf_per_source[band];
+  const int pstart = P.psfgauss_start[e];
+  const float gcal = P.G[e];
+  const float rad2_v = P.rad2[min(tid, n_radii - 1)];
+  const int js = min(tid, n_src - 1);
+  const GeometryInputs gin =
+      load_geometry_inputs(P.D, P.CW, P.crpix, P.crval, props + js, n_src, e, js);
+  gptr<float> pixel_weight = as_global<float>(slot.pixel_weight);
   float xp[PPL], yp[PPL], w2[PPL], dat[PPL], ier[PPL];
   int pix[PPL];
   bool valid[PPL];
 #pragma unroll
   for (int i = 0; i < PPL; ++i) {
     const int off = i * 64 + lane;
-    valid[i] = off < tile.pix_count;
+    valid[i] = off < pix_count;
     pix[i] = tile.pix_start + off;
-    xp[i] = valid[i] ? P->xpix[pix[i]] : 0.0f;
-    yp[i] = valid[i] ? P->ypix[pix[i]] : 0.0f;
-    // chi2 mode: data and weights requested with the coordinates, so that
-    // their latency overlaps the cull, the records and the model pass
-    dat[i] = (valid[i] && !vjp) ? P->data[pix[i]] : 0.0f;
-    ier[i] = (valid[i] && !vjp) ? P->ierr[pix[i]] : 0.0f;
+    xp[i] = yp[i] = dat[i] = ier[i] = w2[i] = 0.0f;
+    if (i * 64 < pix_count) {  // some lane of the block holds a pixel
+      const int pc = valid[i] ? pix[i] : tile.pix_start;
+      const float x = P.xpix[pc], y = P.ypix[pc];
+      xp[i] = valid[i] ? x : 0.0f;
+      yp[i] = valid[i] ? y : 0.0f;
+      if (!vjp) {
+        const float d = P.data[pc], ie = P.ierr[pc];
+        dat[i] = valid[i] ? d : 0.0f;
+        ier[i] = valid[i] ? ie : 0.0f;
+      } else {
+        const float pw = pixel_weight[pc];
+        w2[i] = valid[i] ? pw : 0.0f;
+      }
+    }
   }
+
+  if (!(n_psf >= 1 && n_psf <= FPX_MAX_PSF_PER_SOURCE && n_src * n_psf <= index_cap)) {
+    if (tid == 0 && slot.status != nullptr) atomicOr(slot.status, (uint32_t)FPX_STATUS_LIMIT);
+    return false;
+  }
+  // the cull's first PSF entry, requested before the geometry is computed
+  const int lg0 = min(tid, n_src * n_psf - 1);
+  const FpxPSFSourceGaussian psf0 = load_global(P.psfgauss, pstart + lg0 - (lg0 / n_psf) * n_psf);
+
+  for (int j = tid; j < W * kRowBytes / 4; j += W * 64) reinterpret_cast<float *>(l.rows)[j] = 0.0f;
+  // per-source geometry of the exposure, its CW and the radius-bin variances,
+  // once per tile (the cull and the records read them from LDS)
+  {
+    SourceGeometry g;
+    geometry_from<true>(gin, g);
+    if (tid < n_src) {
+      store_geometry(l.geo + tid * kGeoFloats, g);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) l.cwl[4 * tid + c] = gin.CW[c];
+    }
+    if (tid < n_radii) l.rad2[tid] = rad2_v;
+  }
+
   Box box;
   {
     constexpr float kInf = __builtin_huge_valf();
@@ -512,8 +638,8 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
     box.y1 = uniform_f(wave_max(y1));
   }
 
-  const int n_surv = cull_tile<W>(P, pstart, n_src, n_psf, n_radii, box, l, wave, lane,
-                                  slot.status);
+  const int n_surv = cull_tile<W>(P.psfgauss, pstart, n_src, n_psf, n_radii, box, l, wave, lane,
+                                  slot.status, psf0);
   const int n_chunks = (n_surv + gauss_cap - 1) / gauss_cap;
   auto chunk_size = [&](int c) { return min(gauss_cap, n_surv - c * gauss_cap); };
   if (!vjp) {
@@ -524,14 +650,14 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
     // LDS for the gradient pass, which then rebuilds only the others
     for (int c = n_chunks - 1; c >= 0; --c) {
       const int n_act = chunk_size(c);
-      build_records<W>(P, props, pstart, gcal, band, n_psf, c * gauss_cap, n_act, l);
+      build_records<W>(P.psfgauss, props, pstart, gcal, band, n_psf, c * gauss_cap, n_act, l);
       model_pass<PPL>(l.recs, wave, W, n_act, xp, yp, m);
     }
 #pragma unroll
     for (int i = 0; i < PPL; ++i) l.part[(wave * PPL + i) * 64 + lane] = m[i];
     __syncthreads();
     double chi2 = 0.0;
-    float *resid = P->residual;
+    FPX_GLOBAL_AS float *resid = P.residual;
 #pragma unroll
     for (int i = 0; i < PPL; ++i) {
       float model = 0.0f;
@@ -550,14 +676,11 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
       chi2 = wave_sum(chi2);
       if (lane == 0) row_chi2(l.rows) += chi2;
     }
-  } else {
-#pragma unroll
-    for (int i = 0; i < PPL; ++i) w2[i] = valid[i] ? slot.pixel_weight[pix[i]] : 0.0f;
   }
 
   // a model-only slot stops at its residual and chi2 (the flag is the same
   // for the whole workgroup, and process_tile's barrier follows)
-  if (!vjp && (__builtin_amdgcn_readfirstlane(slot.flags) & FPX_SLOT_MODEL_ONLY)) return;
+  if (!vjp && (__builtin_amdgcn_readfirstlane(slot.flags) & FPX_SLOT_MODEL_ONLY)) return true;
 
   bool wave_has_weight = false;
 #pragma unroll
@@ -569,10 +692,11 @@ __device__ void tile_work(const FpxSlot &slot, const FpxTile &tile, const FpxPat
   for (int c = 0; c < n_chunks; ++c) {
     const int n_act = chunk_size(c);
     if (c != 0 || vjp)
-      build_records<W>(P, props, pstart, gcal, band, n_psf, c * gauss_cap, n_act, l);
+      build_records<W>(P.psfgauss, props, pstart, gcal, band, n_psf, c * gauss_cap, n_act, l);
     if (wave_has_weight)
       grad_pass<PPL>(l.recs, wave, W, n_act, xp, yp, w2, l.cwl, wrow, lane);
   }
+  return true;
 }
 
 // Add the tile's per-wave rows to the slot accumulator (band < 0: nothing
@@ -630,22 +754,50 @@ __device__ void finish_tile(const FpxSlot &slot, int local, int band, int n_src,
 constexpr int kFinalizeThreads = 256;
 constexpr int kFinalizeChunks = (MAXBANDS * kAcc + kFinalizeThreads - 1) / kFinalizeThreads;
 
-// Output value j of a slot (j < n_bands * kAcc) from its accumulator, which
-// is zeroed.
-__device__ __forceinline__ void slot_output(const FpxSlot &slot, int j) {
-  const double v = __hip_atomic_load(slot.accum + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Output value j of a slot (j < n_bands * kAcc) from its accumulator value v
+// (read by the caller), which is zeroed; st is the slot's status word.
+__device__ __forceinline__ void store_output(const FpxSlot &slot, int j, double v, uint32_t st) {
   const int b = j / kAcc;
   const int k = j - b * kAcc;
-  if (k == 0) {
-    const uint32_t st = slot.status == nullptr
-                            ? 0u
-                            : __hip_atomic_load(slot.status, __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT);
+  if (k == 0)
     slot.pchi2[b] = st == 0u ? v : __builtin_nan("");
-  } else {
+  else
     slot.pdchi2_dp[b * (MAXSOURCES * NPARAMS) + (k - 1)] = (float)v;
-  }
   __hip_atomic_store(slot.accum + j, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t load_status(const FpxSlot &slot) {
+  return slot.status == nullptr
+             ? 0u
+             : __hip_atomic_load(slot.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void slot_output(const FpxSlot &slot, int j) {
+  const double v = __hip_atomic_load(slot.accum + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  store_output(slot, j, v, j % kAcc == 0 ? load_status(slot) : 0u);
+}
+
+// All outputs of a slot from one workgroup of `threads` threads: each thread
+// reads kBatch accumulator values before it writes any, so that the loads'
+// latency is paid once per kBatch values.
+template <int kBatch>
+__device__ void slot_outputs(const FpxSlot &slot, int threads) {
+  const int nv = min(slot.n_bands, MAXBANDS) * kAcc;
+  const uint32_t st = load_status(slot);
+  for (int j0 = threadIdx.x; j0 < nv; j0 += kBatch * threads) {
+    double v[kBatch];
+#pragma unroll
+    for (int i = 0; i < kBatch; ++i) {
+      const int j = j0 + i * threads;
+      v[i] = j < nv ? __hip_atomic_load(slot.accum + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                    : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < kBatch; ++i) {
+      const int j = j0 + i * threads;
+      if (j < nv) store_output(slot, j, v[i], st);
+    }
+  }
 }
 
 __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(
@@ -660,33 +812,40 @@ template <int W, int PPL>
 __device__ void process_tile(const FpxSlot &slot, int local, int gauss_cap, int index_cap,
                              unsigned char *smem) {
   constexpr int kPix = 64 * PPL;
-  // Scalars that every lane holds identically.  The loads below are vector
-  // loads (the kernel writes global memory, so they cannot be proven constant),
-  // hence the explicit broadcasts: they put the loop bounds in SGPRs.
-  const FpxTile tile = slot.tiles[local];
-  const FpxPatch *P = slot.patch;
-  const FpxSource *props = slot.proposal;
-  const int e = __builtin_amdgcn_readfirstlane(tile.exposure);
-  const int band = __builtin_amdgcn_readfirstlane(tile.band);
-  const int n_src = __builtin_amdgcn_readfirstlane(P->n_sources);
-  const int n_radii = __builtin_amdgcn_readfirstlane(P->n_radii);
-  const bool band_ok = band >= 0 && band < slot.n_bands && band < MAXBANDS;
-  const int n_psf = band_ok ? __builtin_amdgcn_readfirstlane(P->n_psf_per_source[band]) : 0;
+  // The tile record and the patch header are read with scalar loads: the
+  // sizes and loop bounds below are in SGPRs.
+  const FpxTile tile = load_const(slot.tiles, local);
+  const PatchView P = patch_view(slot.patch);
+  const gptr<FpxSource> props = as_global(slot.proposal);
+  const int e = tile.exposure;
+  const int band = tile.band;
+  const int n_src = P.n_sources;
+  const int n_radii = P.n_radii;
+  const int pix_count = tile.pix_count;
   // A tile whose sizes break the kernel's assumptions contributes nothing and
   // sets FPX_STATUS_LIMIT in the slot's status word, which makes the output
   // kernel write NaN chi2 (every lane sees the same values, so the whole
-  // workgroup agrees).
-  const bool ok = band_ok && n_src >= 1 && n_src <= MAXSOURCES && n_psf >= 1 &&
-                  n_psf <= FPX_MAX_PSF_PER_SOURCE && n_src * n_psf <= index_cap &&
-                  n_radii >= 1 && n_radii <= MAXRADII && tile.pix_count >= 0 &&
-                  tile.pix_count <= kPix;
+  // workgroup agrees).  The sizes read here come from the tile record and the
+  // patch header; tile_work checks the exposure's n_psf.
+  bool ok = band >= 0 && band < slot.n_bands && band < MAXBANDS && n_src >= 1 &&
+            n_src <= MAXSOURCES && n_radii >= 1 && n_radii <= MAXRADII && pix_count >= 0 &&
+            pix_count <= kPix;
   if (!ok && threadIdx.x == 0 && slot.status != nullptr)
     atomicOr(slot.status, (uint32_t)FPX_STATUS_LIMIT);
   if (ok)
-    tile_work<W, PPL>(slot, tile, P, props, e, band, n_src, n_psf, n_radii, gauss_cap, smem);
+    ok = tile_work<W, PPL>(slot, tile, P, props, e, band, n_src, n_radii, gauss_cap, index_cap,
+                           smem);
   __syncthreads();
   finish_tile<W>(slot, local, ok ? band : -1, ok ? n_src : 0, smem, gauss_cap, PPL);
 }
+
+// Launch modes (bits of the kernel's `mode`).  kModePersistent: a resident
+// grid pulls tiles from queue[0].  kModeOutputs: the last workgroup to finish
+// writes the outputs of the launch's one slot (what finalize_kernel does
+// otherwise); it is found with the done counter queue[1].  In either mode
+// that last workgroup leaves queue[0] and queue[1] zero.
+constexpr int kModePersistent = 1;
+constexpr int kModeOutputs = 2;
 
 // kOne: the single slot `one`, passed by value (no slot table in device
 // memory; slots is not read).  A compile-time variant rather than a runtime
@@ -704,9 +863,10 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(kMinWave
 void chi2_grad_kernel(
     const FpxSlot *__restrict__ slots, FpxSlot one, int n_slots,
     const int32_t *__restrict__ item_offsets, int n_items, int gauss_cap, int index_cap,
-    int persistent, uint32_t *__restrict__ queue) {
+    int mode, uint32_t *__restrict__ queue) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_item;
+  const bool persistent = (mode & kModePersistent) != 0;
   for (int it = 0;; ++it) {
     int item;
     if (persistent) {
@@ -725,23 +885,35 @@ void chi2_grad_kernel(
       int lo = 0, hi = n_slots - 1;  // largest si with item_offsets[si] <= item
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (item_offsets[mid] <= item) lo = mid; else hi = mid - 1;
+        if (as_const(item_offsets)[mid] <= item) lo = mid; else hi = mid - 1;
       }
       si = lo;
-      local = item - item_offsets[si];
+      local = item - as_const(item_offsets)[si];
     }
-    const FpxSlot slot = kOne ? one : slots[si];
+    const FpxSlot slot = kOne ? one : load_const(slots, si);
     process_tile<W, PPL>(slot, local, gauss_cap, index_cap, smem);
   }
-  if (persistent) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const unsigned d = atomicAdd(queue + 1, 1u);
-      if (d == gridDim.x - 1) {
-        queue[0] = 0u;
-        queue[1] = 0u;
-      }
-    }
+  if (mode == 0) return;
+  // Count the workgroup done.  With outputs, every wave first waits for its
+  // accumulator atomics to be performed (the vector memory counter drains
+  // only then), the barrier orders all of them before the count, and the
+  // count is an acquire-release at device scope: the last workgroup sees
+  // every other workgroup's sums.
+  const bool outputs = (mode & kModeOutputs) != 0;
+  if (outputs) __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned d = outputs ? __hip_atomic_fetch_add(queue + 1, 1u, __ATOMIC_ACQ_REL,
+                                                         __HIP_MEMORY_SCOPE_AGENT)
+                               : atomicAdd(queue + 1, 1u);
+    s_item = (int)(d == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_item) return;
+  if (outputs) slot_outputs<4>(kOne ? one : load_const(slots, 0), W * 64);
+  if (threadIdx.x == 0) {
+    queue[0] = 0u;
+    queue[1] = 0u;
   }
 }
 
@@ -902,6 +1074,7 @@ struct KernelCfg {
 const KernelCfg kConfigs[] = {
     FPX_CFG(4, 1), FPX_CFG(4, 2), FPX_CFG(4, 4), FPX_CFG(8, 8),
     FPX_CFG(16, 4), FPX_CFG(8, 4), FPX_CFG(2, 4), FPX_CFG(4, 8),
+    FPX_CFG(8, 1),
 };
 #undef FPX_CFG
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
@@ -1017,9 +1190,12 @@ int launch_lds(DeviceState *st, int cfg, int one, int max_gauss, size_t *lds, in
   return FPX_OK;
 }
 
-// Launch the tile kernel (and, with finalize, the output pass after it) over
-// the slot table `slots` (device) or, when slots is NULL, the single slot
-// *one (host struct, passed by value as a kernel argument).
+// Launch the tile kernel (and, with finalize, the output pass) over the slot
+// table `slots` (device) or, when slots is NULL, the single slot *one (host
+// struct, passed by value as a kernel argument).  The output pass is the
+// tile kernel's last workgroup when the launch holds one slot and has a
+// queue counter (kModeOutputs: one kernel), else finalize_kernel launched
+// after it.
 int launch(const FpxSlot *slots, const FpxSlot *one, int32_t n_slots,
            const int32_t *item_offsets, int32_t n_items, int32_t max_gauss, int32_t cfg,
            int32_t persistent, uint32_t *queue, void *stream, bool finalize_outputs = true) {
@@ -1068,11 +1244,13 @@ int launch(const FpxSlot *slots, const FpxSlot *one, int32_t n_slots,
     if (rc) return rc;
     grid = std::min<long>((long)per_cu * st->n_cu, (long)n_items);
   }
+  const bool fused = finalize_outputs && n_slots == 1 && queue != nullptr;
+  int mode = (persistent ? fpx::kModePersistent : 0) | (fused ? fpx::kModeOutputs : 0);
   void *args[] = {(void *)&slots,     (void *)&byval,      (void *)&n_slots,
                   (void *)&item_offsets, (void *)&n_items, (void *)&cap,
-                  (void *)&index_cap, (void *)&persistent, (void *)&queue};
+                  (void *)&index_cap, (void *)&mode,       (void *)&queue};
   FPX_HIP(hipLaunchKernel(kc.fn[v], dim3(grid), dim3(threads), args, lds, (hipStream_t)stream));
-  if (finalize_outputs)
+  if (finalize_outputs && !fused)
     FPX_HIP(hipLaunchKernel(finalize, dim3(n_slots, fpx::kFinalizeChunks),
                             dim3(fpx::kFinalizeThreads), fargs, 0, (hipStream_t)stream));
   return FPX_OK;

@@ -13,6 +13,9 @@ torch is used only to own device buffers and for copies; all computation
 happens in libfpx.so.
 """
 
+import ctypes
+import weakref
+
 import numpy as np
 
 from .._lib import lib, check, TILE_DTYPE
@@ -34,6 +37,25 @@ def device_bytes(arr, device):
     if raw.size == 0:
         raw = np.zeros(8, dtype=np.uint8)
     return torch.from_numpy(raw.copy()).to(device)
+
+
+class _BoundEvaluators:
+    """FpxEvaluator handles (fpx_bind) of one upload of a patch's device
+    struct, per kernel configuration: config -> (handle, n_tiles).  Released
+    when the struct is uploaded again, when the patch is freed and when it is
+    collected."""
+
+    def __init__(self):
+        self.handles = {}
+        self.generation = None
+
+    def release(self):
+        handles, self.handles = self.handles, {}
+        for ev, _ in handles.values():
+            try:
+                lib().fpx_release(ev)
+            except Exception:
+                pass
 
 
 class DevicePatchMixin:
@@ -265,6 +287,27 @@ class HIPPatchMixin(DevicePatchMixin):
     def max_gauss(self):
         return int(self.n_sources) * int(np.max(self.n_psf_per_source))
 
+    def bound_evaluator(self, config):
+        """An FpxEvaluator bound to the current device struct for kernel
+        configuration ``config`` (fpx_bind, one workgroup per tile), kept
+        until the struct is uploaded again; returns (handle, n_tiles)."""
+        cache = self.__dict__.get("_bound")
+        if cache is None:
+            cache = self._bound = _BoundEvaluators()
+            weakref.finalize(self, cache.release)
+        gen = getattr(self, "_struct_generation", None)
+        if cache.generation != gen:
+            cache.release()
+            cache.generation = gen
+        if config not in cache.handles:
+            ev = ctypes.c_void_p()
+            check(lib().fpx_bind(self.device_patch.data_ptr(), int(self.n_bands), int(config), 0,
+                                 ctypes.byref(ev)), "fpx_bind")
+            n = ctypes.c_int32()
+            check(lib().fpx_evaluator_info(ev, ctypes.byref(n), None, None), "fpx_evaluator_info")
+            cache.handles[config] = (ev, n.value)
+        return cache.handles[config]
+
     def check_limits(self):
         """Raise ValueError unless the packed patch is within the limits the
         kernel relies on: n_sources, n_bands and n_radii within MAXSOURCES,
@@ -299,6 +342,9 @@ class HIPPatchMixin(DevicePatchMixin):
         self._limits_checked = gen
 
     def free(self):
+        cache = self.__dict__.get("_bound")
+        if cache is not None:
+            cache.release()
         self.device_ptrs = {}
         self.device_patch = None
         self._tiles = {}

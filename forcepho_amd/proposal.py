@@ -9,9 +9,12 @@ GPUProposer/CPUProposer.evaluate_proposal (forcepho/proposal.py:96-152,
     residuals   only when ``patch.return_residual``: list of per-exposure
                 padded arrays (``unpack=True``) or one flat array
 
-The launch itself is one call into libfpx.so (fpx_evaluate_proposal); the
-proposal goes host->device and the results device->host through pinned
-buffers on the current torch stream.
+A numpy proposal is evaluated by one native call on the current torch
+stream: on a patch of at most HOST_PATH_TILES tiles, fpx_evaluate_host of the
+patch's bound evaluator (one kernel that reads the proposal from, and writes
+its per-tile sums to, mapped host memory); on larger patches
+fpx_evaluate_sync over the proposer's own (balanced) tile table, with the
+proposal and results copied through pinned buffers.
 """
 
 import sys
@@ -50,6 +53,11 @@ response_struct_dtype = np.dtype([("dchi2_dparam", response_float_dt, NPARAMS * 
 
 # kernel configuration used when none is given: 8 wavefronts x 4 pixels/lane
 DEFAULT_CONFIG = 5
+
+# largest tile count for which evaluate_proposal takes the bound evaluator's
+# host path (fpx_evaluate_host: one kernel, no copies); the library sums the
+# tiles on the host up to this count
+HOST_PATH_TILES = 64
 
 
 class ProposerBase:
@@ -98,7 +106,7 @@ class HIPProposer(ProposerBase):
         self._resp = self._out[_CHI2_END:].view(torch.float32)
         self._prop = torch.zeros(MAXSOURCES * source_struct_dtype.itemsize, dtype=torch.uint8,
                                  device=dev)
-        self._slot = torch.zeros(SLOT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+        self._slot = None       # device FpxSlot of the last launch (built by _slot_for)
         self._slots = {}        # device FpxSlot per (patch struct, tiles, weights)
         self._queue = torch.zeros(2, dtype=torch.int32, device=dev)
         pin = dict(pin_memory=True)
@@ -110,6 +118,9 @@ class HIPProposer(ProposerBase):
         self._resp_np = self._out_h.numpy()[_CHI2_END:].view(np.float32)
         self._slot_key = None
         self._h2d_done = None   # event of the last copy out of the pinned proposal buffer
+        # outputs of the host path (fpx_evaluate_host)
+        self._host_chi2 = np.zeros(MAXBANDS, dtype=np.float64)
+        self._host_resp = np.zeros(MAXBANDS * MAXSOURCES * NPARAMS, dtype=np.float32)
 
     def set_config(self, config):
         """Select the kernel configuration (tile shape, ``fpx_config_info``)."""
@@ -218,7 +229,10 @@ class HIPProposer(ProposerBase):
 
     def finalize(self):
         """Write the outputs of the accumulated tile launches and zero the
-        accumulator (fpx_finalize)."""
+        accumulator (fpx_finalize).  Needs a launch before it: the output
+        pass reads the slot that launch built."""
+        if self._slot is None:
+            raise RuntimeError("finalize needs a launch first (no device slot has been built)")
         check(lib().fpx_finalize(self._slot.data_ptr(), 1, self._stream().cuda_stream),
               "fpx_finalize")
 
@@ -335,13 +349,23 @@ class HIPProposer(ProposerBase):
             print(f"fpx launch: config {self.config}, {n_bands} bands, {n_sources} sources",
                   file=sys.stderr, flush=True)
         stream = self._stream()
+        chi2_np, resp_np = self._chi2_np, self._resp_np
         if isinstance(proposal, np.ndarray):
-            # host proposal: copy in, launch, copy out and wait in one native
-            # call (fpx_evaluate_sync) through the pinned staging buffers
             if len(proposal) != n_sources:
                 raise ValueError(f"proposal has {len(proposal)} sources, patch has {n_sources}")
             patch.check_limits()
-            raw = np.ascontiguousarray(proposal, dtype=source_struct_dtype).view(np.uint8)
+            prop = np.ascontiguousarray(proposal, dtype=source_struct_dtype)
+            if patch.device_tiles(self.pix_per_tile)[1] <= HOST_PATH_TILES:
+                # a small patch: the bound evaluator's one-kernel host path
+                ev, _ = patch.bound_evaluator(self.config)
+                chi2_np, resp_np = self._host_chi2, self._host_resp
+                check(lib().fpx_evaluate_host(ev, prop.ctypes.data, n_sources, chi2_np.ctypes.data,
+                                              resp_np.ctypes.data, stream.cuda_stream),
+                      "fpx_evaluate_host")
+                return self._outputs(chi2_np, resp_np, patch, unpack)
+            # host proposal: copy in, launch, copy out and wait in one native
+            # call (fpx_evaluate_sync) through the pinned staging buffers
+            raw = prop.view(np.uint8)
             if self._h2d_done is not None:
                 self._h2d_done.synchronize()   # an earlier launch's copy may be in flight
                 self._h2d_done = None
@@ -361,8 +385,13 @@ class HIPProposer(ProposerBase):
                 self._out_h.copy_(self._out, non_blocking=True)
             stream.synchronize()
             self._check_status()
-        chi_out = self._chi2_np[:n_bands].astype(self.chi_dtype, copy=True)
-        derivs = self._resp_np[:n_bands * MAXSOURCES * NPARAMS].reshape(n_bands, MAXSOURCES, NPARAMS)
+        return self._outputs(chi2_np, resp_np, patch, unpack)
+
+    def _outputs(self, chi2_np, resp_np, patch, unpack):
+        """evaluate_proposal's return value from the output arrays."""
+        n_bands, n_sources = int(patch.n_bands), int(patch.n_sources)
+        chi_out = chi2_np[:n_bands].astype(self.chi_dtype, copy=True)
+        derivs = resp_np[:n_bands * MAXSOURCES * NPARAMS].reshape(n_bands, MAXSOURCES, NPARAMS)
         chi2_derivs = derivs[:, :n_sources].astype(np.float64)
         if patch.return_residual:
             residuals = patch.retrieve_array("residual")

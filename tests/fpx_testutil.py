@@ -12,6 +12,7 @@ GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 # tolerance of the parity bar
 CHI2_RTOL = 1e-6     # chi2 relative to the reference CPU kernel
+N_CONFIGS = 9        # kernel configurations (tile shapes) the library defines
 GRAD_RTOL = 1e-4     # gradient relative error ...
 # ... above a floor of this fraction of the summed magnitude sum|w dC/dtheta|.
 # Near the MAX_EXP_ARG cutoff the float32 exponent e ~ 36 is the result of

@@ -11,6 +11,8 @@ from forcepho_amd import _lib, kernel_limits
 from forcepho_amd.patches import PatchBase, PSF_DTYPE
 from forcepho_amd.proposal import source_struct_dtype, response_struct_dtype
 
+import fpx_testutil as U
+
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -110,7 +112,7 @@ def test_plan_tiles_rejects_bad_sizes():
 def test_configs_and_lds():
     L = _lib.lib()
     n = L.fpx_n_configs()
-    assert n >= 4
+    assert n == U.N_CONFIGS
     w, p, t = (ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32())
     for cfg in range(n):
         assert L.fpx_config_info(cfg, ctypes.byref(w), ctypes.byref(p), ctypes.byref(t)) == 0

@@ -58,7 +58,7 @@ def test_fixture_parity(name, fp64_cache):
 
 
 @pytest.mark.parametrize("persistent", [True, False])
-@pytest.mark.parametrize("config", range(8))
+@pytest.mark.parametrize("config", range(U.N_CONFIGS))
 def test_all_configs_agree(config, persistent):
     name = "multi_small_s2"
     print(f"[gpu] config {config} persistent {persistent} on {name}", flush=True)
@@ -91,17 +91,26 @@ def test_autotune_keeps_results():
 @pytest.mark.parametrize("persistent", [True, False])
 def test_balanced_tiles_keep_results(persistent):
     """Reordering the tile table for load balance changes no result beyond
-    summation order."""
+    summation order.  The proposal is a device tensor, so that the
+    evaluations run over the proposer's own tile table (a numpy proposal on
+    a patch this small takes the bound evaluator's host path); that path
+    gives the same results too."""
+    import torch
     from forcepho_amd.proposal import HIPProposer
     name = "multi_small_s2"
     patch, proposal, expected = U.load_golden(name, return_residual=True)
     patch.send_to_device()
     prop = HIPProposer(patch=patch, config=2, persistent=persistent)
-    before = prop.evaluate_proposal(proposal, patch=patch, unpack=False)
+    pdev = torch.from_numpy(np.ascontiguousarray(proposal).view(np.uint8).reshape(-1).copy()).cuda()
+    before = prop.evaluate_proposal(pdev, patch=patch, unpack=False)
     perm = prop.balance(proposal, patch)
     n_tiles = patch.device_tiles(prop.pix_per_tile)[1]
     assert sorted(perm.tolist()) == list(range(n_tiles))
-    after = prop.evaluate_proposal(proposal, patch=patch, unpack=False)
+    after = prop.evaluate_proposal(pdev, patch=patch, unpack=False)
+    host = prop.evaluate_proposal(proposal, patch=patch, unpack=False)
+    np.testing.assert_allclose(host[0], before[0], rtol=1e-12)
+    np.testing.assert_allclose(host[1], before[1], rtol=1e-5, atol=1e-6 * np.abs(before[1]).max())
+    np.testing.assert_array_equal(host[2], before[2])
     np.testing.assert_allclose(after[0], before[0], rtol=1e-12)
     np.testing.assert_allclose(after[1], before[1], rtol=1e-5,
                                atol=1e-6 * np.abs(before[1]).max())
@@ -139,7 +148,7 @@ def test_chi2_convention():
     np.testing.assert_allclose(chi2, [x.sum() for x in bands], rtol=1e-6)
 
 
-@pytest.mark.parametrize("config", range(8))
+@pytest.mark.parametrize("config", range(U.N_CONFIGS))
 def test_chunked_records_every_config(config):
     """30 sources: the surviving records exceed one LDS chunk for the small
     configurations, so they are built and consumed in several chunks."""
@@ -183,13 +192,27 @@ def test_launch_without_tiles_writes_zeros():
     patch, proposal, expected = U.load_golden("single_s1", return_residual=False)
     patch.send_to_device()
     prop = HIPProposer(patch=patch, persistent=False)
-    chi2, _ = prop.evaluate_proposal(proposal, patch=patch)
-    assert np.all(chi2 != 0)
+    prop.launch(proposal, patch)           # builds the proposer's device slot
+    torch.cuda.synchronize()
+    assert np.all(prop._chi2[:int(patch.n_bands)].cpu().numpy() != 0)
     check(lib().fpx_evaluate_proposal(prop._slot.data_ptr(), 0, patch.max_gauss(), prop.config,
                                       torch.cuda.current_stream().cuda_stream), "empty launch")
     torch.cuda.synchronize()
     assert np.all(prop._chi2[:int(patch.n_bands)].cpu().numpy() == 0)
     assert np.all(prop._resp.cpu().numpy() == 0)
+
+
+def test_finalize_needs_a_launch():
+    """The output pass reads the slot a launch built: before any launch (a
+    numpy evaluation of a small patch builds none) finalize refuses on the
+    host instead of launching over an empty slot."""
+    from forcepho_amd.proposal import HIPProposer
+    patch, proposal, expected = U.load_golden("single_s1", return_residual=False)
+    patch.send_to_device()
+    prop = HIPProposer(patch=patch)
+    prop.evaluate_proposal(proposal, patch=patch)
+    with pytest.raises(RuntimeError, match="launch first"):
+        prop.finalize()
 
 
 @pytest.mark.parametrize("persistent", [True, False])
@@ -305,6 +328,11 @@ def test_too_small_max_gauss_is_an_error(persistent):
     patch.send_to_device()
     prop = HIPProposer(patch=patch, persistent=persistent)
     chi2, d = prop.evaluate_proposal(proposal, patch=patch)
+    # the device slot fpx_evaluate_sync is given below is built by a launch
+    # (a numpy evaluation of a patch this small takes the bound evaluator's
+    # host path and builds none)
+    prop.launch(proposal, patch)
+    torch.cuda.synchronize()
     n_tiles = patch.device_tiles(prop.pix_per_tile)[1]
     nb = int(patch.n_bands)
     raw = np.ascontiguousarray(proposal).view(np.uint8)
