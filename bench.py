@@ -747,8 +747,16 @@ def main():
     plain, plain_prop, patch, scene, proposal = plain_measurement(args)
     print(f"plain multi-band: {plain['ms_per_step']:.4f} ms per evaluation", file=sys.stderr,
           flush=True)
-    head = batched_measurement(args, plain_prop)
-    print(f"batched: {head['ms_per_step']:.3f} ms per launch", file=sys.stderr, flush=True)
+    if args.batch_patches > 0:
+        head = batched_measurement(args, plain_prop)
+        print(f"batched: {head['ms_per_step']:.3f} ms per launch", file=sys.stderr, flush=True)
+    else:
+        # a run of the plain workload alone (counter collection): the plain
+        # measurement stands in for the headline
+        head = {"value": plain["evals_per_s"], "ms_per_step": plain["ms_per_step"],
+                "config": {"workload": plain["workload"],
+                           "evals_per_step": plain["evals_per_step"]},
+                "wall_s": None, "roofline": plain["roofline"], "vs_plain": {}}
     probe = None if args.no_probe else valu_probe()
     if probe is not None:
         for r in (head["roofline"], plain["roofline"]):
