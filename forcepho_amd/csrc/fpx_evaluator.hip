@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -33,7 +34,11 @@
 // fpx_evaluate_host on such a small patch is one kernel: the tiles read the
 // proposal in place from the staging and store their sums in the host rows,
 // which the host adds up; the output kernel's launch and the copies would
-// cost more than the evaluation itself.
+// cost more than the evaluation itself.  fpx_evaluate_host does not wait for
+// the stream: the launch's last workgroup stores the call's sequence number
+// into the first word of the host block once every result is visible to the
+// host, and the host spins on that word (a stream synchronisation takes
+// some 4 us longer to return).
 struct FpxEvaluator {
   int device = 0;
   const FpxPatch *patch = nullptr;
@@ -50,10 +55,12 @@ struct FpxEvaluator {
   FpxTile *tiles = nullptr;
   size_t o_stage = 0, o_part = 0;  // offsets in the host block
   std::vector<int32_t> tile_band;  // per tile, when the partials path is used
+  uint32_t seq = 0;                // sequence number of the last host-path call
 };
 
 namespace {
 
+constexpr size_t kIoFlag = 0;  // host block: the completion flag
 constexpr size_t kIoChi2 = 8;
 constexpr int kPartialTiles = 64;
 constexpr size_t kAccRow = 1 + MAXSOURCES * NPARAMS;  // doubles per tile row
@@ -88,6 +95,9 @@ int check_device(const FpxEvaluator *ev) {
 
 void release(FpxEvaluator *ev) {
   if (!ev) return;
+  // a host-path call returns once its results are visible, which can be
+  // before its kernel has exited
+  (void)hipDeviceSynchronize();
   if (ev->block) (void)hipFree(ev->block);
   if (ev->host) (void)hipHostFree(ev->host);
   delete ev;
@@ -212,6 +222,34 @@ int bind(const FpxPatch *patch_dev, int32_t n_bands, int32_t cfg, int32_t persis
   }
   EV_HIP(hipHostMalloc((void **)&ev->host, host_bytes, hipHostMallocMapped));
   EV_HIP(hipHostGetDevicePointer((void **)&ev->host_dev, ev->host, 0));
+  // pinned blocks are recycled: the flag must not start out holding a
+  // sequence number
+  std::memset(ev->host, 0, host_bytes);
+  return FPX_OK;
+}
+
+// The next sequence number of a host-path call (never 0); the flag is
+// cleared to 0 before the launch that is to store it.
+uint32_t next_seq(FpxEvaluator *ev) {
+  if (++ev->seq == 0) ev->seq = 1;
+  __atomic_store_n(reinterpret_cast<uint32_t *>(ev->host + kIoFlag), 0u, __ATOMIC_RELEASE);
+  return ev->seq;
+}
+
+// Wait until the launch that carries `seq` has stored it in the host block's
+// flag: spin on the mapped word for up to 50 ms, then fall back to a stream
+// synchronisation (which also reports a failed launch).
+int wait_flag(FpxEvaluator *ev, uint32_t seq, hipStream_t s) {
+  const uint32_t *flag = reinterpret_cast<const uint32_t *>(ev->host + kIoFlag);
+  const auto t0 = std::chrono::steady_clock::now();
+  while (__atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq) {
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) {
+      EV_HIP(hipStreamSynchronize(s));
+      if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq)
+        return fpx::report_error(FPX_E_HIP, "the evaluation did not signal completion");
+      break;
+    }
+  }
   return FPX_OK;
 }
 
@@ -228,10 +266,12 @@ int evaluate_small(FpxEvaluator *ev, double *chi2_host, FpxResponse *resp_host, 
                          nullptr, nullptr);
   slot.accum = reinterpret_cast<double *>(ev->host_dev + ev->o_part);
   slot.flags = FPX_SLOT_TILE_PARTIALS;
+  const uint32_t seq = next_seq(ev);
   int rc = fpx::launch_one(slot, ev->n_tiles, ev->max_gauss, ev->cfg, ev->persistent, ev->queue, s,
-                           false);
+                           false, reinterpret_cast<uint32_t *>(ev->host_dev + kIoFlag), seq);
   if (rc) return rc;
-  EV_HIP(hipStreamSynchronize(s));
+  rc = wait_flag(ev, seq, s);
+  if (rc) return rc;
   const double *rows = reinterpret_cast<const double *>(ev->host + ev->o_part);
   const int nv = 1 + NPARAMS * ev->n_sources;
   std::vector<double> sum((size_t)ev->n_bands * nv, 0.0);
@@ -300,16 +340,26 @@ int fpx_evaluate_host(FpxEvaluator *ev, const FpxSource *proposal_host, int32_t 
   std::memcpy(stage, proposal_host, pbytes);
   if (!ev->tile_band.empty()) return evaluate_small(ev, chi2_host, resp_host, s);
   EV_HIP(hipMemcpyAsync(ev->prop, stage, pbytes, hipMemcpyHostToDevice, s));
-  // the output kernel writes pchi2 and the responses into the mapped host
-  // block directly
+  // the launch's output pass (its last workgroup) writes pchi2 and the
+  // responses into the mapped host block directly
   double *chi2_out = reinterpret_cast<double *>(ev->host_dev + kIoChi2);
   FpxResponse *resp_out = reinterpret_cast<FpxResponse *>(ev->host_dev + kIoResp);
-  rc = fpx::launch_one(slot_of(ev, ev->prop, chi2_out, resp_out), ev->n_tiles, ev->max_gauss,
-                       ev->cfg, ev->persistent, ev->queue, stream);
-  if (rc) return rc;
-  EV_HIP(hipStreamSynchronize(s));
+  if (ev->n_tiles > 0) {
+    const uint32_t seq = next_seq(ev);
+    rc = fpx::launch_one(slot_of(ev, ev->prop, chi2_out, resp_out), ev->n_tiles, ev->max_gauss,
+                         ev->cfg, ev->persistent, ev->queue, stream, true,
+                         reinterpret_cast<uint32_t *>(ev->host_dev + kIoFlag), seq);
+    if (rc) return rc;
+    rc = wait_flag(ev, seq, s);
+    if (rc) return rc;
+  } else {  // no tiles: the output kernel alone writes zeros
+    rc = fpx::launch_one(slot_of(ev, ev->prop, chi2_out, resp_out), 0, ev->max_gauss, ev->cfg,
+                         ev->persistent, ev->queue, stream);
+    if (rc) return rc;
+    EV_HIP(hipStreamSynchronize(s));
+  }
   const double *chi2 = reinterpret_cast<const double *>(ev->host + kIoChi2);
-  // a set status word makes the output kernel write NaN chi2: only then is
+  // a set status word makes the output pass write NaN chi2: only then is
   // the word read (and cleared) from the device
   bool nan = false;
   for (int b = 0; b < ev->n_bands; ++b) nan |= chi2[b] != chi2[b];

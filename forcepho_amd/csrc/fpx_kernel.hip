@@ -842,10 +842,14 @@ __device__ void process_tile(const FpxSlot &slot, int local, int gauss_cap, int 
 // Launch modes (bits of the kernel's `mode`).  kModePersistent: a resident
 // grid pulls tiles from queue[0].  kModeOutputs: the last workgroup to finish
 // writes the outputs of the launch's one slot (what finalize_kernel does
-// otherwise); it is found with the done counter queue[1].  In either mode
-// that last workgroup leaves queue[0] and queue[1] zero.
+// otherwise); it is found with the done counter queue[1].  kModeFlag: that
+// last workgroup, after everything else, stores `seq` into *flag (memory the
+// host reads, mapped) with a system-scope release, so that the host can see
+// the launch's results without waiting for the stream.  In every mode the
+// last workgroup leaves queue[0] and queue[1] zero.
 constexpr int kModePersistent = 1;
 constexpr int kModeOutputs = 2;
+constexpr int kModeFlag = 4;
 
 // kOne: the single slot `one`, passed by value (no slot table in device
 // memory; slots is not read).  A compile-time variant rather than a runtime
@@ -863,7 +867,7 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(kMinWave
 void chi2_grad_kernel(
     const FpxSlot *__restrict__ slots, FpxSlot one, int n_slots,
     const int32_t *__restrict__ item_offsets, int n_items, int gauss_cap, int index_cap,
-    int mode, uint32_t *__restrict__ queue) {
+    int mode, uint32_t *__restrict__ queue, uint32_t *__restrict__ flag, uint32_t seq) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_item;
   const bool persistent = (mode & kModePersistent) != 0;
@@ -894,26 +898,37 @@ void chi2_grad_kernel(
     process_tile<W, PPL>(slot, local, gauss_cap, index_cap, smem);
   }
   if (mode == 0) return;
-  // Count the workgroup done.  With outputs, every wave first waits for its
-  // accumulator atomics to be performed (the vector memory counter drains
-  // only then), the barrier orders all of them before the count, and the
-  // count is an acquire-release at device scope: the last workgroup sees
-  // every other workgroup's sums.
+  // Count the workgroup done.  With outputs or a flag, every wave first waits
+  // for its stores and atomics to be performed (the vector memory counter
+  // drains only then), the barrier orders all of them before the count, and
+  // the count is an acquire-release: at device scope for outputs (the last
+  // workgroup sees every other workgroup's sums), at system scope for a flag
+  // (the host reads what every workgroup stored once it sees the flag).
   const bool outputs = (mode & kModeOutputs) != 0;
-  if (outputs) __builtin_amdgcn_s_waitcnt(0);
+  const bool signal = (mode & kModeFlag) != 0;
+  if (outputs || signal) __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
-    const unsigned d = outputs ? __hip_atomic_fetch_add(queue + 1, 1u, __ATOMIC_ACQ_REL,
-                                                         __HIP_MEMORY_SCOPE_AGENT)
-                               : atomicAdd(queue + 1, 1u);
+    unsigned d;
+    if (signal)
+      d = __hip_atomic_fetch_add(queue + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_SYSTEM);
+    else if (outputs)
+      d = __hip_atomic_fetch_add(queue + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    else
+      d = atomicAdd(queue + 1, 1u);
     s_item = (int)(d == gridDim.x - 1);
   }
   __syncthreads();
   if (!s_item) return;
   if (outputs) slot_outputs<4>(kOne ? one : load_const(slots, 0), W * 64);
+  if (signal) {  // the outputs are stored before the flag
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  }
   if (threadIdx.x == 0) {
     queue[0] = 0u;
     queue[1] = 0u;
+    if (signal) __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
@@ -1198,7 +1213,8 @@ int launch_lds(DeviceState *st, int cfg, int one, int max_gauss, size_t *lds, in
 // after it.
 int launch(const FpxSlot *slots, const FpxSlot *one, int32_t n_slots,
            const int32_t *item_offsets, int32_t n_items, int32_t max_gauss, int32_t cfg,
-           int32_t persistent, uint32_t *queue, void *stream, bool finalize_outputs = true) {
+           int32_t persistent, uint32_t *queue, void *stream, bool finalize_outputs = true,
+           uint32_t *flag = nullptr, uint32_t seq = 0) {
   int rc = check_cfg(cfg);
   if (rc) return rc;
   if ((!slots && !one) || n_slots <= 0 || n_items < 0)
@@ -1208,6 +1224,8 @@ int launch(const FpxSlot *slots, const FpxSlot *one, int32_t n_slots,
   if (max_gauss <= 0 || max_gauss > MAXSOURCES * FPX_MAX_PSF_PER_SOURCE)
     return set_err(FPX_E_ARG, "max_gauss out of range [1, MAXSOURCES*FPX_MAX_PSF_PER_SOURCE]");
   if (persistent && !queue) return set_err(FPX_E_ARG, "persistent launch needs a queue counter");
+  if (flag && !queue) return set_err(FPX_E_ARG, "a completion flag needs a queue counter");
+  if (flag && n_items == 0) return set_err(FPX_E_ARG, "a completion flag needs a tile to launch");
   DeviceState *st = nullptr;
   rc = current_device_state(&st);
   if (rc) return rc;
@@ -1245,10 +1263,12 @@ int launch(const FpxSlot *slots, const FpxSlot *one, int32_t n_slots,
     grid = std::min<long>((long)per_cu * st->n_cu, (long)n_items);
   }
   const bool fused = finalize_outputs && n_slots == 1 && queue != nullptr;
-  int mode = (persistent ? fpx::kModePersistent : 0) | (fused ? fpx::kModeOutputs : 0);
+  int mode = (persistent ? fpx::kModePersistent : 0) | (fused ? fpx::kModeOutputs : 0) |
+             (flag ? fpx::kModeFlag : 0);
   void *args[] = {(void *)&slots,     (void *)&byval,      (void *)&n_slots,
                   (void *)&item_offsets, (void *)&n_items, (void *)&cap,
-                  (void *)&index_cap, (void *)&mode,       (void *)&queue};
+                  (void *)&index_cap, (void *)&mode,       (void *)&queue,
+                  (void *)&flag,      (void *)&seq};
   FPX_HIP(hipLaunchKernel(kc.fn[v], dim3(grid), dim3(threads), args, lds, (hipStream_t)stream));
   if (finalize_outputs && !fused)
     FPX_HIP(hipLaunchKernel(finalize, dim3(n_slots, fpx::kFinalizeChunks),
@@ -1262,9 +1282,10 @@ namespace fpx {
 int report_error(int code, const char *msg) { return set_err(code, msg); }
 int report_hip_error(hipError_t e, const char *where) { return hip_err(e, where); }
 int launch_one(const FpxSlot &slot, int32_t n_tiles, int32_t max_gauss, int32_t cfg,
-               int32_t persistent, uint32_t *queue, void *stream, bool finalize_outputs) {
+               int32_t persistent, uint32_t *queue, void *stream, bool finalize_outputs,
+               uint32_t *flag, uint32_t seq) {
   return launch(nullptr, &slot, 1, nullptr, n_tiles, max_gauss, cfg, persistent ? 1 : 0, queue,
-                stream, finalize_outputs);
+                stream, finalize_outputs, flag, seq);
 }
 std::string status_text(uint32_t status) { return status_message(status); }
 int config_pixels(int32_t cfg) {

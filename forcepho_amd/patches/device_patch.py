@@ -285,7 +285,15 @@ class HIPPatchMixin(DevicePatchMixin):
         return self._tiles[pix_per_tile]
 
     def max_gauss(self):
-        return int(self.n_sources) * int(np.max(self.n_psf_per_source))
+        """n_sources x the largest n_psf_per_source of the packed patch (kept
+        per upload of the device struct: launches ask for it every call)."""
+        gen = getattr(self, "_struct_generation", None)
+        cached = self.__dict__.get("_max_gauss")
+        if cached is not None and gen is not None and cached[0] == gen:
+            return cached[1]
+        value = int(self.n_sources) * int(np.max(self.n_psf_per_source))
+        self._max_gauss = (gen, value)
+        return value
 
     def bound_evaluator(self, config):
         """An FpxEvaluator bound to the current device struct for kernel

@@ -342,7 +342,10 @@ int fpx_evaluate(FpxEvaluator *ev, const FpxSource *proposal_dev, double *pchi2_
  * may be NULL), the status checked and cleared (FPX_E_ARG if a tile was left
  * out).  forcepho's GPUProposer.evaluate_proposal kernel call with its
  * cuda.In / cuda.Out arguments (forcepho/proposal.py:133-136) as one native
- * call; n_sources must be the bound patch's. */
+ * call; n_sources must be the bound patch's.  Returns once the results are
+ * in the host buffers, which the launch signals through mapped memory; the
+ * kernel may then still be exiting, so the stream is not idle (later work
+ * on it is ordered after the kernel as usual). */
 int fpx_evaluate_host(FpxEvaluator *ev, const FpxSource *proposal_host, int32_t n_sources,
                       double *chi2_host, FpxResponse *resp_host, void *stream);
 
