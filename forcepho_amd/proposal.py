@@ -118,9 +118,13 @@ class HIPProposer(ProposerBase):
         self._resp_np = self._out_h.numpy()[_CHI2_END:].view(np.float32)
         self._slot_key = None
         self._h2d_done = None   # event of the last copy out of the pinned proposal buffer
-        # outputs of the host path (fpx_evaluate_host)
+        # proposal and outputs of the host path (fpx_evaluate_host), with their
+        # addresses taken once (numpy's .ctypes.data costs microseconds)
+        self._host_prop = np.zeros(MAXSOURCES, dtype=source_struct_dtype)
         self._host_chi2 = np.zeros(MAXBANDS, dtype=np.float64)
         self._host_resp = np.zeros(MAXBANDS * MAXSOURCES * NPARAMS, dtype=np.float32)
+        self._host_ptrs = (self._host_prop.ctypes.data, self._host_chi2.ctypes.data,
+                           self._host_resp.ctypes.data)
 
     def set_config(self, config):
         """Select the kernel configuration (tile shape, ``fpx_config_info``)."""
@@ -354,15 +358,16 @@ class HIPProposer(ProposerBase):
             if len(proposal) != n_sources:
                 raise ValueError(f"proposal has {len(proposal)} sources, patch has {n_sources}")
             patch.check_limits()
-            prop = np.ascontiguousarray(proposal, dtype=source_struct_dtype)
             if patch.device_tiles(self.pix_per_tile)[1] <= HOST_PATH_TILES:
                 # a small patch: the bound evaluator's one-kernel host path
                 ev, _ = patch.bound_evaluator(self.config)
-                chi2_np, resp_np = self._host_chi2, self._host_resp
-                check(lib().fpx_evaluate_host(ev, prop.ctypes.data, n_sources, chi2_np.ctypes.data,
-                                              resp_np.ctypes.data, stream.cuda_stream),
+                self._host_prop[:n_sources] = proposal
+                check(lib().fpx_evaluate_host(ev, self._host_ptrs[0], n_sources,
+                                              self._host_ptrs[1], self._host_ptrs[2],
+                                              stream.cuda_stream),
                       "fpx_evaluate_host")
-                return self._outputs(chi2_np, resp_np, patch, unpack)
+                return self._outputs(self._host_chi2, self._host_resp, patch, unpack)
+            prop = np.ascontiguousarray(proposal, dtype=source_struct_dtype)
             # host proposal: copy in, launch, copy out and wait in one native
             # call (fpx_evaluate_sync) through the pinned staging buffers
             raw = prop.view(np.uint8)
