@@ -173,6 +173,15 @@ class HIPProposer(ProposerBase):
         import torch
         return torch.cuda.current_stream(self.device)
 
+    def _raw_stream(self):
+        """Handle of the current stream of the proposer's device, without
+        building a torch Stream object (which costs ~2 us per call)."""
+        import torch
+        raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+        if raw is None:
+            return self._stream().cuda_stream
+        return raw(self.device.index)
+
     def _slot_for(self, patch, pixel_weight=None, prop_ptr=None, flags=0):
         """(Re)build the device FpxSlot when the patch struct, tiles, weights,
         proposal buffer or flags change."""
@@ -216,19 +225,19 @@ class HIPProposer(ProposerBase):
         and chi2 are written, the gradient is zero."""
         if model_only and pixel_weight is not None:
             raise ValueError("model_only is a chi2-mode option (no pixel_weight)")
-        stream = stream or self._stream()
-        n_tiles = self._load(proposal, patch, pixel_weight, stream,
-                             SLOT_MODEL_ONLY if model_only else 0)
-        if not outputs:
-            check(lib().fpx_evaluate_tiles(self._slot.data_ptr(), 1, None, n_tiles,
-                                           patch.max_gauss(), self.config, int(self.persistent),
-                                           self._queue.data_ptr(), stream.cuda_stream),
-                  "fpx_evaluate_tiles")
+        flags = SLOT_MODEL_ONLY if model_only else 0
+        if stream is None and not isinstance(proposal, np.ndarray):
+            # a device proposal needs only the stream's handle
+            handle = self._raw_stream()
+            n_tiles = self._load(proposal, patch, pixel_weight, None, flags)
         else:
-            check(lib().fpx_evaluate_batch(self._slot.data_ptr(), 1, None, n_tiles,
-                                           patch.max_gauss(), self.config, int(self.persistent),
-                                           self._queue.data_ptr(), stream.cuda_stream),
-                  "fpx_evaluate_batch")
+            stream = stream or self._stream()
+            handle = stream.cuda_stream
+            n_tiles = self._load(proposal, patch, pixel_weight, stream, flags)
+        fn = lib().fpx_evaluate_batch if outputs else lib().fpx_evaluate_tiles
+        check(fn(self._slot.data_ptr(), 1, None, n_tiles, patch.max_gauss(), self.config,
+                 int(self.persistent), self._queue.data_ptr(), handle),
+              "fpx_evaluate_batch" if outputs else "fpx_evaluate_tiles")
         return n_tiles
 
     def finalize(self):
@@ -314,8 +323,8 @@ class HIPProposer(ProposerBase):
         import torch
         patch.check_limits()
         n_src = int(patch.n_sources)
-        stream = stream or self._stream()
         if isinstance(proposal, np.ndarray):
+            stream = stream or self._stream()
             if len(proposal) != n_src:
                 raise ValueError(f"proposal has {len(proposal)} sources, patch has {n_src}")
             raw = np.ascontiguousarray(proposal, dtype=source_struct_dtype).view(np.uint8)
@@ -352,7 +361,6 @@ class HIPProposer(ProposerBase):
         if verbose:
             print(f"fpx launch: config {self.config}, {n_bands} bands, {n_sources} sources",
                   file=sys.stderr, flush=True)
-        stream = self._stream()
         chi2_np, resp_np = self._chi2_np, self._resp_np
         if isinstance(proposal, np.ndarray):
             if len(proposal) != n_sources:
@@ -364,9 +372,10 @@ class HIPProposer(ProposerBase):
                 self._host_prop[:n_sources] = proposal
                 check(lib().fpx_evaluate_host(ev, self._host_ptrs[0], n_sources,
                                               self._host_ptrs[1], self._host_ptrs[2],
-                                              stream.cuda_stream),
+                                              self._raw_stream()),
                       "fpx_evaluate_host")
                 return self._outputs(self._host_chi2, self._host_resp, patch, unpack)
+            stream = self._stream()
             prop = np.ascontiguousarray(proposal, dtype=source_struct_dtype)
             # host proposal: copy in, launch, copy out and wait in one native
             # call (fpx_evaluate_sync) through the pinned staging buffers
@@ -385,6 +394,7 @@ class HIPProposer(ProposerBase):
                                           out_bytes, self._out.data_ptr(), stream.cuda_stream),
                   "fpx_evaluate_sync")
         else:
+            stream = self._stream()
             self.launch(proposal, patch, stream=stream)
             with torch.cuda.stream(stream):
                 self._out_h.copy_(self._out, non_blocking=True)
