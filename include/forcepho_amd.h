@@ -270,14 +270,18 @@ int fpx_device_info(int32_t *n_cu, int32_t *lds_per_group, int32_t *clock_khz);
  *   max_gauss : n_sources x max over bands of n_psf_per_source (host copy)
  * Results: pchi2[b] (double, sum over the band's pixels of
  * (r*ierr - d*ierr)*(r*ierr + d*ierr)), pdchi2_dp[b] (float Response) and,
- * when patch->residual != NULL, residual[pix] = data - model. */
+ * when patch->residual != NULL, residual[pix] = data - model.  With no
+ * queue counter to find the last workgroup, this launch is two kernels (the
+ * tile kernel, then the output kernel); the entry points that take a queue
+ * write a one-slot launch's outputs from its last workgroup. */
 int fpx_evaluate_proposal(const FpxSlot *slot, int32_t n_tiles,
                           int32_t max_gauss, int32_t cfg, void *stream);
 
 /* One evaluation with host buffers, synchronous: proposal_bytes of the
  * proposal are copied from proposal_host to proposal_dev (the slot's
- * proposal buffer), the two kernels of fpx_evaluate_proposal (or the
- * persistent form, with queue) run, and out_bytes from out_dev (the slot's
+ * proposal buffer), the evaluation runs (with a queue: one kernel whose last
+ * workgroup writes the outputs, persistent or not; without: the tile kernel
+ * and the output kernel), and out_bytes from out_dev (the slot's
  * pchi2 / pdchi2_dp region) are copied to out_host before the call returns.
  * status_dev is the slot's status word or NULL; when it lies inside the
  * copied region its value is read from the copy, else it is copied back on
@@ -386,7 +390,9 @@ int fpx_evaluate_static(const FpxPatch *patch_dev, const FpxTile *tiles_dev, int
                         double *pchi2_dev, FpxResponse *pdchi2_dp_dev, int32_t cfg,
                         int32_t persistent, void *stream);
 
-/* Many evaluations in one launch (uneven patches, many chains).
+/* Many evaluations in one launch (uneven patches, many chains).  With one
+ * slot and a queue the tile kernel's last workgroup writes the outputs (one
+ * kernel); otherwise the output kernel follows the tile kernel.
  *   slots        : device FpxSlot[n_slots]
  *   item_offsets : device int32[n_slots + 1], prefix sums of the slots'
  *                  n_tiles (item_offsets[0] = 0); may be NULL if n_slots == 1
@@ -394,8 +400,13 @@ int fpx_evaluate_static(const FpxPatch *patch_dev, const FpxTile *tiles_dev, int
  *   max_gauss    : max over slots of n_sources x max n_psf_per_source
  *   persistent   : 0 = one workgroup per tile; 1 = a resident grid that
  *                  pulls tiles from the queue counter
- *   queue        : device uint32[2], zero before the first persistent launch
- *                  and left zero after each (ignored when persistent == 0) */
+ *   queue        : device uint32[2], zero before the first launch that uses
+ *                  it and left zero after each: the work counter of a
+ *                  persistent launch and the done counter that finds the
+ *                  last workgroup (persistent, or one slot with outputs);
+ *                  may be NULL for a launch that is not persistent (one
+ *                  slot then takes the output kernel too).  Launches that
+ *                  may run at the same time must not share one. */
 int fpx_evaluate_batch(const FpxSlot *slots, int32_t n_slots,
                        const int32_t *item_offsets, int32_t n_items,
                        int32_t max_gauss, int32_t cfg, int32_t persistent,
