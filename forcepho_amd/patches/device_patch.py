@@ -39,6 +39,35 @@ def device_bytes(arr, device):
     return torch.from_numpy(raw.copy()).to(device)
 
 
+# evaluators whose release was put off because a graph was being captured
+_DEFERRED = []
+
+
+def _capturing():
+    try:
+        import torch
+        return torch.cuda.is_available() and torch.cuda.is_current_stream_capturing()
+    except Exception:
+        return False
+
+
+def _release_handles(handles):
+    """fpx_release each handle, or keep them for later while the current
+    stream is capturing a graph (a release synchronises the device and frees
+    memory, neither of which a capture allows; a collected patch can trigger
+    one at any time)."""
+    if _capturing():
+        _DEFERRED.extend(handles)
+        return
+    pending = handles + _DEFERRED
+    del _DEFERRED[:]
+    for ev in pending:
+        try:
+            lib().fpx_release(ev)
+        except Exception:
+            pass
+
+
 class _BoundEvaluators:
     """FpxEvaluator handles (fpx_bind) of one upload of a patch's device
     struct, per kernel configuration: config -> (handle, n_tiles).  Released
@@ -51,11 +80,7 @@ class _BoundEvaluators:
 
     def release(self):
         handles, self.handles = self.handles, {}
-        for ev, _ in handles.values():
-            try:
-                lib().fpx_release(ev)
-            except Exception:
-                pass
+        _release_handles([ev for ev, _ in handles.values()])
 
 
 class DevicePatchMixin:

@@ -130,3 +130,24 @@ def test_optimize_one_band_matches_lstsq():
     fixed = rng.random(200)
     flux1, _ = optimize_one_band(X, w, f_true @ X + fixed, fixedX=fixed)
     np.testing.assert_allclose(flux1[:, 0], f_true, rtol=1e-10)
+
+
+def test_evaluator_release_waits_for_capture_to_end(monkeypatch):
+    """A bound evaluator released while a graph is being captured (a patch
+    collected mid-capture) is kept and released with the next release outside
+    the capture, since fpx_release synchronises the device."""
+    from forcepho_amd.patches import device_patch as dp
+    released = []
+
+    class FakeLib:
+        def fpx_release(self, ev):
+            released.append(ev)
+
+    monkeypatch.setattr(dp, "lib", lambda: FakeLib())
+    monkeypatch.setattr(dp, "_DEFERRED", [])
+    monkeypatch.setattr(dp, "_capturing", lambda: True)
+    dp._release_handles([1, 2])
+    assert released == [] and dp._DEFERRED == [1, 2]
+    monkeypatch.setattr(dp, "_capturing", lambda: False)
+    dp._release_handles([3])
+    assert sorted(released) == [1, 2, 3] and dp._DEFERRED == []
