@@ -13,6 +13,9 @@ plain launch each.  The line also carries:
             radii x 4 PSF Gaussians (1.416e8 pixel-Gaussian pairs), one
             launch per evaluation
   single_plain   1 band x 1 exposure x 64x64 px, 1 source, 9 x 3 Gaussians
+  split_tiles    patches of fewer tiles than workgroup slots (16-144 tiles of
+            540-1080 Gaussians), whole tiles against the split tiles that
+            HIPProposer.balance plans (pieces of a tile share its Gaussians)
   parity    the multi-band patch's gradients and chi2 against the reference
             CPU kernel and the float64 oracle (computed once, untimed)
   autograd, ensemble, valu_probe, cpu_baseline(_parallel)
@@ -60,6 +63,8 @@ def parse():
     ap.add_argument("--no-probe", action="store_true", help="skip the VALU rate probe")
     ap.add_argument("--no-single", action="store_true",
                     help="skip the single-patch plain measurement")
+    ap.add_argument("--no-split", action="store_true",
+                    help="skip the split-tile measurement on few-tile patches")
     ap.add_argument("--no-autograd", action="store_true",
                     help="skip the autograd + leapfrog measurement")
     ap.add_argument("--ensemble-patches", type=int, default=64,
@@ -606,6 +611,59 @@ def plain_measurement(args):
     return out, prop, patch, scene, proposal
 
 
+# few-tile patches of the split measurement: (description, spec_multi keywords)
+FEW_TILE_PATCHES = [
+    ("2 bands x 1 exposure x 64x64 px, 15 sources, 9 radii x 4 PSF Gaussians",
+     dict(side=64, n_bands=2, n_exp=1, n_sources=15, n_psf_comp=4)),
+    ("1 band x 1 exposure x 64x64 px, 10 sources, 9 radii x 6 PSF Gaussians",
+     dict(side=64, n_bands=1, n_exp=1, n_sources=10, n_psf_comp=6)),
+    ("4 bands x 1 exposure x 96x96 px, 20 sources, 9 radii x 6 PSF Gaussians",
+     dict(side=96, n_bands=4, n_exp=1, n_sources=20, n_psf_comp=6)),
+]
+
+
+def split_measurement(args, steps=200):
+    """Patches with fewer tiles than the persistent grid has workgroup slots:
+    one evaluation per launch (8 x 4 tiles, persistent), timed with whole
+    tiles and with the split table HIPProposer.balance plans (the heavy
+    tiles shared by several workgroups, each over its share of the
+    Gaussians); the results of the two compared once, untimed."""
+    import torch
+    from forcepho_amd import scenes
+    from forcepho_amd.patches import SimplePatch
+    from forcepho_amd.proposal import HIPProposer
+    out = []
+    for desc, kw in FEW_TILE_PATCHES:
+        spec = scenes.spec_multi(args.seed + 1, **kw)
+        patch, scene, _ = scenes.build_patch(spec, SimplePatch, scenes.render_on_device)
+        patch.return_residual = False
+        patch.send_to_device()
+        proposal = scene.get_proposal()
+        proposal["q"] *= np.float32(0.97)
+        raw = np.ascontiguousarray(proposal).view(np.uint8).reshape(-1)
+        pdev = torch.from_numpy(raw.copy()).cuda()
+        prop = HIPProposer(patch=patch, config=5, persistent=True)
+        row = {"workload": desc, "evals_per_step": patch.n_evals()}
+        results = {}
+        for label, split in (("whole", False), ("split", True)):
+            prop.balance(proposal, patch, split=split)
+            table = patch.split_tiles(prop.pix_per_tile) if split else None
+            n_items = table[1] if table is not None else patch.device_tiles(prop.pix_per_tile)[1]
+            results[label] = prop.evaluate_proposal(pdev, patch=patch)
+            for _ in range(10):
+                prop.launch(pdev, patch)
+            t = time_region(lambda: prop.launch(pdev, patch), steps)
+            row[label] = {"items": int(n_items), "us": t * 1e6,
+                          "evals_per_s": row["evals_per_step"] / t}
+        row["speedup"] = row["whole"]["us"] / row["split"]["us"]
+        (c0, g0), (c1, g1) = results["whole"], results["split"]
+        scale = np.abs(g0).max(axis=(1, 2), keepdims=True) + 1e-30
+        row["split_vs_whole"] = {"chi2_rel_max": float(np.max(np.abs(c1 - c0) / np.abs(c0))),
+                                 "grad_rel_of_band_max": float(np.max(np.abs(g1 - g0) / scale))}
+        out.append(row)
+    return out
+
+
 def parity_measurement():
     """Parity of the multi-band workload at the north-star bar, once and
     untimed: the patch of tests/golden/make_fullsize.py (the benchmark's
@@ -768,6 +826,7 @@ def main():
         head["vs_plain"]["speedup_vs_single_patch_plain_launches"] = (
             head["value"] / single["launch_evals_per_s"])
     parity = None if args.no_parity else parity_measurement()
+    split = None if args.no_split else split_measurement(args)
     ensemble = ensemble_measurement(args) if args.ensemble_patches > 0 else None
     autograd = None if args.no_autograd else autograd_measurement(args, patch, scene)
     line = {
@@ -788,6 +847,7 @@ def main():
         "vs_plain": head["vs_plain"],
         "plain": plain,
         "single_plain": single,
+        "split_tiles": split,
         "parity": parity,
         "autograd": autograd,
         "ensemble": ensemble,

@@ -5,7 +5,8 @@ The hot path (``Proposer.evaluate_proposal``) runs hand-written HIP kernels in
 the reference's host API and add the batched and device-resident forms:
 
 * ``patches`` (packing: stamps, FITS images, PSF stores; device residency),
-  ``proposal`` (the proposer, struct dtypes), ``sources`` (Scene, Galaxy and
+  ``proposal`` (the proposer, struct dtypes), ``tile_split`` (tiles shared by
+  several workgroups: the planner), ``sources`` (Scene, Galaxy and
   the mixture-amplitude models), ``model`` (Posterior, FastPosterior,
   BoundedTransform), ``kernel_limits``;
 * ``batch`` (many patches and chains in one persistent launch),

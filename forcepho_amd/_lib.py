@@ -13,7 +13,7 @@ import numpy as np
 from .build import LIB_PATH
 
 __all__ = ["lib", "check", "FpxError", "TILE_DTYPE", "SLOT_DTYPE", "SLOT_MODEL_ONLY",
-           "STATUS_LIMIT", "STATUS_RADIUS_BIN",
+           "STATUS_LIMIT", "STATUS_RADIUS_BIN", "STATUS_SPLIT_WAIT", "MAX_TILE_PARTS",
            "STRUCT_PATCH", "STRUCT_SOURCE", "STRUCT_RESPONSE", "STRUCT_PSFGAUSS",
            "STRUCT_TILE", "STRUCT_SLOT", "STRUCT_AMPMODEL", "EXPORTED_SYMBOLS",
            "AmplitudeModel", "AMP_LOGNORMAL", "AMP_SPLINE", "MAX_SPLINE_DEGREE"]
@@ -22,17 +22,20 @@ __all__ = ["lib", "check", "FpxError", "TILE_DTYPE", "SLOT_DTYPE", "SLOT_MODEL_O
  STRUCT_AMPMODEL) = range(7)
 
 # FpxTile / FpxSlot as numpy dtypes (include/forcepho_amd.h)
-TILE_DTYPE = np.dtype([("exposure", np.int32), ("band", np.int32),
-                       ("pix_start", np.int32), ("pix_count", np.int32)])
+TILE_DTYPE = np.dtype([("exposure", np.int32), ("band", np.int16), ("part", np.uint8),
+                       ("parts", np.uint8), ("pix_start", np.int32), ("pix_count", np.int32)])
 SLOT_DTYPE = np.dtype([("patch", np.uintp), ("proposal", np.uintp), ("tiles", np.uintp),
                        ("pixel_weight", np.uintp), ("pchi2", np.uintp),
                        ("pdchi2_dp", np.uintp), ("accum", np.uintp), ("status", np.uintp),
                        ("n_bands", np.int32), ("n_tiles", np.int32),
-                       ("flags", np.int32), ("reserved", np.int32)], align=True)
+                       ("flags", np.int32), ("reserved", np.int32), ("split", np.uintp)],
+                      align=True)
 
 SLOT_MODEL_ONLY = 1    # FpxSlot.flags: model pass only (FPX_SLOT_MODEL_ONLY)
 STATUS_LIMIT = 1       # FpxSlot.status bit: a tile broke a size limit (FPX_STATUS_LIMIT)
 STATUS_RADIUS_BIN = 2  # FpxSlot.status bit: radius bin outside [0, n_radii)
+STATUS_SPLIT_WAIT = 4  # FpxSlot.status bit: a split tile's piece timed out waiting
+MAX_TILE_PARTS = 16    # FPX_MAX_TILE_PARTS: pieces of one split tile
 
 AMP_LOGNORMAL, AMP_SPLINE = 0, 1   # FpxAmplitudeModel.kind
 MAX_SPLINE_DEGREE = 5              # FPX_MAX_SPLINE_DEGREE
@@ -58,6 +61,7 @@ _SIGNATURES = {
     "fpx_last_error": (ctypes.c_char_p, []),
     "fpx_struct_size": (_c_size, [_c_int]),
     "fpx_accum_doubles": (_c_size, [_c_int]),
+    "fpx_split_bytes": (_c_size, [_c_int, _c_int]),
     "fpx_plan_tiles": (_c_int, [_p, _p, _p, _p, _c_int, _c_int, _p, _c_int]),
     "fpx_n_configs": (_c_int, []),
     "fpx_config_info": (_c_int, [_c_int, _p, _p, _p]),

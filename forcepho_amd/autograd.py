@@ -120,9 +120,10 @@ class PatchModel:
         residual buffer (``swap_on_device`` exchanges it with the data), the
         tile table and the kernel configuration."""
         p = self.proposer
-        tiles = self.patch.device_tiles(p.pix_per_tile)[0]
+        tiles, _, split_buf = p.tile_table(self.patch)
         return (self.patch.device_patch.data_ptr(), self.patch.device_ptrs["residual"].data_ptr(),
-                tiles.data_ptr(), p.config, p.persistent)
+                tiles.data_ptr(), 0 if split_buf is None else split_buf.data_ptr(), p.config,
+                p.persistent)
 
     def _call(self, body, *inputs):
         """Run ``body`` on device copies of ``inputs``: directly, or by
@@ -148,7 +149,7 @@ class PatchModel:
             with torch.cuda.graph(graph):
                 out = body(*static)
             # the buffers the graph reads by address stay referenced with it
-            refs = (self.proposer._slot, self.patch.device_tiles(self.proposer.pix_per_tile)[0])
+            refs = (self.proposer._slot, *self.proposer.tile_table(self.patch))
             rec = dict(key=key, graph=graph, inputs=static, out=out, refs=refs)
             self._graphs[body.__name__] = rec
         rec["graph"].replay()

@@ -61,13 +61,25 @@ static_assert(offsetof(FpxSource, damplitude_dnsersic) == 224, "Source.damplitud
 static_assert(sizeof(FpxResponse) == 840, "Response layout (proposal.cu:33-39)");
 static_assert(sizeof(FpxPSFSourceGaussian) == 28, "PSFSourceGaussian layout (patch.cc:103-119)");
 static_assert(sizeof(FpxTile) == 16, "tile layout");
-static_assert(sizeof(FpxSlot) == 80, "slot layout");
+static_assert(offsetof(FpxTile, part) == 6 && offsetof(FpxTile, parts) == 7, "tile parts");
+static_assert(sizeof(FpxSlot) == 88, "slot layout");
+static_assert(offsetof(FpxSlot, split) == 80, "FpxSlot.split offset");
 
 namespace fpx {
 
 constexpr int kAcc = 1 + MAXSOURCES * NPARAMS;        // doubles per band accumulator
 // dx^T F dx > MAX_EXP_ARG  <=>  log2 G < kLog2Min (log2 G = kExp2Scale dx^T F dx)
 constexpr float kLog2Min = kExp2Scale * (float)MAX_EXP_ARG;
+
+// Split buffer of a slot (FpxSlot.split): a uint32 counter per entry of the
+// tile table, padded to 256 bytes, then a row of 64*PPL floats per entry,
+// the partial model of the piece at that entry.  A split tile's counter is
+// the one at its piece 0.
+__host__ __device__ constexpr size_t split_counter_bytes(int n_tiles) {
+  return ((size_t)n_tiles * sizeof(uint32_t) + 255) / 256 * 256;
+}
+// time limit of a piece's wait for the others, in s_memrealtime ticks (100 MHz)
+constexpr unsigned long long kSplitWaitTicks = 100000ull * FPX_SPLIT_WAIT_MS;
 
 // ------------------------------------------------------------------------
 // wavefront helpers
@@ -540,10 +552,47 @@ __device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int wa
 // One tile: returns false when the exposure's n_psf breaks a size limit
 // (FPX_STATUS_LIMIT is then set and the tile contributes nothing), true
 // otherwise, the tile's sums being in the wave rows.
+// Arrive at a split tile's counter and wait until all `parts` pieces have
+// arrived (thread 0 polls; the other wavefronts wait at the barrier).  The
+// rows and the counter are only ever accessed with device-scope atomic loads
+// and stores, which are coherent across the XCDs by themselves; what the
+// exchange needs is order: every wavefront waits until its row stores are
+// performed before the barrier, so they precede the arrival, and the rows
+// are read only after the count is seen (a control dependency).  A release /
+// acquire pair would add a writeback and an invalidation of the whole L2 of
+// the XCD for every piece, which the other workgroups there pay for.  The
+// wait is bounded by kSplitWaitTicks: returns false when the time ran out.
+__device__ bool split_arrive_and_wait(uint32_t *counter, int parts) {
+  __shared__ int s_split_ok;
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+    int ok = 1;
+    while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+           (uint32_t)parts) {
+      if (__builtin_amdgcn_s_memrealtime() - t0 > kSplitWaitTicks) {
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    s_split_ok = ok;
+  }
+  __syncthreads();
+  return s_split_ok != 0;
+}
+
+// Results of tile_work: the tile was left out (a size limit), evaluated, or
+// evaluated by a piece that arrived at its split counter (and must leave it).
+enum TileResult { kTileLeftOut = 0, kTileDone = 1, kTileArrived = 2 };
+
 template <int W, int PPL>
-__device__ bool tile_work(const FpxSlot &slot, const FpxTile &tile, const PatchView &P,
-                          gptr<FpxSource> props, int e, int band, int n_src, int n_radii,
-                          int gauss_cap, int index_cap, unsigned char *smem) {
+__device__ TileResult tile_work(const FpxSlot &slot, const FpxTile &tile, const PatchView &P,
+                                gptr<FpxSource> props, int e, int band, int n_src,
+                                int n_radii, int gauss_cap, int index_cap, int local, int part,
+                                int parts, unsigned char *smem) {
   // The tile's Gaussians that reach it are processed in chunks of gauss_cap
   // records (one chunk when they all fit).
   const int tid = threadIdx.x;
@@ -599,7 +648,7 @@ __device__ bool tile_work(const FpxSlot &slot, const FpxTile &tile, const PatchV
 
   if (!(n_psf >= 1 && n_psf <= FPX_MAX_PSF_PER_SOURCE && n_src * n_psf <= index_cap)) {
     if (tid == 0 && slot.status != nullptr) atomicOr(slot.status, (uint32_t)FPX_STATUS_LIMIT);
-    return false;
+    return kTileLeftOut;
   }
   // the cull's first PSF entry, requested before the geometry is computed
   const int lg0 = min(tid, n_src * n_psf - 1);
@@ -640,8 +689,12 @@ __device__ bool tile_work(const FpxSlot &slot, const FpxTile &tile, const PatchV
 
   const int n_surv = cull_tile<W>(P.psfgauss, pstart, n_src, n_psf, n_radii, box, l, wave, lane,
                                   slot.status, psf0);
-  const int n_chunks = (n_surv + gauss_cap - 1) / gauss_cap;
-  auto chunk_size = [&](int c) { return min(gauss_cap, n_surv - c * gauss_cap); };
+  // this piece's run of the survivors (all of them when the tile is whole)
+  const int r_first = (n_surv * part) / parts;
+  const int r_count = (n_surv * (part + 1)) / parts - r_first;
+  const int n_chunks = (r_count + gauss_cap - 1) / gauss_cap;
+  auto chunk_size = [&](int c) { return min(gauss_cap, r_count - c * gauss_cap); };
+  bool arrived = false;
   if (!vjp) {
     float m[PPL];
 #pragma unroll
@@ -650,19 +703,57 @@ __device__ bool tile_work(const FpxSlot &slot, const FpxTile &tile, const PatchV
     // LDS for the gradient pass, which then rebuilds only the others
     for (int c = n_chunks - 1; c >= 0; --c) {
       const int n_act = chunk_size(c);
-      build_records<W>(P.psfgauss, props, pstart, gcal, band, n_psf, c * gauss_cap, n_act, l);
+      build_records<W>(P.psfgauss, props, pstart, gcal, band, n_psf,
+                       r_first + c * gauss_cap, n_act, l);
       model_pass<PPL>(l.recs, wave, W, n_act, xp, yp, m);
     }
 #pragma unroll
     for (int i = 0; i < PPL; ++i) l.part[(wave * PPL + i) * 64 + lane] = m[i];
     __syncthreads();
+    float model_at[PPL];
+    if (parts > 1) {
+      // A piece of a split tile: its partial model (the sum over its
+      // wavefronts) to its row of the split buffer, then, once every piece
+      // has arrived, the model is the sum of the pieces' rows in piece order
+      // (the same sum whichever piece finishes first).
+      constexpr int kPix = 64 * PPL;
+      uint32_t *counter = reinterpret_cast<uint32_t *>(slot.split) + (local - part);
+      const float *rows = reinterpret_cast<const float *>(
+          reinterpret_cast<const unsigned char *>(slot.split) + split_counter_bytes(slot.n_tiles));
+      float *mine = const_cast<float *>(rows) + (size_t)local * kPix;
+      for (int j = tid; j < kPix; j += W * 64) {
+        float sum = 0.0f;
+#pragma unroll
+        for (int w = 0; w < W; ++w) sum += l.part[(w * PPL + (j >> 6)) * 64 + (j & 63)];
+        __hip_atomic_store(mine + j, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (!split_arrive_and_wait(counter, parts) && tid == 0 && slot.status != nullptr)
+        atomicOr(slot.status, (uint32_t)FPX_STATUS_SPLIT_WAIT);
+      arrived = true;
+      const float *first_row = rows + (size_t)(local - part) * kPix;
+#pragma unroll
+      for (int i = 0; i < PPL; ++i) {
+        float sum = 0.0f;
+        for (int k = 0; k < parts; ++k)
+          sum += __hip_atomic_load(first_row + (size_t)k * kPix + i * 64 + lane, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        model_at[i] = sum;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < PPL; ++i) {
+        float sum = 0.0f;
+#pragma unroll
+        for (int w = 0; w < W; ++w) sum += l.part[(w * PPL + i) * 64 + lane];
+        model_at[i] = sum;
+      }
+    }
     double chi2 = 0.0;
-    FPX_GLOBAL_AS float *resid = P.residual;
+    // piece 0 of a split tile writes the residual and adds the chi2
+    FPX_GLOBAL_AS float *resid = part == 0 ? P.residual : nullptr;
 #pragma unroll
     for (int i = 0; i < PPL; ++i) {
-      float model = 0.0f;
-#pragma unroll
-      for (int w = 0; w < W; ++w) model += l.part[(w * PPL + i) * 64 + lane];
+      const float model = model_at[i];
       const float d = dat[i];
       const float ie = ier[i];
       const float r = d - model;
@@ -672,15 +763,16 @@ __device__ bool tile_work(const FpxSlot &slot, const FpxTile &tile, const PatchV
       chi2 += ((double)ri - (double)di) * ((double)ri + (double)di);
       w2[i] = -2.0f * (ri * ie);
     }
-    if (wave == 0) {
+    if (wave == 0 && part == 0) {
       chi2 = wave_sum(chi2);
       if (lane == 0) row_chi2(l.rows) += chi2;
     }
   }
+  const TileResult done = arrived ? kTileArrived : kTileDone;
 
   // a model-only slot stops at its residual and chi2 (the flag is the same
   // for the whole workgroup, and process_tile's barrier follows)
-  if (!vjp && (__builtin_amdgcn_readfirstlane(slot.flags) & FPX_SLOT_MODEL_ONLY)) return true;
+  if (!vjp && (__builtin_amdgcn_readfirstlane(slot.flags) & FPX_SLOT_MODEL_ONLY)) return done;
 
   bool wave_has_weight = false;
 #pragma unroll
@@ -692,11 +784,12 @@ __device__ bool tile_work(const FpxSlot &slot, const FpxTile &tile, const PatchV
   for (int c = 0; c < n_chunks; ++c) {
     const int n_act = chunk_size(c);
     if (c != 0 || vjp)
-      build_records<W>(P.psfgauss, props, pstart, gcal, band, n_psf, c * gauss_cap, n_act, l);
+      build_records<W>(P.psfgauss, props, pstart, gcal, band, n_psf, r_first + c * gauss_cap,
+                       n_act, l);
     if (wave_has_weight)
       grad_pass<PPL>(l.recs, wave, W, n_act, xp, yp, w2, l.cwl, wrow, lane);
   }
-  return true;
+  return done;
 }
 
 // Add the tile's per-wave rows to the slot accumulator (band < 0: nothing
@@ -808,7 +901,10 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalize_kernel(
   if (j < nb * kAcc) slot_output(slot, j);
 }
 
-template <int W, int PPL>
+// kSplit: the kernel variant that evaluates split tiles (FpxTile.parts >=
+// 2).  The other variant leaves such a tile out (FPX_STATUS_LIMIT), and its
+// code has no trace of the pieces' bookkeeping, which costs registers.
+template <int W, int PPL, bool kSplit>
 __device__ void process_tile(const FpxSlot &slot, int local, int gauss_cap, int index_cap,
                              unsigned char *smem) {
   constexpr int kPix = 64 * PPL;
@@ -822,21 +918,40 @@ __device__ void process_tile(const FpxSlot &slot, int local, int gauss_cap, int 
   const int n_src = P.n_sources;
   const int n_radii = P.n_radii;
   const int pix_count = tile.pix_count;
+  const int parts = kSplit && tile.parts > 1 ? (int)tile.parts : 1;
+  const int part = parts > 1 ? (int)tile.part : 0;
   // A tile whose sizes break the kernel's assumptions contributes nothing and
   // sets FPX_STATUS_LIMIT in the slot's status word, which makes the output
   // kernel write NaN chi2 (every lane sees the same values, so the whole
   // workgroup agrees).  The sizes read here come from the tile record and the
-  // patch header; tile_work checks the exposure's n_psf.
+  // patch header; tile_work checks the exposure's n_psf.  A piece of a split
+  // tile needs the split buffer and its whole run of entries in the table.
   bool ok = band >= 0 && band < slot.n_bands && band < MAXBANDS && n_src >= 1 &&
             n_src <= MAXSOURCES && n_radii >= 1 && n_radii <= MAXRADII && pix_count >= 0 &&
-            pix_count <= kPix;
+            pix_count <= kPix &&
+            (kSplit ? (parts == 1 ||
+                       (parts <= FPX_MAX_TILE_PARTS && part < parts && part <= local &&
+                        slot.split != nullptr && local - part + parts <= slot.n_tiles))
+                    : tile.parts <= 1);
   if (!ok && threadIdx.x == 0 && slot.status != nullptr)
     atomicOr(slot.status, (uint32_t)FPX_STATUS_LIMIT);
+  TileResult res = kTileLeftOut;
   if (ok)
-    ok = tile_work<W, PPL>(slot, tile, P, props, e, band, n_src, n_radii, gauss_cap, index_cap,
-                           smem);
+    res = tile_work<W, PPL>(slot, tile, P, props, e, band, n_src, n_radii, gauss_cap, index_cap,
+                            local, part, parts, smem);
   __syncthreads();
-  finish_tile<W>(slot, local, ok ? band : -1, ok ? n_src : 0, smem, gauss_cap, PPL);
+  // A piece leaves its split counter after every wavefront has read the
+  // pieces' rows (the barrier above); the piece that makes the count 2 *
+  // parts is the last to leave and sets it back to zero for the next launch.
+  if (res == kTileArrived && threadIdx.x == 0) {
+    uint32_t *counter = reinterpret_cast<uint32_t *>(slot.split) + (local - part);
+    const uint32_t before =
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (before == 2u * (uint32_t)parts - 1u)
+      __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  const bool used = res != kTileLeftOut;
+  finish_tile<W>(slot, local, used ? band : -1, used ? n_src : 0, smem, gauss_cap, PPL);
 }
 
 // Launch modes (bits of the kernel's `mode`).  kModePersistent: a resident
@@ -862,7 +977,7 @@ constexpr int kModeFlag = 4;
 template <int W, int PPL>
 constexpr int kMinWavesPerEU = (PPL <= 4 && W <= 8) ? 4 : 1;
 
-template <int W, int PPL, bool kOne>
+template <int W, int PPL, bool kOne, bool kSplit>
 __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(kMinWavesPerEU<W, PPL>)))
 void chi2_grad_kernel(
     const FpxSlot *__restrict__ slots, FpxSlot one, int n_slots,
@@ -895,7 +1010,7 @@ void chi2_grad_kernel(
       local = item - as_const(item_offsets)[si];
     }
     const FpxSlot slot = kOne ? one : load_const(slots, si);
-    process_tile<W, PPL>(slot, local, gauss_cap, index_cap, smem);
+    process_tile<W, PPL, kSplit>(slot, local, gauss_cap, index_cap, smem);
   }
   if (mode == 0) return;
   // Count the workgroup done.  With outputs or a flag, every wave first waits
@@ -968,6 +1083,7 @@ __global__ __launch_bounds__(256) void census_kernel(
   const int band = __builtin_amdgcn_readfirstlane(tile.band);
   const int n_src = __builtin_amdgcn_readfirstlane(P->n_sources);
   if (band < 0 || band >= slot.n_bands || band >= MAXBANDS) return;
+  if (tile.parts > 1 && tile.part != 0) return;  // a split tile is counted at its piece 0
   const int n_psf = __builtin_amdgcn_readfirstlane(P->n_psf_per_source[band]);
   if (n_src < 1 || n_src > MAXSOURCES || n_psf < 1 || n_src * n_psf > index_cap) return;
   const int n = n_src * n_psf;
@@ -1056,7 +1172,11 @@ std::string status_message(uint32_t status) {
     m += " a tile broke a size limit (n_sources, n_psf_per_source, n_radii, band, pixel count, "
          "or max_gauss below n_sources x n_psf_per_source);";
   if (status & FPX_STATUS_RADIUS_BIN) m += " a PSF entry's radius bin is outside [0, n_radii);";
-  if (status & ~(uint32_t)(FPX_STATUS_LIMIT | FPX_STATUS_RADIUS_BIN)) m += " unknown status bits;";
+  if (status & FPX_STATUS_SPLIT_WAIT)
+    m += " a piece of a split tile timed out waiting for the others (not a persistent launch, "
+         "or the split buffer was not zero);";
+  if (status & ~(uint32_t)(FPX_STATUS_LIMIT | FPX_STATUS_RADIUS_BIN | FPX_STATUS_SPLIT_WAIT))
+    m += " unknown status bits;";
   return m;
 }
 
@@ -1070,22 +1190,26 @@ std::string status_message(uint32_t status) {
 // per SIMD (16 per CU) fit in the 160 KiB of a CU; a tile whose surviving
 // Gaussians exceed the records that fit is processed in chunks.
 constexpr int kLdsPerWave = 10 * 1024;
-// static LDS of the chi2 kernel (the queue item), for the host-only size
-// query; launches use the kernel's reported size
+// static LDS of the chi2 kernel (the queue item and the split-tile wait
+// flag), for the host-only size query; launches use the kernel's reported
+// size
 constexpr int kStaticLds = 16;
 
-// A tile shape and its two kernels: fn[0] reads a slot table, fn[1] takes
-// one slot by value.
+// A tile shape and its four kernels, fn[2 * split + one]: one = 0 reads a
+// slot table, 1 takes one slot by value; split = 1 evaluates split tiles.
+constexpr int kVariants = 4;
 struct KernelCfg {
   int waves;
   int ppl;
-  const void *fn[2];
+  const void *fn[kVariants];
 };
 
 #define FPX_CFG(W, P)                                                               \
   {W, P,                                                                            \
-   {reinterpret_cast<const void *>(&fpx::chi2_grad_kernel<W, P, false>),            \
-    reinterpret_cast<const void *>(&fpx::chi2_grad_kernel<W, P, true>)}}
+   {reinterpret_cast<const void *>(&fpx::chi2_grad_kernel<W, P, false, false>),     \
+    reinterpret_cast<const void *>(&fpx::chi2_grad_kernel<W, P, true, false>),      \
+    reinterpret_cast<const void *>(&fpx::chi2_grad_kernel<W, P, false, true>),      \
+    reinterpret_cast<const void *>(&fpx::chi2_grad_kernel<W, P, true, true>)}}
 const KernelCfg kConfigs[] = {
     FPX_CFG(4, 1), FPX_CFG(4, 2), FPX_CFG(4, 4), FPX_CFG(8, 8),
     FPX_CFG(16, 4), FPX_CFG(8, 4), FPX_CFG(2, 4), FPX_CFG(4, 8),
@@ -1100,14 +1224,14 @@ struct DeviceState {
   int n_cu = 0;
   int lds_max = 0;
   int clock_khz = 0;
-  // per kernel, index 2 * configuration + (one slot by value)
-  size_t lds_set[2 * kNumConfigs] = {};
+  // per kernel, index kVariants * configuration + variant
+  size_t lds_set[kVariants * kNumConfigs] = {};
   // resident workgroups per CU, cached per kernel and LDS size, so that a
   // launch recorded into a graph issues no occupancy query
-  size_t occ_lds[2 * kNumConfigs] = {};
-  int occ_per_cu[2 * kNumConfigs] = {};
+  size_t occ_lds[kVariants * kNumConfigs] = {};
+  int occ_per_cu[kVariants * kNumConfigs] = {};
   size_t census_lds = 0;
-  int static_lds[2 * kNumConfigs];  // static LDS of each kernel, -1 until queried
+  int static_lds[kVariants * kNumConfigs];  // static LDS of each kernel, -1 until queried
   DeviceState() {
     for (int &v : static_lds) v = -1;
   }
@@ -1160,11 +1284,11 @@ int pick_gauss_cap(const KernelCfg &kc, int max_gauss, int lds_max, int static_l
   return (int)std::max<long>(0, std::min<long>(max_gauss, fit));
 }
 
-// Workgroups of configuration cfg with `lds` bytes of dynamic LDS that fit
-// one CU at once (cached, so that a launch recorded into a graph makes no
-// occupancy query).
-int groups_per_cu(DeviceState *st, int cfg, int one, size_t lds, int *per_cu) {
-  const int k = 2 * cfg + one;
+// Workgroups of configuration cfg, kernel variant v, with `lds` bytes of
+// dynamic LDS that fit one CU at once (cached, so that a launch recorded into
+// a graph makes no occupancy query).
+int groups_per_cu(DeviceState *st, int cfg, int v, size_t lds, int *per_cu) {
+  const int k = kVariants * cfg + v;
   {
     std::lock_guard<std::mutex> lock(g_mu);
     if (st->occ_lds[k] == lds && st->occ_per_cu[k] > 0) {
@@ -1173,7 +1297,7 @@ int groups_per_cu(DeviceState *st, int cfg, int one, size_t lds, int *per_cu) {
     }
   }
   int n = 0;
-  FPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kConfigs[cfg].fn[one],
+  FPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kConfigs[cfg].fn[v],
                                                        kConfigs[cfg].waves * 64, lds));
   if (n < 1) n = 1;
   std::lock_guard<std::mutex> lock(g_mu);
@@ -1183,17 +1307,18 @@ int groups_per_cu(DeviceState *st, int cfg, int one, size_t lds, int *per_cu) {
   return FPX_OK;
 }
 
-// Dynamic LDS bytes and records per chunk of a launch of configuration cfg.
-int launch_lds(DeviceState *st, int cfg, int one, int max_gauss, size_t *lds, int *cap,
+// Dynamic LDS bytes and records per chunk of a launch of configuration cfg,
+// kernel variant v.
+int launch_lds(DeviceState *st, int cfg, int v, int max_gauss, size_t *lds, int *cap,
                int *index_cap) {
   const KernelCfg &kc = kConfigs[cfg];
-  const int k = 2 * cfg + one;
+  const int k = kVariants * cfg + v;
   int static_lds = 0;
   {
     std::lock_guard<std::mutex> lock(g_mu);
     if (st->static_lds[k] < 0) {
       hipFuncAttributes attr;
-      FPX_HIP(hipFuncGetAttributes(&attr, kc.fn[one]));
+      FPX_HIP(hipFuncGetAttributes(&attr, kc.fn[v]));
       st->static_lds[k] = (int)attr.sharedSizeBytes;
     }
     static_lds = st->static_lds[k];
@@ -1211,12 +1336,22 @@ int launch_lds(DeviceState *st, int cfg, int one, int max_gauss, size_t *lds, in
 // tile kernel's last workgroup when the launch holds one slot and has a
 // queue counter (kModeOutputs: one kernel), else finalize_kernel launched
 // after it.
+//
+// launch_flags: FPX_LAUNCH_PERSISTENT, FPX_LAUNCH_SPLIT (the kernel variant
+// that evaluates split tiles; its pieces wait for each other, which only a
+// persistent launch guarantees to end well).
 int launch(const FpxSlot *slots, const FpxSlot *one, int32_t n_slots,
            const int32_t *item_offsets, int32_t n_items, int32_t max_gauss, int32_t cfg,
-           int32_t persistent, uint32_t *queue, void *stream, bool finalize_outputs = true,
+           int32_t launch_flags, uint32_t *queue, void *stream, bool finalize_outputs = true,
            uint32_t *flag = nullptr, uint32_t seq = 0) {
   int rc = check_cfg(cfg);
   if (rc) return rc;
+  if (launch_flags & ~(int32_t)(FPX_LAUNCH_PERSISTENT | FPX_LAUNCH_SPLIT))
+    return set_err(FPX_E_ARG, "unknown launch flags");
+  const bool persistent = (launch_flags & FPX_LAUNCH_PERSISTENT) != 0;
+  const bool split = (launch_flags & FPX_LAUNCH_SPLIT) != 0;
+  if (split && !persistent)
+    return set_err(FPX_E_ARG, "split tiles need a persistent launch (FPX_LAUNCH_PERSISTENT)");
   if ((!slots && !one) || n_slots <= 0 || n_items < 0)
     return set_err(FPX_E_ARG, "empty or invalid slot list");
   if (!slots && n_slots != 1) return set_err(FPX_E_ARG, "a slot passed by value is one slot");
@@ -1241,17 +1376,17 @@ int launch(const FpxSlot *slots, const FpxSlot *one, int32_t n_slots,
     return FPX_OK;
   }
   const KernelCfg &kc = kConfigs[cfg];
-  const int v = slots ? 0 : 1;
+  const int v = (split ? 2 : 0) + (slots ? 0 : 1);
   int cap = 0, index_cap = 0;
   size_t lds = 0;
   rc = launch_lds(st, cfg, v, max_gauss, &lds, &cap, &index_cap);
   if (rc) return rc;
   {
     std::lock_guard<std::mutex> lock(g_mu);
-    if (st->lds_set[2 * cfg + v] < lds) {
+    if (st->lds_set[kVariants * cfg + v] < lds) {
       FPX_HIP(hipFuncSetAttribute(kc.fn[v], hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds));
-      st->lds_set[2 * cfg + v] = lds;
+      st->lds_set[kVariants * cfg + v] = lds;
     }
   }
   const int threads = kc.waves * 64;
@@ -1284,8 +1419,8 @@ int report_hip_error(hipError_t e, const char *where) { return hip_err(e, where)
 int launch_one(const FpxSlot &slot, int32_t n_tiles, int32_t max_gauss, int32_t cfg,
                int32_t persistent, uint32_t *queue, void *stream, bool finalize_outputs,
                uint32_t *flag, uint32_t seq) {
-  return launch(nullptr, &slot, 1, nullptr, n_tiles, max_gauss, cfg, persistent ? 1 : 0, queue,
-                stream, finalize_outputs, flag, seq);
+  return launch(nullptr, &slot, 1, nullptr, n_tiles, max_gauss, cfg,
+                persistent ? FPX_LAUNCH_PERSISTENT : 0, queue, stream, finalize_outputs, flag, seq);
 }
 std::string status_text(uint32_t status) { return status_message(status); }
 int config_pixels(int32_t cfg) {
@@ -1336,7 +1471,9 @@ int fpx_plan_tiles(const int32_t *exposure_start, const int32_t *exposure_N,
           if (count >= max_tiles) return set_err(FPX_E_ARG, "tiles_out too small");
           FpxTile &t = tiles_out[count];
           t.exposure = e;
-          t.band = b;
+          t.band = (int16_t)b;
+          t.part = 0;
+          t.parts = 0;
           t.pix_start = exposure_start[e] + off;
           t.pix_count = std::min(pix_per_tile, n - off);
         }
@@ -1349,6 +1486,12 @@ int fpx_plan_tiles(const int32_t *exposure_start, const int32_t *exposure_N,
 }
 
 int fpx_n_configs(void) { return kNumConfigs; }
+
+size_t fpx_split_bytes(int32_t n_tiles, int32_t cfg) {
+  if (n_tiles <= 0 || cfg < 0 || cfg >= kNumConfigs) return 0;
+  return fpx::split_counter_bytes(n_tiles) +
+         (size_t)n_tiles * 64 * kConfigs[cfg].ppl * sizeof(float);
+}
 
 int fpx_config_info(int32_t cfg, int32_t *waves_per_group, int32_t *pixels_per_lane,
                     int32_t *pixels_per_tile) {
@@ -1425,7 +1568,7 @@ int fpx_evaluate_sync(const FpxSlot *slot, int32_t n_tiles, int32_t max_gauss, i
   const hipStream_t s = (hipStream_t)stream;
   if (proposal_bytes > 0)
     FPX_HIP(hipMemcpyAsync(proposal_dev, proposal_host, proposal_bytes, hipMemcpyHostToDevice, s));
-  const int rc = launch(slot, nullptr, 1, nullptr, n_tiles, max_gauss, cfg, persistent ? 1 : 0,
+  const int rc = launch(slot, nullptr, 1, nullptr, n_tiles, max_gauss, cfg, persistent,
                         queue, stream);
   if (rc) return rc;
   FPX_HIP(hipMemcpyAsync(out_host, out_dev, out_bytes, hipMemcpyDeviceToHost, s));
@@ -1450,14 +1593,14 @@ int fpx_evaluate_batch(const FpxSlot *slots, int32_t n_slots, const int32_t *ite
                        int32_t n_items, int32_t max_gauss, int32_t cfg, int32_t persistent,
                        uint32_t *queue, void *stream) {
   return launch(slots, nullptr, n_slots, item_offsets, n_items, max_gauss, cfg,
-                persistent ? 1 : 0, queue, stream);
+                persistent, queue, stream);
 }
 
 int fpx_evaluate_tiles(const FpxSlot *slots, int32_t n_slots, const int32_t *item_offsets,
                        int32_t n_items, int32_t max_gauss, int32_t cfg, int32_t persistent,
                        uint32_t *queue, void *stream) {
   return launch(slots, nullptr, n_slots, item_offsets, n_items, max_gauss, cfg,
-                persistent ? 1 : 0, queue, stream, false);
+                persistent, queue, stream, false);
 }
 
 int fpx_finalize(const FpxSlot *slots, int32_t n_slots, void *stream) {

@@ -208,6 +208,7 @@ class HIPPatchMixin(DevicePatchMixin):
                                                        device=dev)
         self._tiles = {}
         self._tiles_host = {}
+        self._split_tables = {}
         self.send_patchstruct_to_device()
         self._dirty_data = False
         return self.device_patch
@@ -307,7 +308,44 @@ class HIPPatchMixin(DevicePatchMixin):
         tiles[:n] = tiles[:n][perm]
         self._tiles[pix_per_tile] = (device_bytes(tiles, self._dev()), n)
         self._tiles_host[pix_per_tile] = tiles
+        self.__dict__.setdefault("_split_tables", {}).pop(pix_per_tile, None)
         return self._tiles[pix_per_tile]
+
+    def tile_table_host(self, pix_per_tile):
+        """Host copy (numpy TILE_DTYPE) of the tile table ``device_tiles``
+        returns."""
+        _, n = self.device_tiles(pix_per_tile)
+        return self._tiles_host[pix_per_tile][:n].copy()
+
+    def set_split_tiles(self, pix_per_tile, tiles):
+        """Keep a tile table with split tiles (FpxTile parts >= 2, each split
+        tile's pieces in a row) for persistent launches of this tile size,
+        beside the plain table of ``device_tiles``; ``tiles=None`` drops it.
+        Dropped too when the plain table is reordered or the patch is sent
+        to the device again."""
+        tables = self.__dict__.setdefault("_split_tables", {})
+        if tiles is None:
+            tables.pop(pix_per_tile, None)
+            return None
+        tiles = np.ascontiguousarray(tiles, dtype=TILE_DTYPE)
+        parts = np.maximum(tiles["parts"].astype(np.int64), 1)
+        k = 0
+        while k < len(tiles):   # every split tile's pieces 0..parts-1 in a row
+            p = int(parts[k])
+            run = tiles[k:k + p]
+            if (len(run) != p or not np.array_equal(run["part"], np.arange(p))
+                    or np.any(run["pix_start"] != run["pix_start"][0])
+                    or np.any(run["parts"] != tiles["parts"][k])):
+                raise ValueError(f"tile table entry {k}: pieces of a split tile must follow "
+                                 "each other with part 0..parts-1")
+            k += p
+        tables[pix_per_tile] = (device_bytes(tiles, self._dev()), len(tiles))
+        return tables[pix_per_tile]
+
+    def split_tiles(self, pix_per_tile):
+        """(tensor, n_items) of the split table set for this tile size, or
+        None."""
+        return self.__dict__.get("_split_tables", {}).get(pix_per_tile)
 
     def max_gauss(self):
         """n_sources x the largest n_psf_per_source of the packed patch (kept
@@ -382,3 +420,4 @@ class HIPPatchMixin(DevicePatchMixin):
         self.device_patch = None
         self._tiles = {}
         self._tiles_host = {}
+        self._split_tables = {}

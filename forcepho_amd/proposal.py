@@ -22,7 +22,8 @@ import sys
 import numpy as np
 
 from .kernel_limits import MAXBANDS, MAXRADII, MAXSOURCES, NPARAMS
-from ._lib import lib, check, FpxError, SLOT_DTYPE, SLOT_MODEL_ONLY
+from ._lib import lib, check, FpxError, SLOT_DTYPE, SLOT_MODEL_ONLY, STATUS_SPLIT_WAIT
+from . import tile_split
 
 # device output block of a proposer: the slot's status word (uint32, padded
 # to 8 bytes), pchi2 (double[MAXBANDS]), then the Responses
@@ -53,6 +54,17 @@ response_struct_dtype = np.dtype([("dchi2_dparam", response_float_dt, NPARAMS * 
 
 # kernel configuration used when none is given: 8 wavefronts x 4 pixels/lane
 DEFAULT_CONFIG = 5
+
+# launch flags (FPX_LAUNCH_PERSISTENT, FPX_LAUNCH_SPLIT)
+LAUNCH_PERSISTENT, LAUNCH_SPLIT = 1, 2
+
+# fixed cost of one piece of a split tile, in pairs inside the cutoff: its
+# pixel loads, cull, partial-model exchange and wait.  Measured on the MI355X
+# with 8 x 4 tiles: splitting pays on patches with fewer tiles than
+# workgroup slots (16-144 tiles of 540-1080 Gaussians: 1.6-2.1x faster) and
+# this overhead keeps the planner from splitting patches of many tiles,
+# where the pieces cost more than they balance (384 and 1024 tiles)
+PIECE_OVERHEAD = 30000
 
 # largest tile count for which evaluate_proposal takes the bound evaluator's
 # host path (fpx_evaluate_host: one kernel, no copies); the library sums the
@@ -117,6 +129,9 @@ class HIPProposer(ProposerBase):
         self._chi2_np = self._out_h.numpy()[_STATUS_BYTES:_CHI2_END].view(np.float64)
         self._resp_np = self._out_h.numpy()[_CHI2_END:].view(np.float32)
         self._slot_key = None
+        self._launch_flags = int(self.persistent)
+        self._split_bufs = {}   # split buffer (FpxSlot.split) per split tile table
+        self._split_buf = None  # the one the current slot uses, or None
         self._h2d_done = None   # event of the last copy out of the pinned proposal buffer
         # proposal and outputs of the host path (fpx_evaluate_host), with their
         # addresses taken once (numpy's .ctypes.data costs microseconds)
@@ -136,37 +151,45 @@ class HIPProposer(ProposerBase):
         self._slot_key = None
 
     def autotune(self, proposal, patch=None, repeats=5, configs=None, persistent=(True, False),
-                 balance=True):
+                 balance=True, split=(False, True)):
         """Time every (configuration, persistent) pair on ``proposal`` and keep
         the fastest; with ``balance`` the tiles are first reordered for each
-        pair (``balance``).  Returns {(config, persistent): seconds per
-        evaluation}."""
+        pair (``balance``), and persistent launches are timed with each
+        ``split`` choice of ``balance`` (the faster is kept).  Returns
+        {(config, persistent): seconds per evaluation}; the times of every
+        split choice are in ``split_timings`` {(config, persistent, split):
+        seconds}."""
         import torch
         if patch is None:
             patch = self.patch
         configs = range(lib().fpx_n_configs()) if configs is None else configs
-        timings = {}
+        timings, chosen = {}, {}
+        self.split_timings = {}
         for cfg in configs:
             self.set_config(cfg)
             for pers in persistent:
                 self.persistent = pers
-                if balance:
-                    self.balance(proposal, patch)
-                self.launch(proposal, patch)
-                torch.cuda.synchronize()
-                a = torch.cuda.Event(enable_timing=True)
-                b = torch.cuda.Event(enable_timing=True)
-                a.record(self._stream())
-                for _ in range(repeats):
+                for sp in (split if (pers and balance) else (False,)):
+                    if balance:
+                        self.balance(proposal, patch, split=sp)
                     self.launch(proposal, patch)
-                b.record(self._stream())
-                torch.cuda.synchronize()
-                timings[(cfg, pers)] = a.elapsed_time(b) * 1e-3 / repeats
+                    torch.cuda.synchronize()
+                    a = torch.cuda.Event(enable_timing=True)
+                    b = torch.cuda.Event(enable_timing=True)
+                    a.record(self._stream())
+                    for _ in range(repeats):
+                        self.launch(proposal, patch)
+                    b.record(self._stream())
+                    torch.cuda.synchronize()
+                    t = a.elapsed_time(b) * 1e-3 / repeats
+                    self.split_timings[(cfg, pers, sp)] = t
+                    if (cfg, pers) not in timings or t < timings[(cfg, pers)]:
+                        timings[(cfg, pers)], chosen[(cfg, pers)] = t, sp
         best = min(timings, key=timings.get)
         self.set_config(best[0])
         self.persistent = best[1]
         if balance:
-            self.balance(proposal, patch)
+            self.balance(proposal, patch, split=chosen[best])
         return timings
 
     def _stream(self):
@@ -182,14 +205,45 @@ class HIPProposer(ProposerBase):
             return self._stream().cuda_stream
         return raw(self.device.index)
 
-    def _slot_for(self, patch, pixel_weight=None, prop_ptr=None, flags=0):
+    def tile_table(self, patch, split=True):
+        """(device tile table, entries, split buffer or None) that this
+        proposer's launches on ``patch`` use: the patch's split table
+        (``balance``) when the launch is persistent and one is set, else its
+        plain table."""
+        tiles, n = patch.device_tiles(self.pix_per_tile)
+        if split and self.persistent:
+            table = patch.split_tiles(self.pix_per_tile)
+            if table is not None:
+                tiles, n = table
+                return tiles, n, self._split_buffer(tiles, n)
+        return tiles, n, None
+
+    def _split_buffer(self, tiles, n):
+        """Zeroed split buffer (FpxSlot.split) for a split table, one per
+        table and configuration, kept with the proposer."""
+        import torch
+        key = (tiles.data_ptr(), n, self.config)
+        buf = self._split_bufs.get(key)
+        if buf is None:
+            if len(self._split_bufs) >= 64:
+                self._split_bufs.clear()
+            nbytes = int(lib().fpx_split_bytes(n, self.config))
+            buf = torch.zeros(max(nbytes, 8), dtype=torch.uint8, device=self.device)
+            self._split_bufs[key] = buf
+        return buf
+
+    def _slot_for(self, patch, pixel_weight=None, prop_ptr=None, flags=0, split=True):
         """(Re)build the device FpxSlot when the patch struct, tiles, weights,
         proposal buffer or flags change."""
-        tiles, n_tiles = patch.device_tiles(self.pix_per_tile)
+        tiles, n_tiles, split_buf = self.tile_table(patch, split)
+        self._split_buf = split_buf
+        sp = 0 if split_buf is None else split_buf.data_ptr()
+        self._launch_flags = ((LAUNCH_PERSISTENT if self.persistent else 0)
+                              | (LAUNCH_SPLIT if split_buf is not None else 0))
         pw = 0 if pixel_weight is None else pixel_weight.data_ptr()
         prop_ptr = self._prop.data_ptr() if prop_ptr is None else prop_ptr
         key = (patch.device_patch.data_ptr(), tiles.data_ptr(), n_tiles, int(patch.n_bands), pw,
-               prop_ptr, flags)
+               prop_ptr, flags, sp)
         if key != self._slot_key and key in self._slots:
             self._slot, self._slot_key = self._slots[key], key
         elif key != self._slot_key:
@@ -210,6 +264,7 @@ class HIPProposer(ProposerBase):
             slot["n_bands"] = int(patch.n_bands)
             slot["n_tiles"] = n_tiles
             slot["flags"] = flags
+            slot["split"] = sp
             self._slot.copy_(torch.from_numpy(slot.view(np.uint8).reshape(-1).copy()))
             self._slot_key = key
         return n_tiles
@@ -236,7 +291,7 @@ class HIPProposer(ProposerBase):
             n_tiles = self._load(proposal, patch, pixel_weight, stream, flags)
         fn = lib().fpx_evaluate_batch if outputs else lib().fpx_evaluate_tiles
         check(fn(self._slot.data_ptr(), 1, None, n_tiles, patch.max_gauss(), self.config,
-                 int(self.persistent), self._queue.data_ptr(), handle),
+                 self._launch_flags, self._queue.data_ptr(), handle),
               "fpx_evaluate_batch" if outputs else "fpx_evaluate_tiles")
         return n_tiles
 
@@ -258,7 +313,7 @@ class HIPProposer(ProposerBase):
         import torch
         if patch is None:
             patch = self.patch
-        n_tiles = self._load(proposal, patch, None)
+        n_tiles = self._load(proposal, patch, None, split=False)
         out = torch.zeros(1, dtype=torch.int64, device=self.device)
         tiles = torch.zeros(max(n_tiles, 1), dtype=torch.int32, device=self.device)
         check(lib().fpx_count_pairs(self._slot.data_ptr(), 1, None, n_tiles, patch.max_gauss(),
@@ -269,7 +324,8 @@ class HIPProposer(ProposerBase):
             return int(out.item()), tiles[:n_tiles].cpu().numpy().view(np.uint32).astype(np.int64)
         return int(out.item())
 
-    def balance(self, proposal, patch=None, tile_overhead=5000):
+    def balance(self, proposal, patch=None, tile_overhead=5000, split=True,
+                piece_overhead=PIECE_OVERHEAD):
         """Reorder the patch's tiles for the current configuration so that the
         compute units get equal work, the work of a tile being its pairs
         inside the cutoff for ``proposal`` (fpx_count_pairs) plus
@@ -278,8 +334,13 @@ class HIPProposer(ProposerBase):
         When the whole table is resident at once (fpx_resident_groups) and the
         launch is not persistent, tile i runs on compute unit i mod n_cu: the
         tiles are dealt heaviest first to the least-loaded unit that still
-        has a free slot.  Otherwise (a queue) they are sorted heaviest first.
-        Returns the permutation applied (new position -> old position)."""
+        has a free slot.  Otherwise (a queue) they are sorted heaviest first,
+        and with ``split`` the tiles that would hold the launch back are split
+        into pieces (``tile_split.plan_parts``; a piece costs its share plus
+        ``piece_overhead`` pair-equivalents): the patch then keeps a
+        split table beside the plain one, which persistent launches of this
+        proposer use.  Returns the permutation applied to the plain table
+        (new position -> old position)."""
         import ctypes
         import heapq
         if patch is None:
@@ -311,10 +372,17 @@ class HIPProposer(ProposerBase):
                 filled[u] += 1
                 if filled[u] < cap[u]:
                     heapq.heappush(load, (w + cost[t], u))
-        patch.reorder_tiles(self.pix_per_tile, perm)
+        patch.reorder_tiles(self.pix_per_tile, perm)   # drops an earlier split table
+        if self.persistent and split and n > 0:
+            over = float(piece_overhead)
+            parts = tile_split.plan_parts(cost[perm], resident.value, over)
+            if np.any(parts > 1):
+                table = tile_split.split_table(patch.tile_table_host(self.pix_per_tile),
+                                               cost[perm], parts, over)
+                patch.set_split_tiles(self.pix_per_tile, table)
         return perm
 
-    def _load(self, proposal, patch, pixel_weight, stream=None, flags=0):
+    def _load(self, proposal, patch, pixel_weight, stream=None, flags=0, split=True):
         """Stage the proposal and the slot for ``patch``; returns the tile
         count.  A numpy proposal is copied through pinned memory into the
         proposer's buffer on ``stream`` (the launch stream); a device tensor
@@ -335,12 +403,13 @@ class HIPProposer(ProposerBase):
                 self._prop.copy_(self._prop_h, non_blocking=True)
             self._h2d_done = torch.cuda.Event()
             self._h2d_done.record(stream)
-            return self._slot_for(patch, pixel_weight, flags=flags)
+            return self._slot_for(patch, pixel_weight, flags=flags, split=split)
         if proposal.device != self.device or not proposal.is_contiguous():
             raise ValueError("device proposals must be contiguous tensors on the proposer's device")
         if proposal.numel() * proposal.element_size() < n_src * source_struct_dtype.itemsize:
             raise ValueError("device proposal holds fewer records than the patch has sources")
-        return self._slot_for(patch, pixel_weight, prop_ptr=proposal.data_ptr(), flags=flags)
+        return self._slot_for(patch, pixel_weight, prop_ptr=proposal.data_ptr(), flags=flags,
+                              split=split)
 
     def _check_status(self):
         """Raise FpxError if the status word copied back with the outputs is
@@ -349,6 +418,8 @@ class HIPProposer(ProposerBase):
         if st:
             self._status_np[0] = 0
             self._status.zero_()
+            if st & STATUS_SPLIT_WAIT and self._split_buf is not None:
+                self._split_buf.zero_()   # counters left behind by the timed-out pieces
             raise FpxError(f"the kernel left tiles out (status {st:#x}): a size limit or a "
                            "radius bin of the packed patch is out of range")
 
@@ -386,13 +457,15 @@ class HIPProposer(ProposerBase):
             self._prop_np[:raw.size] = raw.reshape(-1)
             n_tiles = self._slot_for(patch, None)
             out_bytes = _CHI2_END + n_bands * MAXSOURCES * NPARAMS * 4
-            check(lib().fpx_evaluate_sync(self._slot.data_ptr(), n_tiles, patch.max_gauss(),
-                                          self.config, int(self.persistent),
-                                          self._queue.data_ptr(), self._prop.data_ptr(),
-                                          self._prop_h.data_ptr(), raw.size,
-                                          self._out.data_ptr(), self._out_h.data_ptr(),
-                                          out_bytes, self._out.data_ptr(), stream.cuda_stream),
-                  "fpx_evaluate_sync")
+            rc = lib().fpx_evaluate_sync(self._slot.data_ptr(), n_tiles, patch.max_gauss(),
+                                         self.config, self._launch_flags,
+                                         self._queue.data_ptr(), self._prop.data_ptr(),
+                                         self._prop_h.data_ptr(), raw.size,
+                                         self._out.data_ptr(), self._out_h.data_ptr(),
+                                         out_bytes, self._out.data_ptr(), stream.cuda_stream)
+            if rc != 0 and self._split_buf is not None:
+                self._split_buf.zero_()   # counters a timed-out piece may have left
+            check(rc, "fpx_evaluate_sync")
         else:
             stream = self._stream()
             self.launch(proposal, patch, stream=stream)

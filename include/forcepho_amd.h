@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define FPX_ABI_VERSION 2
+#define FPX_ABI_VERSION 3
 
 /* ------------------------------------------------------------------------ */
 /* Wire structs.  Byte layouts equal the reference's (checked by           */
@@ -101,17 +101,33 @@ typedef struct FpxResponse {
 } FpxResponse;
 
 /* One unit of kernel work: a run of at most pixels-per-tile consecutive
- * pixels of one exposure.  Built on the host by fpx_plan_tiles.  16 bytes. */
+ * pixels of one exposure.  Built on the host by fpx_plan_tiles.  16 bytes.
+ *   parts, part : a tile shared by `parts` workgroups (2 <= parts <=
+ *     FPX_MAX_TILE_PARTS) appears `parts` times in a row in the table, with
+ *     part = 0, 1, ..., parts - 1 and the same exposure, band and pixels.
+ *     Piece `part` takes the part-th of `parts` equal runs of the tile's
+ *     surviving ImageGaussians (in their (source, component) order) in both
+ *     passes; the pieces exchange their partial models through the slot's
+ *     split buffer, piece 0 adds the chi2 and writes the residual.  For the
+ *     few tiles that carry several times the mean work (Gaussians crowded
+ *     near a few pixels), so that no single workgroup holds the launch back.
+ *     parts 0 or 1: the whole tile (what fpx_plan_tiles writes).  The pieces
+ *     wait for each other, so they must be able to run at the same time: a
+ *     persistent launch, whose queue hands them out together. */
 typedef struct FpxTile {
     int32_t exposure;
-    int32_t band;
+    int16_t band;
+    uint8_t part;
+    uint8_t parts;
     int32_t pix_start;   /* absolute index into the patch pixel arrays */
     int32_t pix_count;
 } FpxTile;
 
+#define FPX_MAX_TILE_PARTS 16
+
 /* One evaluation in a launch: a patch, a proposal for it and where the
  * results go.  An array of these lives in device memory; a launch may hold
- * many (many patches, many HMC chains on one patch, ...).  80 bytes.
+ * many (many patches, many HMC chains on one patch, ...).
  *   accum  : double[fpx_accum_doubles(n_bands)], zero before the first launch;
  *            the kernel leaves it zero again after each launch
  *   status : NULL, or a device uint32 word, zero before the launch.  A tile
@@ -128,7 +144,13 @@ typedef struct FpxTile {
  *            residual and pchi2 are written, pdchi2_dp is zero.  Other bits
  *            reserved, 0.  No counterpart in the reference, whose kernel
  *            always runs both passes; it serves forward passes that need
- *            the residual image and not the gradient. */
+ *            the residual image and not the gradient.
+ *   split  : NULL when no tile of the table is split (parts <= 1), else
+ *            device memory of fpx_split_bytes(n_tiles, cfg) bytes, zero
+ *            before the first launch that uses it and left zero by every
+ *            launch (after one that sets FPX_STATUS_SPLIT_WAIT, zero it
+ *            again).  Slots that may run at the same time need their own.
+ * 88 bytes. */
 typedef struct FpxSlot {
     const FpxPatch *patch;
     const FpxSource *proposal;
@@ -142,6 +164,7 @@ typedef struct FpxSlot {
     int32_t n_tiles;
     int32_t flags;
     int32_t reserved;
+    float *split;
 } FpxSlot;
 
 #define FPX_SLOT_MODEL_ONLY 1
@@ -162,6 +185,19 @@ typedef struct FpxSlot {
                                      table planned for a larger tile), or max_gauss below
                                      n_sources x n_psf_per_source */
 #define FPX_STATUS_RADIUS_BIN 2u  /* a PSF entry's sersic_radius_bin is outside [0, n_radii) */
+#define FPX_STATUS_SPLIT_WAIT 4u  /* a piece of a split tile waited FPX_SPLIT_WAIT_MS for the
+                                     others and went on without their partial models (the
+                                     pieces were not running together) */
+#define FPX_SPLIT_WAIT_MS 50
+
+/* Bits of the `persistent` argument of the launches that take one (1 and 0
+ * keep their meaning).
+ *   FPX_LAUNCH_PERSISTENT : a resident grid that pulls tiles from the queue
+ *   FPX_LAUNCH_SPLIT      : the tile tables may hold split tiles (FpxTile
+ *                           parts >= 2); needs FPX_LAUNCH_PERSISTENT.  Without
+ *                           it a split tile is left out (FPX_STATUS_LIMIT). */
+#define FPX_LAUNCH_PERSISTENT 1
+#define FPX_LAUNCH_SPLIT 2
 
 /* A mixture-amplitude model that fpx_build_sources evaluates on the device:
  * the mixture amplitudes of a source and their derivatives with respect to
@@ -222,6 +258,11 @@ size_t fpx_struct_size(int which);
 
 /* Doubles of accumulator space one FpxSlot needs for n_bands bands. */
 size_t fpx_accum_doubles(int32_t n_bands);
+
+/* Bytes of the split buffer (FpxSlot.split) of a slot whose tile table holds
+ * n_tiles entries for kernel configuration cfg: a counter per entry and the
+ * partial model of every piece.  0 for an invalid cfg or n_tiles <= 0. */
+size_t fpx_split_bytes(int32_t n_tiles, int32_t cfg);
 
 /* Split every exposure of a packed patch into tiles of at most pix_per_tile
  * pixels (pix_per_tile a multiple of FPX_WAVE).  Inputs are the host copies of
@@ -398,8 +439,9 @@ int fpx_evaluate_static(const FpxPatch *patch_dev, const FpxTile *tiles_dev, int
  *                  n_tiles (item_offsets[0] = 0); may be NULL if n_slots == 1
  *   n_items      : item_offsets[n_slots] (host copy)
  *   max_gauss    : max over slots of n_sources x max n_psf_per_source
- *   persistent   : 0 = one workgroup per tile; 1 = a resident grid that
- *                  pulls tiles from the queue counter
+ *   persistent   : 0 = one workgroup per tile; 1 (FPX_LAUNCH_PERSISTENT) =
+ *                  a resident grid that pulls tiles from the queue counter;
+ *                  | FPX_LAUNCH_SPLIT when a tile table holds split tiles
  *   queue        : device uint32[2], zero before the first launch that uses
  *                  it and left zero after each: the work counter of a
  *                  persistent launch and the done counter that finds the

@@ -33,13 +33,15 @@ def test_every_declared_symbol_is_exported_and_bound():
 
 def test_struct_sizes_match_dtypes():
     L = _lib.lib()
-    assert L.fpx_abi_version() == 2
+    assert L.fpx_abi_version() == 3
     assert L.fpx_struct_size(_lib.STRUCT_PATCH) == PatchBase.pdt.itemsize == 152
     assert L.fpx_struct_size(_lib.STRUCT_SOURCE) == source_struct_dtype.itemsize == 264
     assert L.fpx_struct_size(_lib.STRUCT_RESPONSE) == response_struct_dtype.itemsize == 840
     assert L.fpx_struct_size(_lib.STRUCT_PSFGAUSS) == PSF_DTYPE.itemsize == 28
     assert L.fpx_struct_size(_lib.STRUCT_TILE) == _lib.TILE_DTYPE.itemsize == 16
-    assert L.fpx_struct_size(_lib.STRUCT_SLOT) == _lib.SLOT_DTYPE.itemsize == 80
+    assert L.fpx_struct_size(_lib.STRUCT_SLOT) == _lib.SLOT_DTYPE.itemsize == 88
+    assert _lib.SLOT_DTYPE.fields["split"][1] == 80
+    assert _lib.TILE_DTYPE.fields["part"][1] == 6 and _lib.TILE_DTYPE.fields["parts"][1] == 7
     assert L.fpx_struct_size(_lib.STRUCT_AMPMODEL) == ctypes.sizeof(_lib.AmplitudeModel) == 88
     assert _lib.AmplitudeModel.tx.offset == 48 and _lib.AmplitudeModel.coef.offset == 80
     assert L.fpx_struct_size(99) == 0
@@ -96,7 +98,22 @@ def test_plan_tiles_covers_every_pixel_once():
         e = t["exposure"]
         assert exp_start[e] <= s and s + t["pix_count"] <= exp_start[e] + exp_n[e]
         assert t["band"] == (0 if e == 0 else 1)
+        assert t["part"] == 0 and t["parts"] == 0      # whole tiles
     assert np.all(covered == 1)
+
+
+def test_split_bytes():
+    """Split buffer: a counter per table entry (padded to 256 bytes) and a
+    partial-model row of pixels-per-tile floats per entry."""
+    L = _lib.lib()
+    for cfg in range(L.fpx_n_configs()):
+        ppt = ctypes.c_int32()
+        assert L.fpx_config_info(cfg, None, None, ctypes.byref(ppt)) == 0
+        for n in (1, 63, 64, 65, 1000):
+            assert L.fpx_split_bytes(n, cfg) == -(-4 * n // 256) * 256 + 4 * n * ppt.value
+    assert L.fpx_split_bytes(0, 0) == 0
+    assert L.fpx_split_bytes(10, -1) == 0
+    assert L.fpx_split_bytes(10, L.fpx_n_configs()) == 0
 
 
 def test_plan_tiles_rejects_bad_sizes():
