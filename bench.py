@@ -653,9 +653,18 @@ def split_measurement(args, steps=200):
             for _ in range(10):
                 prop.launch(pdev, patch)
             t = time_region(lambda: prop.launch(pdev, patch), steps)
+            # forcepho's call: numpy proposal in, numpy results out
+            prop.evaluate_proposal(proposal, patch=patch)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                prop.evaluate_proposal(proposal, patch=patch)
+            api = (time.perf_counter() - t0) / steps
             row[label] = {"items": int(n_items), "us": t * 1e6,
-                          "evals_per_s": row["evals_per_step"] / t}
+                          "evals_per_s": row["evals_per_step"] / t,
+                          "evaluate_proposal_us": api * 1e6}
         row["speedup"] = row["whole"]["us"] / row["split"]["us"]
+        row["evaluate_proposal_speedup"] = (row["whole"]["evaluate_proposal_us"]
+                                            / row["split"]["evaluate_proposal_us"])
         (c0, g0), (c1, g1) = results["whole"], results["split"]
         scale = np.abs(g0).max(axis=(1, 2), keepdims=True) + 1e-30
         row["split_vs_whole"] = {"chi2_rel_max": float(np.max(np.abs(c1 - c0) / np.abs(c0))),

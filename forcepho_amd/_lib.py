@@ -72,6 +72,7 @@ _SIGNATURES = {
     "fpx_evaluate_sync": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _p, _p, _p, _c_size,
                                    _p, _p, _c_size, _p, _p]),
     "fpx_bind": (_c_int, [_p, _c_int, _c_int, _c_int, _p]),
+    "fpx_bind_tiles": (_c_int, [_p, _c_int, _c_int, _c_int, _p, _c_int, _p]),
     "fpx_evaluate": (_c_int, [_p, _p, _p, _p, _p]),
     "fpx_evaluate_host": (_c_int, [_p, _p, _c_int, _p, _p, _p]),
     "fpx_evaluator_status": (_c_int, [_p, _p]),

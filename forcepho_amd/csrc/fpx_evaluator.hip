@@ -25,6 +25,8 @@
 //   prop   FpxSource[MAXSOURCES] (fpx_evaluate_host's proposal)
 //   accum  double[fpx_accum_doubles(n_bands)]
 //   tiles  FpxTile[n_tiles]
+//   split  fpx_split_bytes(n_tiles, cfg), when the tile table holds split
+//          tiles (fpx_bind_tiles)
 // and one mapped pinned host block: fpx_evaluate_host's outputs in the io
 // layout (pchi2 double[MAXBANDS] at kIoChi2, FpxResponse[MAXBANDS] at
 // kIoResp), written there by the output kernel itself (no copy back), then
@@ -42,7 +44,8 @@
 struct FpxEvaluator {
   int device = 0;
   const FpxPatch *patch = nullptr;
-  int32_t n_bands = 0, n_sources = 0, max_gauss = 0, cfg = 0, persistent = 0, n_tiles = 0;
+  int32_t n_bands = 0, n_sources = 0, max_gauss = 0, cfg = 0, n_tiles = 0;
+  int32_t launch_flags = 0;  // FPX_LAUNCH_PERSISTENT, FPX_LAUNCH_SPLIT
   unsigned char *block = nullptr;
   unsigned char *host = nullptr;      // mapped pinned block (host address)
   unsigned char *host_dev = nullptr;  // the same block's device address
@@ -53,6 +56,7 @@ struct FpxEvaluator {
   FpxSource *prop = nullptr;
   double *accum = nullptr;
   FpxTile *tiles = nullptr;
+  float *split = nullptr;          // split buffer, or NULL
   size_t o_stage = 0, o_part = 0;  // offsets in the host block
   std::vector<int32_t> tile_band;  // per tile, when the partials path is used
   uint32_t seq = 0;                // sequence number of the last host-path call
@@ -116,11 +120,51 @@ FpxSlot slot_of(const FpxEvaluator *ev, const FpxSource *prop, double *pchi2, Fp
   s.status = ev->status;
   s.n_bands = ev->n_bands;
   s.n_tiles = ev->n_tiles;
+  s.split = ev->split;
   return s;
 }
 
+// Check a caller's tile table against the patch's exposures (index arrays
+// read back by bind): every entry inside one exposure of its band, at most
+// ppt pixels, and every split tile's pieces 0..parts-1 in a row with the
+// same pixels.  Sets *split when some tile is split.
+int check_tiles(const FpxTile *t, int32_t n, int ppt, int n_bands, int n_exp,
+                const std::vector<int16_t> &band_start, const std::vector<int16_t> &band_N,
+                const std::vector<int32_t> &exp_start, const std::vector<int32_t> &exp_N,
+                bool *split) {
+  *split = false;
+  for (int32_t k = 0; k < n; ++k) {
+    const FpxTile &x = t[k];
+    if (x.band < 0 || x.band >= n_bands) return arg_err("tile table: band out of range");
+    if (x.exposure < band_start[x.band] || x.exposure >= band_start[x.band] + band_N[x.band] ||
+        x.exposure >= n_exp)
+      return arg_err("tile table: exposure not of the tile's band");
+    if (x.pix_count < 0 || x.pix_count > ppt)
+      return arg_err("tile table: pixel count outside [0, pixels per tile of cfg]");
+    if (x.pix_start < exp_start[x.exposure] ||
+        (long)x.pix_start + x.pix_count > (long)exp_start[x.exposure] + exp_N[x.exposure])
+      return arg_err("tile table: pixels outside the tile's exposure");
+    const int parts = x.parts > 1 ? x.parts : 1;
+    if (parts > FPX_MAX_TILE_PARTS) return arg_err("tile table: more than FPX_MAX_TILE_PARTS pieces");
+    if (x.part >= parts) return arg_err("tile table: piece index outside [0, parts)");
+    const int part = x.part;
+    if (parts > 1) {
+      *split = true;
+      const int32_t first = k - part;
+      if (first < 0 || first + parts > n) return arg_err("tile table: a split tile's pieces are cut off");
+      const FpxTile &p0 = t[first];
+      if (p0.parts != x.parts || p0.exposure != x.exposure || p0.band != x.band ||
+          p0.pix_start != x.pix_start || p0.pix_count != x.pix_count ||
+          t[k].part != (uint8_t)(k - first))
+        return arg_err("tile table: pieces of a split tile must follow each other, part 0..parts-1");
+    }
+  }
+  return FPX_OK;
+}
+
+// tiles_in: the caller's tile table (host), or NULL to plan one.
 int bind(const FpxPatch *patch_dev, int32_t n_bands, int32_t cfg, int32_t persistent,
-         FpxEvaluator *ev) {
+         FpxEvaluator *ev, const FpxTile *tiles_in = nullptr, int32_t n_in = 0) {
   if (!patch_dev) return arg_err("NULL patch");
   if (n_bands < 1 || n_bands > MAXBANDS) return arg_err("n_bands out of range [1, MAXBANDS]");
   if (cfg < 0) cfg = fpx::kDefaultConfig;
@@ -176,31 +220,48 @@ int bind(const FpxPatch *patch_dev, int32_t n_bands, int32_t cfg, int32_t persis
           return arg_err("a PSF entry's sersic_radius_bin is outside [0, n_radii)");
       }
     }
-  // tile table
-  const int n_tiles = fpx_plan_tiles(exp_start.data(), exp_N.data(), band_start.data(),
-                                     band_N.data(), n_bands, ppt, nullptr, 0);
-  if (n_tiles < 0) return n_tiles;
-  std::vector<FpxTile> tiles((size_t)std::max(n_tiles, 1));
-  if (n_tiles > 0) {
-    rc = fpx_plan_tiles(exp_start.data(), exp_N.data(), band_start.data(), band_N.data(), n_bands,
-                        ppt, tiles.data(), n_tiles);
-    if (rc < 0) return rc;
+  // tile table: the caller's, checked, or planned here
+  bool has_split = false;
+  int n_tiles = 0;
+  std::vector<FpxTile> tiles;
+  if (tiles_in) {
+    if (n_in < 0) return arg_err("negative tile count");
+    rc = check_tiles(tiles_in, n_in, ppt, n_bands, n_exp, band_start, band_N, exp_start, exp_N,
+                     &has_split);
+    if (rc) return rc;
+    if (has_split && !persistent)
+      return arg_err("split tiles need a persistent evaluator (persistent = 1)");
+    n_tiles = n_in;
+    tiles.assign(tiles_in, tiles_in + n_in);
+    if (tiles.empty()) tiles.resize(1);
+  } else {
+    n_tiles = fpx_plan_tiles(exp_start.data(), exp_N.data(), band_start.data(), band_N.data(),
+                             n_bands, ppt, nullptr, 0);
+    if (n_tiles < 0) return n_tiles;
+    tiles.resize((size_t)std::max(n_tiles, 1));
+    if (n_tiles > 0) {
+      rc = fpx_plan_tiles(exp_start.data(), exp_N.data(), band_start.data(), band_N.data(),
+                          n_bands, ppt, tiles.data(), n_tiles);
+      if (rc < 0) return rc;
+    }
   }
   ev->patch = patch_dev;
   ev->n_bands = n_bands;
   ev->n_sources = P.n_sources;
   ev->max_gauss = P.n_sources * max_psf;
   ev->cfg = cfg;
-  ev->persistent = persistent ? 1 : 0;
+  ev->launch_flags = (persistent ? FPX_LAUNCH_PERSISTENT : 0) | (has_split ? FPX_LAUNCH_SPLIT : 0);
   ev->n_tiles = n_tiles;
   // device block
   const size_t o_queue = align256(kIoBytes);
   const size_t o_prop = o_queue + 256;
   const size_t o_accum = o_prop + align256(sizeof(FpxSource) * MAXSOURCES);
   const size_t o_tiles = o_accum + align256(sizeof(double) * fpx_accum_doubles(n_bands));
-  const size_t bytes = o_tiles + align256(sizeof(FpxTile) * tiles.size());
+  const size_t o_split = o_tiles + align256(sizeof(FpxTile) * tiles.size());
+  const size_t bytes = o_split + (has_split ? align256(fpx_split_bytes(n_tiles, cfg)) : 0);
   EV_HIP(hipMalloc((void **)&ev->block, bytes));
   EV_HIP(hipMemset(ev->block, 0, o_tiles));
+  if (has_split) EV_HIP(hipMemset(ev->block + o_split, 0, bytes - o_split));
   EV_HIP(hipMemcpy(ev->block + o_tiles, tiles.data(), sizeof(FpxTile) * tiles.size(),
                    hipMemcpyHostToDevice));
   ev->status = reinterpret_cast<uint32_t *>(ev->block);
@@ -210,6 +271,7 @@ int bind(const FpxPatch *patch_dev, int32_t n_bands, int32_t cfg, int32_t persis
   ev->prop = reinterpret_cast<FpxSource *>(ev->block + o_prop);
   ev->accum = reinterpret_cast<double *>(ev->block + o_accum);
   ev->tiles = reinterpret_cast<FpxTile *>(ev->block + o_tiles);
+  ev->split = has_split ? reinterpret_cast<float *>(ev->block + o_split) : nullptr;
   // mapped pinned host block: the outputs of fpx_evaluate_host, the
   // proposal staging, and the tile rows of a small patch
   ev->o_stage = align256(kIoBytes);
@@ -256,6 +318,9 @@ int wait_flag(FpxEvaluator *ev, uint32_t seq, hipStream_t s) {
 int read_status(FpxEvaluator *ev, hipStream_t s, uint32_t status) {
   if (status == 0) return FPX_OK;
   EV_HIP(hipMemsetAsync(ev->status, 0, sizeof(uint32_t), s));
+  // pieces that timed out may have left their split counters behind
+  if ((status & FPX_STATUS_SPLIT_WAIT) && ev->split)
+    EV_HIP(hipMemsetAsync(ev->split, 0, fpx_split_bytes(ev->n_tiles, ev->cfg), s));
   EV_HIP(hipStreamSynchronize(s));
   return arg_err(fpx::status_text(status));
 }
@@ -267,7 +332,8 @@ int evaluate_small(FpxEvaluator *ev, double *chi2_host, FpxResponse *resp_host, 
   slot.accum = reinterpret_cast<double *>(ev->host_dev + ev->o_part);
   slot.flags = FPX_SLOT_TILE_PARTIALS;
   const uint32_t seq = next_seq(ev);
-  int rc = fpx::launch_one(slot, ev->n_tiles, ev->max_gauss, ev->cfg, ev->persistent, ev->queue, s,
+  int rc = fpx::launch_one(slot, ev->n_tiles, ev->max_gauss, ev->cfg, ev->launch_flags, ev->queue,
+                           s,
                            false, reinterpret_cast<uint32_t *>(ev->host_dev + kIoFlag), seq);
   if (rc) return rc;
   rc = wait_flag(ev, seq, s);
@@ -317,6 +383,23 @@ int fpx_bind(const FpxPatch *patch_dev, int32_t n_bands, int32_t cfg, int32_t pe
   return FPX_OK;
 }
 
+int fpx_bind_tiles(const FpxPatch *patch_dev, int32_t n_bands, int32_t cfg, int32_t persistent,
+                   const FpxTile *tiles_host, int32_t n_tiles, FpxEvaluator **out) {
+  if (!out) return arg_err("NULL output handle");
+  *out = nullptr;
+  if (!tiles_host && n_tiles != 0) return arg_err("NULL tile table");
+  FpxEvaluator *ev = new FpxEvaluator();
+  static const FpxTile kNone = {};
+  const int rc = bind(patch_dev, n_bands, cfg, persistent, ev, tiles_host ? tiles_host : &kNone,
+                      n_tiles);
+  if (rc) {
+    release(ev);
+    return rc;
+  }
+  *out = ev;
+  return FPX_OK;
+}
+
 int fpx_evaluate(FpxEvaluator *ev, const FpxSource *proposal_dev, double *pchi2_dev,
                  FpxResponse *pdchi2_dp_dev, void *stream) {
   if (!ev || !proposal_dev || !pchi2_dev || !pdchi2_dp_dev)
@@ -324,7 +407,7 @@ int fpx_evaluate(FpxEvaluator *ev, const FpxSource *proposal_dev, double *pchi2_
   const int rc = check_device(ev);
   if (rc) return rc;
   return fpx::launch_one(slot_of(ev, proposal_dev, pchi2_dev, pdchi2_dp_dev), ev->n_tiles,
-                         ev->max_gauss, ev->cfg, ev->persistent, ev->queue, stream);
+                         ev->max_gauss, ev->cfg, ev->launch_flags, ev->queue, stream);
 }
 
 int fpx_evaluate_host(FpxEvaluator *ev, const FpxSource *proposal_host, int32_t n_sources,
@@ -347,14 +430,14 @@ int fpx_evaluate_host(FpxEvaluator *ev, const FpxSource *proposal_host, int32_t 
   if (ev->n_tiles > 0) {
     const uint32_t seq = next_seq(ev);
     rc = fpx::launch_one(slot_of(ev, ev->prop, chi2_out, resp_out), ev->n_tiles, ev->max_gauss,
-                         ev->cfg, ev->persistent, ev->queue, stream, true,
+                         ev->cfg, ev->launch_flags, ev->queue, stream, true,
                          reinterpret_cast<uint32_t *>(ev->host_dev + kIoFlag), seq);
     if (rc) return rc;
     rc = wait_flag(ev, seq, s);
     if (rc) return rc;
   } else {  // no tiles: the output kernel alone writes zeros
     rc = fpx::launch_one(slot_of(ev, ev->prop, chi2_out, resp_out), 0, ev->max_gauss, ev->cfg,
-                         ev->persistent, ev->queue, stream);
+                         ev->launch_flags, ev->queue, stream);
     if (rc) return rc;
     EV_HIP(hipStreamSynchronize(s));
   }

@@ -22,12 +22,12 @@ int report_hip_error(hipError_t e, const char *where);
 std::string status_text(uint32_t status);
 
 // The tile kernel and (finalize_outputs) the output pass for one slot passed
-// by value (host struct): no slot table in device memory.  With a flag (a
-// host-visible word; needs queue and n_tiles > 0) the launch's last
-// workgroup stores seq into it once every result of the launch is visible
-// to the host.
+// by value (host struct): no slot table in device memory.  launch_flags:
+// FPX_LAUNCH_PERSISTENT, FPX_LAUNCH_SPLIT.  With a flag (a host-visible word;
+// needs queue and n_tiles > 0) the launch's last workgroup stores seq into
+// it once every result of the launch is visible to the host.
 int launch_one(const FpxSlot &slot, int32_t n_tiles, int32_t max_gauss, int32_t cfg,
-               int32_t persistent, uint32_t *queue, void *stream, bool finalize_outputs = true,
+               int32_t launch_flags, uint32_t *queue, void *stream, bool finalize_outputs = true,
                uint32_t *flag = nullptr, uint32_t seq = 0);
 // Pixels per tile of kernel configuration cfg (0 if cfg is invalid).
 int config_pixels(int32_t cfg);

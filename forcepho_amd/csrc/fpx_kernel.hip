@@ -1016,18 +1016,23 @@ void chi2_grad_kernel(
   // Count the workgroup done.  With outputs or a flag, every wave first waits
   // for its stores and atomics to be performed (the vector memory counter
   // drains only then), the barrier orders all of them before the count, and
-  // the count is an acquire-release: at device scope for outputs (the last
-  // workgroup sees every other workgroup's sums), at system scope for a flag
-  // (the host reads what every workgroup stored once it sees the flag).
+  // the count is an acquire-release: at device scope when the last workgroup
+  // writes the outputs (it sees every other workgroup's sums; its own output
+  // stores reach the host through the flag's system-scope release below), at
+  // system scope when every workgroup left its tiles' sums in host memory
+  // for the host to read once it sees the flag (FPX_SLOT_TILE_PARTIALS).
   const bool outputs = (mode & kModeOutputs) != 0;
   const bool signal = (mode & kModeFlag) != 0;
+  const bool partials =
+      signal && (__builtin_amdgcn_readfirstlane((kOne ? one : load_const(slots, 0)).flags) &
+                 FPX_SLOT_TILE_PARTIALS);
   if (outputs || signal) __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned d;
-    if (signal)
+    if (partials)
       d = __hip_atomic_fetch_add(queue + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_SYSTEM);
-    else if (outputs)
+    else if (outputs || signal)
       d = __hip_atomic_fetch_add(queue + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     else
       d = atomicAdd(queue + 1, 1u);
@@ -1417,10 +1422,10 @@ namespace fpx {
 int report_error(int code, const char *msg) { return set_err(code, msg); }
 int report_hip_error(hipError_t e, const char *where) { return hip_err(e, where); }
 int launch_one(const FpxSlot &slot, int32_t n_tiles, int32_t max_gauss, int32_t cfg,
-               int32_t persistent, uint32_t *queue, void *stream, bool finalize_outputs,
+               int32_t launch_flags, uint32_t *queue, void *stream, bool finalize_outputs,
                uint32_t *flag, uint32_t seq) {
-  return launch(nullptr, &slot, 1, nullptr, n_tiles, max_gauss, cfg,
-                persistent ? FPX_LAUNCH_PERSISTENT : 0, queue, stream, finalize_outputs, flag, seq);
+  return launch(nullptr, &slot, 1, nullptr, n_tiles, max_gauss, cfg, launch_flags, queue, stream,
+                finalize_outputs, flag, seq);
 }
 std::string status_text(uint32_t status) { return status_message(status); }
 int config_pixels(int32_t cfg) {

@@ -104,7 +104,7 @@ int fpx_evaluate_static(const FpxPatch *patch_dev, const FpxTile *tiles_dev, int
   slot.status = reinterpret_cast<uint32_t *>(base + kStatusOff);
   slot.n_bands = n_bands;
   slot.n_tiles = n_tiles;
-  return fpx::launch_one(slot, n_tiles, max_gauss, cfg, persistent,
+  return fpx::launch_one(slot, n_tiles, max_gauss, cfg, persistent ? FPX_LAUNCH_PERSISTENT : 0,
                          reinterpret_cast<uint32_t *>(base + kQueueOff), stream);
 }
 

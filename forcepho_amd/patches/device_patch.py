@@ -68,11 +68,27 @@ def _release_handles(handles):
             pass
 
 
+_PIXELS_PER_TILE = {}
+
+
+def _config_pixels(config):
+    """Pixels per tile of a kernel configuration (fpx_config_info, cached)."""
+    ppt = _PIXELS_PER_TILE.get(config)
+    if ppt is None:
+        v = ctypes.c_int32()
+        check(lib().fpx_config_info(int(config), None, None, ctypes.byref(v)), "fpx_config_info")
+        ppt = _PIXELS_PER_TILE[config] = v.value
+    return ppt
+
+
 class _BoundEvaluators:
-    """FpxEvaluator handles (fpx_bind) of one upload of a patch's device
-    struct, per kernel configuration: config -> (handle, n_tiles).  Released
-    when the struct is uploaded again, when the patch is freed and when it is
-    collected."""
+    """FpxEvaluator handles of one upload of a patch's device struct: per
+    kernel configuration (fpx_bind), config -> (handle, n_tiles), and per
+    tile table of the patch (fpx_bind_tiles), (kind, config, persistent,
+    table generation) -> (handle, n_entries), kind "split" or "plain".
+    Released when the struct is uploaded again, when the patch is freed and
+    when it is collected; a table's evaluators also when the table is
+    replaced."""
 
     def __init__(self):
         self.handles = {}
@@ -81,6 +97,10 @@ class _BoundEvaluators:
     def release(self):
         handles, self.handles = self.handles, {}
         _release_handles([ev for ev, _ in handles.values()])
+
+    def release_kind(self, kind):
+        keys = [k for k in self.handles if isinstance(k, tuple) and k[0] == kind]
+        _release_handles([self.handles.pop(k)[0] for k in keys])
 
 
 class DevicePatchMixin:
@@ -302,13 +322,17 @@ class HIPPatchMixin(DevicePatchMixin):
         tiles = self._tiles_host[pix_per_tile]
         n = self._tiles[pix_per_tile][1]
         perm = np.asarray(perm, dtype=np.int64)
+        self.set_split_tiles(pix_per_tile, None)
         if sorted(perm.tolist()) != list(range(n)):
             raise ValueError("perm is not a permutation of the tile table")
         tiles = tiles.copy()
         tiles[:n] = tiles[:n][perm]
         self._tiles[pix_per_tile] = (device_bytes(tiles, self._dev()), n)
         self._tiles_host[pix_per_tile] = tiles
-        self.__dict__.setdefault("_split_tables", {}).pop(pix_per_tile, None)
+        self._tiles_generation = self.__dict__.get("_tiles_generation", 0) + 1
+        cache = self.__dict__.get("_bound")
+        if cache is not None:
+            cache.release_kind("plain")
         return self._tiles[pix_per_tile]
 
     def tile_table_host(self, pix_per_tile):
@@ -324,6 +348,10 @@ class HIPPatchMixin(DevicePatchMixin):
         Dropped too when the plain table is reordered or the patch is sent
         to the device again."""
         tables = self.__dict__.setdefault("_split_tables", {})
+        self._split_generation = self.__dict__.get("_split_generation", 0) + 1
+        cache = self.__dict__.get("_bound")
+        if cache is not None:
+            cache.release_kind("split")
         if tiles is None:
             tables.pop(pix_per_tile, None)
             return None
@@ -339,13 +367,16 @@ class HIPPatchMixin(DevicePatchMixin):
                 raise ValueError(f"tile table entry {k}: pieces of a split tile must follow "
                                  "each other with part 0..parts-1")
             k += p
-        tables[pix_per_tile] = (device_bytes(tiles, self._dev()), len(tiles))
-        return tables[pix_per_tile]
+        tables[pix_per_tile] = (device_bytes(tiles, self._dev()), len(tiles), tiles.copy())
+        return tables[pix_per_tile][:2]
 
-    def split_tiles(self, pix_per_tile):
-        """(tensor, n_items) of the split table set for this tile size, or
-        None."""
-        return self.__dict__.get("_split_tables", {}).get(pix_per_tile)
+    def split_tiles(self, pix_per_tile, host=False):
+        """(tensor, n_items) of the split table set for this tile size (with
+        ``host``, its numpy copy), or None."""
+        table = self.__dict__.get("_split_tables", {}).get(pix_per_tile)
+        if table is None:
+            return None
+        return table[2] if host else table[:2]
 
     def max_gauss(self):
         """n_sources x the largest n_psf_per_source of the packed patch (kept
@@ -358,10 +389,15 @@ class HIPPatchMixin(DevicePatchMixin):
         self._max_gauss = (gen, value)
         return value
 
-    def bound_evaluator(self, config):
+    def bound_evaluator(self, config, table=None, persistent=False):
         """An FpxEvaluator bound to the current device struct for kernel
-        configuration ``config`` (fpx_bind, one workgroup per tile), kept
-        until the struct is uploaded again; returns (handle, n_tiles)."""
+        configuration ``config``, kept until the struct is uploaded again;
+        returns (handle, n_entries).  table=None: the library plans the
+        tiles (fpx_bind, one workgroup per tile).  table="plain": the patch's
+        own tile table for the configuration's tile size (``device_tiles``,
+        balanced by ``HIPProposer.balance``), launched persistent or not;
+        table="split": the split table set for it (persistent), or None if
+        there is none (both fpx_bind_tiles, kept until the table changes)."""
         cache = self.__dict__.get("_bound")
         if cache is None:
             cache = self._bound = _BoundEvaluators()
@@ -370,6 +406,34 @@ class HIPPatchMixin(DevicePatchMixin):
         if cache.generation != gen:
             cache.release()
             cache.generation = gen
+        if table is not None:
+            # the key first (dictionary lookups: this runs on every
+            # evaluation), the table only when binding
+            ppt = _config_pixels(config)
+            if table == "split":
+                if ppt not in self.__dict__.get("_split_tables", {}):
+                    return None
+                persistent = True
+                gen = self.__dict__.get("_split_generation", 0)
+            elif table == "plain":
+                gen = self.__dict__.get("_tiles_generation", 0)
+            else:
+                raise ValueError("table must be None, 'plain' or 'split'")
+            key = (table, config, bool(persistent), gen)
+            hit = cache.handles.get(key)
+            if hit is not None:
+                return hit
+            if key not in cache.handles:
+                tiles = (self.split_tiles(ppt, host=True) if table == "split"
+                         else self.tile_table_host(ppt))
+                tiles = np.ascontiguousarray(tiles, dtype=TILE_DTYPE)
+                ev = ctypes.c_void_p()
+                check(lib().fpx_bind_tiles(self.device_patch.data_ptr(), int(self.n_bands),
+                                           int(config), int(bool(persistent)),
+                                           tiles.ctypes.data, len(tiles), ctypes.byref(ev)),
+                      "fpx_bind_tiles")
+                cache.handles[key] = (ev, len(tiles))
+            return cache.handles[key]
         if config not in cache.handles:
             ev = ctypes.c_void_p()
             check(lib().fpx_bind(self.device_patch.data_ptr(), int(self.n_bands), int(config), 0,

@@ -10,11 +10,14 @@ GPUProposer/CPUProposer.evaluate_proposal (forcepho/proposal.py:96-152,
                 padded arrays (``unpack=True``) or one flat array
 
 A numpy proposal is evaluated by one native call on the current torch
-stream: on a patch of at most HOST_PATH_TILES tiles, fpx_evaluate_host of the
-patch's bound evaluator (one kernel that reads the proposal from, and writes
-its per-tile sums to, mapped host memory); on larger patches
-fpx_evaluate_sync over the proposer's own (balanced) tile table, with the
-proposal and results copied through pinned buffers.
+stream, fpx_evaluate_host of one of the patch's bound evaluators: over the
+split table ``balance`` planned for the patch (persistent launches), else on
+a patch of at most HOST_PATH_TILES tiles over tiles the library plans (the
+tiles read the proposal from, and leave their sums in, mapped host memory),
+else over the patch's own (balanced) tile table.  One kernel in each case;
+the call returns when the launch signals through mapped memory that the
+results are there.  A device proposal is launched over the proposer's own
+slot.
 """
 
 import sys
@@ -437,42 +440,29 @@ class HIPProposer(ProposerBase):
             if len(proposal) != n_sources:
                 raise ValueError(f"proposal has {len(proposal)} sources, patch has {n_sources}")
             patch.check_limits()
-            if patch.device_tiles(self.pix_per_tile)[1] <= HOST_PATH_TILES:
-                # a small patch: the bound evaluator's one-kernel host path
-                ev, _ = patch.bound_evaluator(self.config)
-                self._host_prop[:n_sources] = proposal
-                check(lib().fpx_evaluate_host(ev, self._host_ptrs[0], n_sources,
-                                              self._host_ptrs[1], self._host_ptrs[2],
-                                              self._raw_stream()),
-                      "fpx_evaluate_host")
-                return self._outputs(self._host_chi2, self._host_resp, patch, unpack)
-            stream = self._stream()
-            prop = np.ascontiguousarray(proposal, dtype=source_struct_dtype)
-            # host proposal: copy in, launch, copy out and wait in one native
-            # call (fpx_evaluate_sync) through the pinned staging buffers
-            raw = prop.view(np.uint8)
-            if self._h2d_done is not None:
-                self._h2d_done.synchronize()   # an earlier launch's copy may be in flight
-                self._h2d_done = None
-            self._prop_np[:raw.size] = raw.reshape(-1)
-            n_tiles = self._slot_for(patch, None)
-            out_bytes = _CHI2_END + n_bands * MAXSOURCES * NPARAMS * 4
-            rc = lib().fpx_evaluate_sync(self._slot.data_ptr(), n_tiles, patch.max_gauss(),
-                                         self.config, self._launch_flags,
-                                         self._queue.data_ptr(), self._prop.data_ptr(),
-                                         self._prop_h.data_ptr(), raw.size,
-                                         self._out.data_ptr(), self._out_h.data_ptr(),
-                                         out_bytes, self._out.data_ptr(), stream.cuda_stream)
-            if rc != 0 and self._split_buf is not None:
-                self._split_buf.zero_()   # counters a timed-out piece may have left
-            check(rc, "fpx_evaluate_sync")
-        else:
-            stream = self._stream()
-            self.launch(proposal, patch, stream=stream)
-            with torch.cuda.stream(stream):
-                self._out_h.copy_(self._out, non_blocking=True)
-            stream.synchronize()
-            self._check_status()
+            # one native call through a bound evaluator (fpx_evaluate_host):
+            # the split table ``balance`` planned for the patch (persistent
+            # launches), else, on a small patch, tiles the library plans
+            # (read the proposal from and leave their sums in mapped host
+            # memory), else the patch's own (balanced) table
+            bound = patch.bound_evaluator(self.config, "split") if self.persistent else None
+            if bound is None:
+                if patch.device_tiles(self.pix_per_tile)[1] <= HOST_PATH_TILES:
+                    bound = patch.bound_evaluator(self.config)
+                else:
+                    bound = patch.bound_evaluator(self.config, "plain", self.persistent)
+            self._host_prop[:n_sources] = proposal
+            check(lib().fpx_evaluate_host(bound[0], self._host_ptrs[0], n_sources,
+                                          self._host_ptrs[1], self._host_ptrs[2],
+                                          self._raw_stream()),
+                  "fpx_evaluate_host")
+            return self._outputs(self._host_chi2, self._host_resp, patch, unpack)
+        stream = self._stream()
+        self.launch(proposal, patch, stream=stream)
+        with torch.cuda.stream(stream):
+            self._out_h.copy_(self._out, non_blocking=True)
+        stream.synchronize()
+        self._check_status()
         return self._outputs(chi2_np, resp_np, patch, unpack)
 
     def _outputs(self, chi2_np, resp_np, patch, unpack):

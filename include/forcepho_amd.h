@@ -367,6 +367,17 @@ typedef struct FpxEvaluator FpxEvaluator;
 int fpx_bind(const FpxPatch *patch_dev, int32_t n_bands, int32_t cfg, int32_t persistent,
              FpxEvaluator **out);
 
+/* fpx_bind with the caller's tile table (host FpxTile[n_tiles], built for
+ * kernel configuration cfg) instead of a planned one: a balanced order, or
+ * split tiles (FpxTile parts >= 2, which need persistent = 1; the evaluator
+ * then owns the split buffer).  The table is checked against the patch's
+ * exposures (each entry inside one exposure of its band, at most the
+ * configuration's pixels per tile, every split tile's pieces in a row) and
+ * copied; that it covers each pixel once is the caller's to ensure.  No
+ * counterpart in the reference. */
+int fpx_bind_tiles(const FpxPatch *patch_dev, int32_t n_bands, int32_t cfg, int32_t persistent,
+                   const FpxTile *tiles_host, int32_t n_tiles, FpxEvaluator **out);
+
 /* chi2 and dchi2/dparam of one proposal on a bound patch, asynchronous on
  * stream.  Replaces the kernel EvaluateProposal(patch, proposal, pchi2,
  * pdchi2_dp) (forcepho/src/compute_gaussians_kernel.cu:212-214, launched

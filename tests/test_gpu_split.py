@@ -190,6 +190,12 @@ def test_few_tile_patch_is_split_and_matches():
     split = prop.evaluate_proposal(pdev, patch=patch)
     assert prop._launch_flags == 3
     _check_same(split, whole, patch)
+    # forcepho's call with a numpy proposal takes the split table too, through
+    # the evaluator bound to it (fpx_bind_tiles)
+    host = prop.evaluate_proposal(proposal, patch=patch)
+    bound = patch.bound_evaluator(prop.config, "split")
+    assert bound is not None and bound[1] == table[1]
+    _check_same(host, split, patch)
     mod = oracle.reference_module()
     if mod is None:
         pytest.skip("the reference CPU kernel is not built (oracle/_ref)")
@@ -197,3 +203,64 @@ def test_few_tile_patch_is_split_and_matches():
     assert np.max(np.abs(split[0] - c_ref) / np.abs(c_ref)) <= U.CHI2_RTOL
     assert U.bandmax_rel(split[1], d_ref).max() <= U.GRAD_BANDMAX_RTOL
 
+
+def test_bind_tiles_checks_and_matches_bind():
+    """fpx_bind_tiles through raw ctypes: a split table bound to a persistent
+    evaluator gives fpx_bind's results; tables that break the rules are
+    refused with FPX_E_ARG (a piece cut off, a piece out of order, an
+    exposure not of the tile's band, too many pixels, split tiles on a
+    non-persistent evaluator)."""
+    import ctypes
+    from forcepho_amd import tile_split
+    from forcepho_amd._lib import lib, TILE_DTYPE
+    from forcepho_amd.proposal import response_struct_dtype
+    print("[gpu] fpx_bind_tiles", flush=True)
+    patch, proposal, expected, prop = _setup("multi_small_s2", 5, return_residual=False)
+    L = lib()
+    tiles = patch.tile_table_host(prop.pix_per_tile)
+    parts = np.array([c for c, _ in zip(itertools.cycle(PARTS_CYCLE), range(len(tiles)))])
+    table = np.ascontiguousarray(tile_split.split_table(tiles, np.ones(len(tiles)), parts, 0.0))
+    nb, ns = int(patch.n_bands), int(patch.n_sources)
+    dev = patch.device_patch.data_ptr()
+
+    def bind(t, persistent=1):
+        ev = ctypes.c_void_p()
+        rc = L.fpx_bind_tiles(dev, nb, prop.config, persistent, t.ctypes.data, len(t),
+                              ctypes.byref(ev))
+        return rc, ev
+
+    def run(ev):
+        chi2 = np.zeros(nb)
+        resp = np.zeros(nb, dtype=response_struct_dtype)
+        src = np.ascontiguousarray(proposal)
+        assert L.fpx_evaluate_host(ev, src.ctypes.data, ns, chi2.ctypes.data, resp.ctypes.data,
+                                   None) == 0, L.fpx_last_error()
+        d = resp["dchi2_dparam"].reshape(nb, -1, 7)[:, :ns].astype(np.float64)
+        return chi2, d
+
+    ev_plain = ctypes.c_void_p()
+    assert L.fpx_bind(dev, nb, prop.config, 0, ctypes.byref(ev_plain)) == 0
+    rc, ev_split = bind(table)
+    assert rc == 0, L.fpx_last_error()
+    try:
+        for _ in range(2):
+            _check_same(run(ev_split), run(ev_plain), patch)
+    finally:
+        L.fpx_release(ev_split)
+        L.fpx_release(ev_plain)
+
+    first = int(np.argmax(table["parts"] > 1))
+    k = int(table["parts"][first])
+    bad = {"a piece cut off": table[:first + k - 1],
+           "a piece out of order": table.copy(),
+           "an exposure not of the band": table.copy(),
+           "too many pixels": table.copy()}
+    bad["a piece out of order"]["part"][first] = 1
+    bad["an exposure not of the band"]["band"][0] = (int(table["band"][0]) + 1) % nb if nb > 1 else 5
+    bad["too many pixels"]["pix_count"][0] = prop.pix_per_tile + 64
+    for why, t in bad.items():
+        rc, ev = bind(np.ascontiguousarray(t, dtype=TILE_DTYPE))
+        assert rc == -1 and not ev.value, why
+        assert b"tile table" in L.fpx_last_error(), why
+    rc, ev = bind(table, persistent=0)
+    assert rc == -1 and b"persistent" in L.fpx_last_error()
