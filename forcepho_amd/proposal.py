@@ -451,10 +451,13 @@ class HIPProposer(ProposerBase):
                     bound = patch.bound_evaluator(self.config)
                 else:
                     bound = patch.bound_evaluator(self.config, "plain", self.persistent)
-            self._host_prop[:n_sources] = proposal
-            check(lib().fpx_evaluate_host(bound[0], self._host_ptrs[0], n_sources,
-                                          self._host_ptrs[1], self._host_ptrs[2],
-                                          self._raw_stream()),
+            if proposal.dtype == source_struct_dtype and proposal.flags.c_contiguous:
+                src = proposal.ctypes.data    # read in place (the call copies it)
+            else:
+                self._host_prop[:n_sources] = proposal
+                src = self._host_ptrs[0]
+            check(lib().fpx_evaluate_host(bound[0], src, n_sources, self._host_ptrs[1],
+                                          self._host_ptrs[2], self._raw_stream()),
                   "fpx_evaluate_host")
             return self._outputs(self._host_chi2, self._host_resp, patch, unpack)
         stream = self._stream()
