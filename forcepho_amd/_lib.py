@@ -74,6 +74,8 @@ _SIGNATURES = {
     "fpx_bind": (_c_int, [_p, _c_int, _c_int, _c_int, _p]),
     "fpx_bind_tiles": (_c_int, [_p, _c_int, _c_int, _c_int, _p, _c_int, _p]),
     "fpx_evaluate": (_c_int, [_p, _p, _p, _p, _p]),
+    "fpx_evaluate_vjp": (_c_int, [_p, _p, _p, _p, _p, _p]),
+    "fpx_evaluate_model": (_c_int, [_p, _p, _p, _p, _p]),
     "fpx_evaluate_host": (_c_int, [_p, _p, _c_int, _p, _p, _p]),
     "fpx_evaluator_status": (_c_int, [_p, _p]),
     "fpx_evaluator_info": (_c_int, [_p, _p, _p, _p]),

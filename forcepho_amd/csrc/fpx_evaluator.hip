@@ -410,6 +410,30 @@ int fpx_evaluate(FpxEvaluator *ev, const FpxSource *proposal_dev, double *pchi2_
                          ev->max_gauss, ev->cfg, ev->launch_flags, ev->queue, stream);
 }
 
+int fpx_evaluate_vjp(FpxEvaluator *ev, const FpxSource *proposal_dev, const float *pixel_weight_dev,
+                     double *pchi2_dev, FpxResponse *pdchi2_dp_dev, void *stream) {
+  if (!ev || !proposal_dev || !pixel_weight_dev || !pchi2_dev || !pdchi2_dp_dev)
+    return arg_err("NULL evaluator, proposal, pixel weights or output");
+  const int rc = check_device(ev);
+  if (rc) return rc;
+  FpxSlot slot = slot_of(ev, proposal_dev, pchi2_dev, pdchi2_dp_dev);
+  slot.pixel_weight = pixel_weight_dev;
+  return fpx::launch_one(slot, ev->n_tiles, ev->max_gauss, ev->cfg, ev->launch_flags, ev->queue,
+                         stream);
+}
+
+int fpx_evaluate_model(FpxEvaluator *ev, const FpxSource *proposal_dev, double *pchi2_dev,
+                       FpxResponse *pdchi2_dp_dev, void *stream) {
+  if (!ev || !proposal_dev || !pchi2_dev || !pdchi2_dp_dev)
+    return arg_err("NULL evaluator, proposal or output");
+  const int rc = check_device(ev);
+  if (rc) return rc;
+  FpxSlot slot = slot_of(ev, proposal_dev, pchi2_dev, pdchi2_dp_dev);
+  slot.flags = FPX_SLOT_MODEL_ONLY;
+  return fpx::launch_one(slot, ev->n_tiles, ev->max_gauss, ev->cfg, ev->launch_flags, ev->queue,
+                         stream);
+}
+
 int fpx_evaluate_host(FpxEvaluator *ev, const FpxSource *proposal_host, int32_t n_sources,
                       double *chi2_host, FpxResponse *resp_host, void *stream) {
   if (!ev || !proposal_host || !chi2_host) return arg_err("NULL evaluator, proposal or output");

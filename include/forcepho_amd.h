@@ -393,6 +393,23 @@ int fpx_bind_tiles(const FpxPatch *patch_dev, int32_t n_bands, int32_t cfg, int3
 int fpx_evaluate(FpxEvaluator *ev, const FpxSource *proposal_dev, double *pchi2_dev,
                  FpxResponse *pdchi2_dp_dev, void *stream);
 
+/* The vector-Jacobian product of the model image on a bound patch, the
+ * autograd backward: pdchi2_dp_dev[b] receives sum_pix pixel_weight[pix] *
+ * d model[pix] / d param over band b's pixels (pixel_weight_dev: device
+ * float[npix] over the packed pixels), pchi2_dev[b] is written 0; the
+ * gradient pass alone, the Gaussians recomputed from the proposal.
+ * Asynchronous, as fpx_evaluate.  No counterpart in the reference, whose
+ * kernel always weights by the residual. */
+int fpx_evaluate_vjp(FpxEvaluator *ev, const FpxSource *proposal_dev, const float *pixel_weight_dev,
+                     double *pchi2_dev, FpxResponse *pdchi2_dp_dev, void *stream);
+
+/* The model pass alone on a bound patch: the residual image (written when
+ * the patch struct's residual is not NULL) and pchi2_dev; pdchi2_dp_dev is
+ * written 0.  Asynchronous, as fpx_evaluate.  For forward passes that need
+ * the residual and not the gradient; no counterpart in the reference. */
+int fpx_evaluate_model(FpxEvaluator *ev, const FpxSource *proposal_dev, double *pchi2_dev,
+                       FpxResponse *pdchi2_dp_dev, void *stream);
+
 /* The same evaluation with host buffers, synchronous: n_sources records of
  * proposal_host in, chi2_host[n_bands] and resp_host[n_bands] out (resp_host
  * may be NULL), the status checked and cleared (FPX_E_ARG if a tile was left

@@ -302,3 +302,69 @@ def test_bound_patch_small_path_reports_device_radius_bin():
         assert np.max(np.abs(good - expected["chi2"]) / np.abs(expected["chi2"])) <= U.CHI2_RTOL
     finally:
         L.fpx_release(ev)
+
+
+@pytest.mark.parametrize("name", ["multi_small_s2", "padding_s5"])
+def test_bound_patch_vjp_and_model_only(name):
+    """The bound patch's vector-Jacobian entry (fpx_evaluate_vjp) against the
+    float64 oracle's sum_pix w dmodel/dtheta for the same pixel weights, with
+    chi2 written 0; its model-only entry (fpx_evaluate_model) gives
+    fpx_evaluate's chi2 and residual image, with a zero gradient."""
+    import oracle.fp64 as fp64
+    print(f"[gpu] bound-patch vjp and model-only entries on {name}", flush=True)
+    patch, proposal, expected = U.load_golden(name, return_residual=True)
+    L = _lib.lib()
+    dev = RawDevice()
+    ev = ctypes.c_void_p()
+    try:
+        d_patch, ptrs = _upload_patch(dev, patch)
+        assert L.fpx_bind(d_patch, int(patch.n_bands), -1, 1, ctypes.byref(ev)) == 0
+        nb, ns = int(patch.n_bands), int(patch.n_sources)
+        prop = np.ascontiguousarray(proposal, dtype=source_struct_dtype)
+        d_prop = dev.malloc(prop.nbytes)
+        dev.h2d(d_prop, prop)
+        d_chi2 = dev.malloc(8 * nb)
+        d_resp = dev.malloc(response_struct_dtype.itemsize * nb)
+
+        def outputs():
+            c = np.zeros(nb)
+            dev.d2h(c, d_chi2)
+            r = np.zeros(nb, dtype=response_struct_dtype)
+            dev.d2h(r, d_resp)
+            return c, r["dchi2_dparam"].reshape(nb, MAXSOURCES, NPARAMS)[:, :ns].astype(np.float64)
+
+        def residual():
+            r = np.zeros(patch.npix, dtype=np.float32)
+            dev.d2h(r, ptrs["residual"])
+            return r
+
+        assert L.fpx_evaluate(ev, d_prop, d_chi2, d_resp, None) == 0, L.fpx_last_error()
+        assert L.fpx_evaluator_status(ev, None) == 0
+        c_full, _ = outputs()
+        r_full = residual()
+        # model-only: the same chi2 and residual, no gradient
+        dev.h2d(ptrs["residual"], np.zeros(patch.npix, dtype=np.float32))
+        assert L.fpx_evaluate_model(ev, d_prop, d_chi2, d_resp, None) == 0, L.fpx_last_error()
+        assert L.fpx_evaluator_status(ev, None) == 0
+        c_model, g_model = outputs()
+        np.testing.assert_allclose(c_model, c_full, rtol=1e-12)
+        np.testing.assert_array_equal(residual(), r_full)
+        assert np.all(g_model == 0)
+        # vector-Jacobian product against the float64 oracle
+        w = np.linspace(-1.0, 1.0, patch.npix).astype(np.float32)
+        d_w = dev.malloc(w.nbytes)
+        dev.h2d(d_w, w)
+        assert L.fpx_evaluate_vjp(ev, d_prop, d_w, d_chi2, d_resp, None) == 0, L.fpx_last_error()
+        assert L.fpx_evaluator_status(ev, None) == 0
+        c_vjp, g_vjp = outputs()
+        assert np.all(c_vjp == 0)
+        _, g64, _, mass = fp64.evaluate(patch, proposal, pixel_weight=w.astype(np.float64),
+                                        mass=True)
+        tol = U.GRAD_RTOL * np.abs(g64) + U.GRAD_FLOOR * mass
+        assert np.all(np.abs(g_vjp - g64) <= tol), np.max(np.abs(g_vjp - g64) / np.maximum(tol, 1e-300))
+        assert (L.fpx_evaluate_vjp(ev, d_prop, None, d_chi2, d_resp, None) == -1
+                and b"pixel weights" in L.fpx_last_error())
+    finally:
+        if ev.value:
+            L.fpx_release(ev)
+        dev.free()
