@@ -264,3 +264,56 @@ def test_bind_tiles_checks_and_matches_bind():
         assert b"tile table" in L.fpx_last_error(), why
     rc, ev = bind(table, persistent=0)
     assert rc == -1 and b"persistent" in L.fpx_last_error()
+
+
+def test_split_tiles_in_recorded_graphs():
+    """torch.autograd's recorded graphs (chi2 with gradient, the residual
+    image, the backward pass) over a split table: each replay, with eager
+    launches of the same proposer between replays, gives the eager results,
+    and the pieces' counters are zero after every replay (the kernel resets
+    them; a graph records no memset)."""
+    import torch
+    from forcepho_amd import scenes
+    from forcepho_amd.autograd import PatchModel
+    from forcepho_amd.patches import SimplePatch
+    print("[gpu] split tiles in recorded graphs", flush=True)
+    spec = scenes.spec_multi(2, side=64, n_bands=2, n_exp=1, n_sources=15, n_psf_comp=4)
+    patch, scene, theta = scenes.build_patch(spec, SimplePatch, oracle.render_model,
+                                             return_residual=True)
+    patch.send_to_device()
+    model = PatchModel(patch, scene)
+    p = model.proposer
+    assert p.persistent
+    p.balance(model.proposal(theta), patch)
+    assert patch.split_tiles(p.pix_per_tile) is not None
+    nb = int(patch.n_bands)
+    rng = np.random.default_rng(11)
+    thetas = []
+    for _ in range(3):
+        t = theta.copy().reshape(-1, nb + 6)
+        t[:, :nb] *= 1 + 0.05 * rng.normal(size=(len(t), nb))
+        t[:, nb + 2] = np.clip(t[:, nb + 2] * 0.95, 0.2, 0.99)
+        thetas.append(t.reshape(-1))
+    weight = torch.tensor(rng.normal(size=patch.npix), dtype=torch.float32, device="cuda")
+
+    def run(th):
+        t = torch.tensor(th, dtype=torch.float64, device="cuda")
+        v, g = model.chi2_and_grad(t)
+        r = model.residual_device(t)
+        w = model.vjp_theta(t, weight)
+        return v.item(), g.cpu().numpy(), r.cpu().numpy(), w.cpu().numpy()
+
+    model.use_graph = False
+    eager = [run(th) for th in thetas]
+    assert p._split_buf is not None and p._launch_flags == 3
+    model.use_graph = True
+    for _ in range(2):
+        for th, ref in zip(thetas, eager):
+            got = run(th)
+            model.forward_residual(model.proposal(thetas[0]))     # eager work between replays
+            np.testing.assert_allclose(got[0], ref[0], rtol=1e-12)
+            for x, y in ((got[1], ref[1]), (got[3], ref[3])):
+                np.testing.assert_allclose(x, y, rtol=1e-6, atol=1e-6 * np.abs(y).max())
+            np.testing.assert_array_equal(got[2], ref[2])
+            n_items = patch.split_tiles(p.pix_per_tile)[1]
+            assert p._split_buf[:4 * n_items].cpu().numpy().view(np.uint32).max() == 0
