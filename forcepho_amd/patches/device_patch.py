@@ -355,18 +355,8 @@ class HIPPatchMixin(DevicePatchMixin):
         if tiles is None:
             tables.pop(pix_per_tile, None)
             return None
-        tiles = np.ascontiguousarray(tiles, dtype=TILE_DTYPE)
-        parts = np.maximum(tiles["parts"].astype(np.int64), 1)
-        k = 0
-        while k < len(tiles):   # every split tile's pieces 0..parts-1 in a row
-            p = int(parts[k])
-            run = tiles[k:k + p]
-            if (len(run) != p or not np.array_equal(run["part"], np.arange(p))
-                    or np.any(run["pix_start"] != run["pix_start"][0])
-                    or np.any(run["parts"] != tiles["parts"][k])):
-                raise ValueError(f"tile table entry {k}: pieces of a split tile must follow "
-                                 "each other with part 0..parts-1")
-            k += p
+        from ..tile_split import check_table
+        tiles = check_table(tiles)
         tables[pix_per_tile] = (device_bytes(tiles, self._dev()), len(tiles), tiles.copy())
         return tables[pix_per_tile][:2]
 

@@ -22,7 +22,7 @@ import numpy as np
 
 from ._lib import TILE_DTYPE, MAX_TILE_PARTS
 
-__all__ = ["list_schedule", "plan_parts", "split_table"]
+__all__ = ["list_schedule", "plan_parts", "split_table", "check_table"]
 
 
 def list_schedule(costs, groups):
@@ -91,3 +91,26 @@ def split_table(tiles, cost, parts, piece_overhead):
     out["part"] = np.where(np.repeat(reps, reps) > 1, np.arange(len(out)) - first, 0)
     out["parts"] = np.where(np.repeat(reps, reps) > 1, np.repeat(reps, reps), 0)
     return out
+
+
+def check_table(tiles):
+    """The table as a contiguous TILE_DTYPE array, checked the way the split
+    kernel variant reads it: every split tile (parts >= 2, at most
+    MAX_TILE_PARTS) appears as parts entries in a row, with part 0..parts-1
+    and the same exposure, band and pixels; whole tiles have part 0.
+    Raises ValueError otherwise."""
+    tiles = np.ascontiguousarray(tiles, dtype=TILE_DTYPE)
+    parts = np.maximum(tiles["parts"].astype(np.int64), 1)
+    if np.any(parts > MAX_TILE_PARTS):
+        raise ValueError(f"a split tile has more than {MAX_TILE_PARTS} pieces")
+    k = 0
+    while k < len(tiles):
+        p = int(parts[k])
+        run = tiles[k:k + p]
+        same = all(np.all(run[f] == run[f][0]) for f in ("exposure", "band", "pix_start",
+                                                           "pix_count", "parts"))
+        if len(run) != p or not np.array_equal(run["part"], np.arange(p)) or not same:
+            raise ValueError(f"tile table entry {k}: the pieces of a split tile must follow "
+                             "each other with part 0..parts-1 and the same pixels")
+        k += p
+    return tiles

@@ -103,3 +103,27 @@ def test_split_table_rejects_bad_parts():
         tile_split.split_table(tiles, np.ones(3), [1, 0, 1], 0)
     with pytest.raises(ValueError):
         tile_split.split_table(tiles, np.ones(3), [1, MAX_TILE_PARTS + 1, 1], 0)
+
+
+def test_check_table_accepts_planned_tables_and_refuses_broken_runs():
+    tiles = _tiles(6)
+    out = tile_split.split_table(tiles, np.arange(6.0), [1, 3, 1, 2, 1, 1], 0.0)
+    tile_split.check_table(out)
+    tile_split.check_table(tiles)                       # whole tiles only
+    first = int(np.argmax(out["parts"] > 1))
+    broken = {
+        "a piece missing": out[np.arange(len(out)) != first + 1],
+        "pieces out of order": out.copy(),
+        "a piece of other pixels": out.copy(),
+        "a whole tile with a piece index": out.copy(),
+        "too many pieces": out.copy(),
+    }
+    broken["pieces out of order"]["part"][first] = 1
+    broken["pieces out of order"]["part"][first + 1] = 0
+    broken["a piece of other pixels"]["pix_start"][first + 1] += 256
+    whole = int(np.argmax(out["parts"] == 0))
+    broken["a whole tile with a piece index"]["part"][whole] = 1
+    broken["too many pieces"]["parts"][first:first + 3] = MAX_TILE_PARTS + 1
+    for why, t in broken.items():
+        with pytest.raises(ValueError, match="split tile"):
+            tile_split.check_table(t)
