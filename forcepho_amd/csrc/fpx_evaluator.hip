@@ -144,20 +144,22 @@ int check_tiles(const FpxTile *t, int32_t n, int ppt, int n_bands, int n_exp,
     if (x.pix_start < exp_start[x.exposure] ||
         (long)x.pix_start + x.pix_count > (long)exp_start[x.exposure] + exp_N[x.exposure])
       return arg_err("tile table: pixels outside the tile's exposure");
+  }
+  // runs: a whole tile has part 0; a split tile's pieces 0..parts-1 follow
+  // each other with the same exposure, band, pixels and piece count
+  for (int32_t k = 0; k < n;) {
+    const FpxTile &x = t[k];
     const int parts = x.parts > 1 ? x.parts : 1;
     if (parts > FPX_MAX_TILE_PARTS) return arg_err("tile table: more than FPX_MAX_TILE_PARTS pieces");
-    if (x.part >= parts) return arg_err("tile table: piece index outside [0, parts)");
-    const int part = x.part;
-    if (parts > 1) {
-      *split = true;
-      const int32_t first = k - part;
-      if (first < 0 || first + parts > n) return arg_err("tile table: a split tile's pieces are cut off");
-      const FpxTile &p0 = t[first];
-      if (p0.parts != x.parts || p0.exposure != x.exposure || p0.band != x.band ||
-          p0.pix_start != x.pix_start || p0.pix_count != x.pix_count ||
-          t[k].part != (uint8_t)(k - first))
+    if (k + parts > n) return arg_err("tile table: a split tile's pieces are cut off");
+    for (int q = 0; q < parts; ++q) {
+      const FpxTile &y = t[k + q];
+      if (y.part != q || y.parts != x.parts || y.exposure != x.exposure || y.band != x.band ||
+          y.pix_start != x.pix_start || y.pix_count != x.pix_count)
         return arg_err("tile table: pieces of a split tile must follow each other, part 0..parts-1");
     }
+    *split = *split || parts > 1;
+    k += parts;
   }
   return FPX_OK;
 }

@@ -253,9 +253,11 @@ def test_bind_tiles_checks_and_matches_bind():
     k = int(table["parts"][first])
     bad = {"a piece cut off": table[:first + k - 1],
            "a piece out of order": table.copy(),
+           "piece 0 twice": table.copy(),
            "an exposure not of the band": table.copy(),
            "too many pixels": table.copy()}
     bad["a piece out of order"]["part"][first] = 1
+    bad["piece 0 twice"]["part"][first + 1] = 0
     bad["an exposure not of the band"]["band"][0] = (int(table["band"][0]) + 1) % nb if nb > 1 else 5
     bad["too many pixels"]["pix_count"][0] = prop.pix_per_tile + 64
     for why, t in bad.items():
