@@ -268,48 +268,88 @@ __device__ __forceinline__ float bin_covar(const Lds &l, const FpxPSFSourceGauss
 }
 
 // Cull the n_src x n_psf Gaussians of the exposure against the tile's box:
-// phase 1 flags every Gaussian (one lane each), phase 2 compacts the flags
-// into the list of survivors in (source, component) order with one
-// wavefront (ballot + prefix count).  Returns the number of survivors.  A
-// PSF entry whose radius bin lies outside [0, n_radii) never survives and
-// sets FPX_STATUS_RADIUS_BIN in the slot's status word.  psf0 is the PSF
-// entry of Gaussian threadIdx.x (when below n), loaded by the caller ahead of
-// time.
+// every thread flags one Gaussian per round of W * 64, then the flags are
+// compacted into the list of survivors in (source, component) order.
+// Returns the number of survivors.  A PSF entry whose radius bin lies
+// outside [0, n_radii) never survives and sets FPX_STATUS_RADIUS_BIN in the
+// slot's status word.  psf0 is the PSF entry of Gaussian threadIdx.x (when
+// below n), loaded by the caller ahead of time.
+//
+// Compaction: workgroups of 8 or more wavefronts keep their flags in
+// registers (one bit per round; n <= MAXSOURCES * FPX_MAX_PSF_PER_SOURCE =
+// 3600 gives at most 64 rounds x wavefronts) and write the list in parallel
+// from per-wavefront ballot counts and a prefix over them; narrower
+// workgroups, with fewer wavefronts to share the work and more rounds, flag
+// into LDS and compact with wavefront 0 alone, which measured faster there.
 template <int W>
 __device__ int cull_tile(const FpxPSFSourceGaussian *psfg, int pstart, int n_src, int n_psf,
                          int n_radii, const Box &box, const Lds &l, int wave, int lane,
                          uint32_t *status, const FpxPSFSourceGaussian &psf0) {
+  constexpr int kThreads = W * 64;
+  constexpr bool kParallel = W >= 8;
   const int n = n_src * n_psf;
+  const int t = threadIdx.x;
   uint16_t *index = l.index;
-  auto flag = [&](int lg, const FpxPSFSourceGaussian &psf) {
+  auto reaches = [&](int lg, const FpxPSFSourceGaussian &psf) {
     const int s = lg / n_psf;
     const bool bin_ok = psf.sersic_radius_bin >= 0 && psf.sersic_radius_bin < n_radii;
     if (!bin_ok && status != nullptr) atomicOr(status, (uint32_t)FPX_STATUS_RADIUS_BIN);
-    index[lg] = bin_ok && gauss_reaches(load_geometry(l.geo + s * kGeoFloats), psf,
-                                        bin_covar(l, psf), box.x0, box.x1, box.y0, box.y1)
-                    ? 1 : 0;
+    return bin_ok && gauss_reaches(load_geometry(l.geo + s * kGeoFloats), psf, bin_covar(l, psf),
+                                   box.x0, box.x1, box.y0, box.y1);
   };
-  __syncthreads();  // previous users of the index are done; geometry is in LDS
-  if ((int)threadIdx.x < n) flag(threadIdx.x, psf0);
-  for (int lg = threadIdx.x + W * 64; lg < n; lg += W * 64) {
-    const int s = lg / n_psf;
-    flag(lg, load_global(psfg, pstart + lg - s * n_psf));
-  }
-  __syncthreads();
-  if (wave == 0) {
-    int count = 0;
-    for (int b0 = 0; b0 < n; b0 += 64) {
-      const int lg = b0 + lane;
-      const bool keep = lg < n && index[lg] != 0;
-      const unsigned long long m = __builtin_amdgcn_ballot_w64(keep);
-      // every lane has read its flag above; writes land at or below b0 + lane
-      if (keep) index[count + lanes_below(m)] = lg;
-      count += __builtin_popcountll(m);
+  __syncthreads();  // previous users of the index and l.part are done; geometry is in LDS
+  if constexpr (kParallel) {
+    static_assert(((MAXSOURCES * FPX_MAX_PSF_PER_SOURCE + kThreads - 1) / kThreads) * W <= 64,
+                  "one flag bit per round and one count per (round, wavefront) lane");
+    int *counts = reinterpret_cast<int *>(l.part);  // [round][wave]
+    const int rounds = (n + kThreads - 1) / kThreads;
+    uint64_t keep = 0;
+    for (int k = 0; k < rounds; ++k) {
+      const int lg = k * kThreads + t;
+      bool in = false;
+      if (lg < n) in = reaches(lg, k == 0 ? psf0 : load_global(psfg, pstart + lg - (lg / n_psf) * n_psf));
+      keep |= (in ? 1ull : 0ull) << k;
+      const unsigned long long m = __builtin_amdgcn_ballot_w64(in);
+      if (lane == 0) counts[k * W + wave] = __builtin_popcountll(m);
     }
-    if (lane == 0) index[n] = count;
+    __syncthreads();
+    const int n_blocks = rounds * W;
+    const int c = lane < n_blocks ? counts[lane] : 0;
+    int incl = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += v;
+    }
+    const int total = __shfl(incl, 63, 64);
+    const int excl = incl - c;
+    for (int k = 0; k < rounds; ++k) {
+      const bool in = ((keep >> k) & 1ull) != 0;
+      const unsigned long long m = __builtin_amdgcn_ballot_w64(in);
+      const int base = __shfl(excl, k * W + wave, 64);
+      if (in) index[base + lanes_below(m)] = (uint16_t)(k * kThreads + t);
+    }
+    return __builtin_amdgcn_readfirstlane(total);
+  } else {
+    if (t < n) index[t] = reaches(t, psf0) ? 1 : 0;
+    for (int lg = t + kThreads; lg < n; lg += kThreads)
+      index[lg] = reaches(lg, load_global(psfg, pstart + lg - (lg / n_psf) * n_psf)) ? 1 : 0;
+    __syncthreads();
+    if (wave == 0) {
+      int count = 0;
+      for (int b0 = 0; b0 < n; b0 += 64) {
+        const int lg = b0 + lane;
+        const bool keep = lg < n && index[lg] != 0;
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(keep);
+        // every lane has read its flag above; writes land at or below b0 + lane
+        if (keep) index[count + lanes_below(m)] = lg;
+        count += __builtin_popcountll(m);
+      }
+      if (lane == 0) index[n] = count;
+    }
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(l.index[n]);
   }
-  __syncthreads();
-  return __builtin_amdgcn_readfirstlane(l.index[n]);
 }
 
 // Build the records of survivors [first, first + count) into LDS.
