@@ -1,6 +1,8 @@
 """Parity of the HIP kernel with the reference CPU kernel (golden fixtures),
 the C restatement and the float64 oracle.  Calls go through the C ABI."""
 
+import functools
+
 import numpy as np
 import pytest
 
@@ -183,6 +185,43 @@ def test_thirty_bands():
     assert np.max(np.abs(chi2 - c32) / np.abs(c32)) <= U.CHI2_RTOL
     c64, g64, mass, smass = U.fp64_reference(patch, proposal)
     assert U.grad_errors(dchi2, g64, mass, smass).max() <= 1.0
+
+
+@functools.lru_cache(maxsize=1)
+def _max_gaussians_case():
+    """30 sources x 120 PSF entries (10 radii x 12 PSF components) in one
+    24x24 exposure: 3600 Gaussians, MAXSOURCES x FPX_MAX_PSF_PER_SOURCE."""
+    from forcepho_amd import scenes
+    from forcepho_amd.patches import SimplePatch
+    rng = np.random.default_rng(120)
+    radii = 0.005 * 2.0 ** np.arange(10)
+    spec = scenes.draw_patch_spec(rng, 1, [1], [24], 30, [12], radii=radii)
+    patch, scene, theta = scenes.build_patch(spec, SimplePatch, oracle.render_model)
+    proposal = scene.get_proposal()
+    proposal["q"] *= np.float32(0.95)
+    c32, _, _ = oracle.evaluate_cpu(patch, proposal)
+    return patch, proposal, c32, U.fp64_reference(patch, proposal)
+
+
+@pytest.mark.parametrize("config", range(U.N_CONFIGS))
+def test_most_gaussians_per_exposure(config):
+    """An exposure at the limit of Gaussians the cull takes (30 sources x 120
+    PSF entries): several cull rounds per tile and the longest survivor list,
+    in every configuration, against the C restatement and the FP64 model.
+    Both the numpy path (a bound evaluator, slot by value) and a device
+    proposal (the proposer's slot table)."""
+    import torch
+    from forcepho_amd.proposal import HIPProposer
+    patch, proposal, c32, (c64, g64, mass, smass) = _max_gaussians_case()
+    assert patch.max_gauss() == 30 * 120
+    print(f"[gpu] 3600 Gaussians, config {config}", flush=True)
+    patch.send_to_device()
+    prop = HIPProposer(patch=patch, config=config)
+    pdev = torch.from_numpy(np.ascontiguousarray(proposal).view(np.uint8).reshape(-1).copy()).cuda()
+    for p in (proposal, pdev):
+        chi2, dchi2 = prop.evaluate_proposal(p, patch=patch)
+        assert np.max(np.abs(chi2 - c32) / np.abs(c32)) <= U.CHI2_RTOL
+        assert U.grad_errors(dchi2, g64, mass, smass).max() <= 1.0
 
 
 def test_launch_without_tiles_writes_zeros():
