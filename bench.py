@@ -674,7 +674,8 @@ def split_measurement(args, steps=200):
 
 
 def parity_measurement():
-    """Parity of the multi-band workload at the north-star bar, once and
+    """Parity of the multi-band workload against its bar (chi2 1e-6 relative,
+    gradients 1e-4 of each band's largest, against the reference CPU kernel), once and
     untimed: the patch of tests/golden/make_fullsize.py (the benchmark's
     scene with data rendered on the host, every q scaled by 0.97) evaluated
     by the HIP kernel, by the reference's CPU kernel (oracle/_ref, the
