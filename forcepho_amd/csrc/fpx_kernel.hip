@@ -13,8 +13,8 @@
 //             Gaussian can reach the tile's pixel box and the survivors are
 //             compacted in order; their ImageGaussian records (fpx_gauss.h)
 //             are built into LDS, in chunks when they exceed the LDS budget.
-//   pass 1    model = sum_g amp_g G_g H_g over the wave's Gaussians (runs of
-//             8 records dealt round-robin to the wavefronts)
+//   pass 1    model = sum_g amp_g G_g H_g over the wave's Gaussians (each
+//             wavefront an equal contiguous range of the records)
 //             (common_kernel.cc:57-83), partial models combined through LDS,
 //             residual r = d - model, chi2 += (r ierr - d ierr)(r ierr + d ierr)
 //             in FP64 (compute_gaussians_kernel.cc:286-293), weight
@@ -416,15 +416,13 @@ __device__ __forceinline__ long long centre_key(const float4 r0) {
 // no record has this centre (two NaN halves)
 constexpr long long kNoCentre = (long long)0x7fc000017fc00001ull;
 
-// A wavefront's share of a chunk's n records: runs of kRun consecutive
-// records dealt round-robin over the W wavefronts (run r to wavefront r mod W),
-// which spreads costly and cheap Gaussians evenly while keeping a wave's
-// records mostly within one source (few per-source flushes).
-constexpr int kRun = 8;
-// Records per run for a chunk of n records: kRun, or fewer when the chunk
-// holds fewer than kRun per wavefront, so that every wavefront gets a share
-// of a small chunk instead of a few wavefronts taking all of it.
-__device__ __forceinline__ int run_length(int n, int W) { return max(1, min(kRun, (n + W - 1) / W)); }
+// A wavefront's share of a chunk's n records: the contiguous range
+// [n w / W, n (w + 1) / W) of wavefront w.  Equal counts (to one record)
+// balance the wavefronts better than dealing runs of records round-robin,
+// whose last round leaves some wavefronts a run more than others, and a
+// contiguous range holds few sources and keeps runs of one centre together
+// (fewer per-source flushes and offset computations).
+__device__ __forceinline__ int share_begin(int n, int W, int wave) { return (n * wave) / W; }
 
 // Pass 1: add this wave's share of records [0, n) to the lane's model.
 template <int PPL>
@@ -434,9 +432,8 @@ __device__ __forceinline__ void model_pass(const float *__restrict__ recs, int w
   const float4 *R4 = reinterpret_cast<const float4 *>(recs);
   float dx[PPL], dy[PPL], dxx[PPL], dyy[PPL], dxy[PPL];
   long long centre = kNoCentre;
-  const int run = run_length(n, W);
-  for (int r0 = wave * run; r0 < n; r0 += W * run)
-  for (int g = r0; g < min(r0 + run, n); ++g) {
+  const int g_end = share_begin(n, W, wave + 1);
+  for (int g = share_begin(n, W, wave); g < g_end; ++g) {
     const float4 r0 = R4[g * (kRec / 4) + 0];  // xc yc cxx cxy
     const float4 r1 = R4[g * (kRec / 4) + 1];  // cyy h0 sxx sxy
     const float4 r2 = R4[g * (kRec / 4) + 2];  // syy amp src dA/dflux
@@ -524,9 +521,8 @@ __device__ __forceinline__ void grad_pass(const float *__restrict__ recs, int wa
   int cur = -1;
   float dx[PPL], dy[PPL], dxx[PPL], dyy[PPL], dxy[PPL];
   long long centre = kNoCentre;
-  const int run = run_length(n, W);
-  for (int r0 = wave * run; r0 < n; r0 += W * run)
-  for (int g = r0; g < min(r0 + run, n); ++g) {
+  const int g_end = share_begin(n, W, wave + 1);
+  for (int g = share_begin(n, W, wave); g < g_end; ++g) {
     const float4 *rg = R4 + g * (kRec / 4);
     const float4 r0 = rg[0];  // xc yc cxx cxy
     const float4 r1 = rg[1];  // cyy h0 sxx sxy
