@@ -26,10 +26,12 @@
 //             Jacobian CW is applied once per source after the Gaussian
 //             loop), wave-reduced into per-wave LDS rows.
 //   combine   per-wave rows summed in fixed order, one FP64 global atomic per
-//             value into the slot's per-band accumulator.  A second kernel
-//             (finalize_kernel, same stream) writes pchi2 / Response from the
-//             accumulators and re-zeroes them: the kernel boundary orders it
-//             after every tile, with no per-tile fence or completion count.
+//             value into the slot's per-band accumulator.  pchi2 / Response
+//             are written from the accumulators, which are re-zeroed, by a
+//             second kernel (finalize_kernel, same stream: the kernel boundary
+//             orders it after every tile) or, in a launch of one slot, by the
+//             workgroup that finishes last (found with a device-scope done
+//             count, kModeOutputs).
 // A Gaussian is skipped at a pixel when dx^T F dx > MAX_EXP_ARG
 // (common_kernel.cc:73,108).  Two shortcuts skip work that this test would
 // discard anyway: the prologue builds records only for Gaussians whose
