@@ -219,13 +219,18 @@ __device__ __forceinline__ void image_gaussian_shape(const FpxPatch *__restrict_
 
 // Half-width of the box holding dx^T Sigma^-1 dx <= MAX_EXP_ARG along an
 // axis of variance S, with a relative 1e-3 and an absolute 0.01 px margin.
+// A NaN variance gives a NaN half-width (a comparison, not fmaxf, clamps
+// negative S: fmaxf would turn NaN into 0).
 __device__ __forceinline__ float cutoff_halfwidth(float S) {
-  return sqrtf((float)MAX_EXP_ARG) * sqrtf(fmaxf(S, 0.0f)) * 1.001f + 0.01f;
+  return sqrtf((float)MAX_EXP_ARG) * sqrtf(S < 0.0f ? 0.0f : S) * 1.001f + 0.01f;
 }
 
 // True when the MAX_EXP_ARG box of the convolution of the source (geometry
 // g) with PSF Gaussian psf (radius bin variance covar) can reach a pixel of
-// [x0, x1] x [y0, y1].
+// [x0, x1] x [y0, y1].  A box that cannot be decided (NaN from NaN source
+// parameters) counts as reaching: the reference's cutoff test `exparg >
+// MAX_EXP_ARG` is false for NaN (common_kernel.cc:73,108), so NaN parameters
+// make every pixel's model, and chi2, NaN there too.
 __device__ __forceinline__ bool gauss_reaches(const SourceGeometry &g,
                                               const FpxPSFSourceGaussian &psf, float covar,
                                               float x0, float x1, float y0, float y1) {
@@ -235,7 +240,7 @@ __device__ __forceinline__ bool gauss_reaches(const SourceGeometry &g,
   const float yc = g.ys + psf.ycen;
   const float gx = fmaxf(fmaxf(x0 - xc, xc - x1), 0.0f);
   const float gy = fmaxf(fmaxf(y0 - yc, yc - y1), 0.0f);
-  return gx <= cutoff_halfwidth(Sxx) && gy <= cutoff_halfwidth(Syy);
+  return !(gx > cutoff_halfwidth(Sxx)) && !(gy > cutoff_halfwidth(Syy));
 }
 
 // Build the record of source s (geometry g for the exposure, with dT/dq and

@@ -76,6 +76,25 @@ def test_batch_matches_checkers(batch_setup, config, persistent):
     _check(ev.results(), refs)
 
 
+def test_batch_nan_proposal_stays_in_its_slot(batch_setup):
+    """A NaN axis ratio in one slot's proposal makes that slot's chi2 NaN in
+    every band (the reference's behaviour) and leaves every other slot's
+    results as they are, in this launch and the next clean one."""
+    from forcepho_amd.batch import BatchEvaluator
+    pats, chains, proposals, refs = batch_setup
+    ev = BatchEvaluator([p for p, _ in pats], chains=chains, config=6, persistent=True)
+    bad = [p.copy() for p in proposals]
+    bad[2]["q"][0] = np.nan
+    ev.set_proposals(bad)
+    ev.launch()
+    res = ev.results()
+    assert np.all(np.isnan(res[2][0]))
+    _check(res[:2] + res[3:], refs[:2] + refs[3:])
+    ev.set_proposals(proposals)
+    ev.launch()
+    _check(ev.results(), refs)
+
+
 def test_batch_autotune_keeps_results(batch_setup):
     from forcepho_amd.batch import BatchEvaluator
     pats, chains, proposals, refs = batch_setup

@@ -224,6 +224,34 @@ def test_most_gaussians_per_exposure(config):
         assert U.grad_errors(dchi2, g64, mass, smass).max() <= 1.0
 
 
+@pytest.mark.parametrize("field", ["q", "pa", "ra", "dec"])
+def test_nan_geometry_gives_nan_chi2(field):
+    """A NaN in one source's position, axis ratio or angle gives NaN chi2 in
+    every band and non-finite gradients, as the reference CPU kernel does
+    (tests/test_oracle.py pins it): the cull must keep a Gaussian whose box
+    it cannot decide rather than drop the source.  A clean evaluation right
+    after still matches the reference (the accumulators were cleared)."""
+    import torch
+    from forcepho_amd.proposal import HIPProposer
+    patch, proposal, expected = U.load_golden("multi_small_s2", return_residual=False)
+    patch.send_to_device()
+    bad = proposal.copy()
+    bad[field][3] = np.nan
+    as_dev = lambda p: torch.from_numpy(
+        np.ascontiguousarray(p).view(np.uint8).reshape(-1).copy()).cuda()
+    for config in (5, 2, 8):
+        print(f"[gpu] NaN {field}, config {config}", flush=True)
+        prop = HIPProposer(patch=patch, config=config)
+        for arg in (bad, as_dev(bad)):
+            chi2, dchi2 = prop.evaluate_proposal(arg, patch=patch)
+            assert np.all(np.isnan(chi2)), (config, chi2)
+            assert not np.all(np.isfinite(dchi2))
+        for arg in (proposal, as_dev(proposal)):
+            chi2, dchi2 = prop.evaluate_proposal(arg, patch=patch)
+            rel = np.abs(chi2 - expected["chi2"]) / np.abs(expected["chi2"])
+            assert rel.max() <= U.CHI2_RTOL
+
+
 def test_launch_without_tiles_writes_zeros():
     """A launch with no tiles (no pixels) still writes the outputs: chi2 and
     gradients of an empty pixel set are zero."""

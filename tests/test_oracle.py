@@ -238,3 +238,22 @@ def test_reference_cpu_kernel_inside_band_at_full_size():
     c32, d32, _ = oracle.evaluate_cpu(patch, proposal)
     assert np.max(np.abs(c32 - ref["chi2"]) / np.abs(ref["chi2"])) <= 1e-6
     assert U.grad_errors(d32, ref["dchi2"], ref["mass"], ref["smass"]).max() <= 1.0
+
+
+@pytest.mark.parametrize("field", ["q", "pa", "ra", "dec"])
+def test_nan_geometry_gives_nan_chi2(field):
+    """A NaN in one source's position, axis ratio or angle makes the
+    reference CPU kernel's chi2 NaN in every band: its cutoff test
+    `exparg > MAX_EXP_ARG` (common_kernel.cc:73,108) is false for NaN, so the
+    NaN reaches every pixel's model.  (rh and sersic enter only through the
+    host's mixture amplitudes.)"""
+    patch, proposal, _ = U.load_golden("multi_small_s2", return_residual=False)
+    p = proposal.copy()
+    p[field][3] = np.nan
+    runs = [oracle.evaluate_cpu(patch, p)]
+    mod = oracle.reference_module()
+    if mod is not None:
+        runs.append(oracle.evaluate_reference(patch, p, module=mod))
+    for chi2, dchi2, _ in runs:
+        assert np.all(np.isnan(chi2))
+        assert not np.all(np.isfinite(dchi2))
