@@ -42,7 +42,10 @@ struct Basis {
 };
 
 __device__ void bspline_basis(const double *__restrict__ t, int len, int k, double x, Basis &out) {
-  x = fmin(fmax(x, t[k]), t[len - k - 1]);
+  // clamp into the knot range as fitpack does; comparisons rather than
+  // fmin/fmax, which would turn a NaN into the first knot (NaN stays NaN and
+  // gives NaN values, as scipy's splines do)
+  x = x < t[k] ? t[k] : (x > t[len - k - 1] ? t[len - k - 1] : x);
   // the knot interval: the largest l <= len - k - 2 with t[l] <= x, at least k
   int l = k;
   for (int j = k + 1; j <= len - k - 2; ++j)

@@ -79,6 +79,22 @@ def test_reference_table_splines(spline_type):
     _close(dr, dr0, 1e-11)
 
 
+@pytest.mark.parametrize("spline_type", ["rect", "smooth"])
+def test_nan_shape_gives_nan_amplitudes(spline_type):
+    """A NaN sersic index or half-light radius gives NaN amplitudes and
+    derivatives, as scipy's splines do (the reference's Galaxy.amplitudes,
+    forcepho/sources.py:655-681): the knot-range clamp must not turn NaN
+    into the first knot.  Finite points beyond the table stay clamped."""
+    model = verification_amplitudes(spline_type)
+    n = np.array([np.nan, 2.2, np.nan, 0.5, 2.2])
+    r = np.array([0.1, np.nan, np.nan, 0.1, 0.5])
+    A, dn, dr, _ = _device_amplitudes(model, n, r)
+    A0, dn0, dr0 = _scipy(model, n, r)
+    for got, ref in ((A, A0), (dn, dn0), (dr, dr0)):
+        assert np.all(np.isnan(got[:3])) and np.all(np.isnan(ref[:3]))
+        _close(got[3:], ref[3:], 1e-11)
+
+
 @pytest.mark.parametrize("kx,ky", [(2, 2), (2, 3), (3, 2), (5, 4)])
 def test_other_degrees_and_uneven_knots(kx, ky):
     """Degrees other than cubic and per-radius knots of different lengths
